@@ -1,0 +1,139 @@
+"""Benchmark: RT steps/s of the real-time Kohn-Sham propagation on one MI355X.
+
+Workload (default): 64-atom Si supercell (2x2x2 diamond cells, a = 10.26 bohr),
+32^3 grid, 128 complex128 orbitals x 8 k-points (2x2x2 Monkhorst-Pack), s+p
+Gaussian KB projectors, 4th-order Taylor, dt = 0.08 au, sin^2 A(t) pulse of
+amplitude 0.005 au.  One step = one full RT step (phase table, 4 H applications
+with the Taylor accumulation, density, current, dipole) on HBM-resident
+orbitals.  Synthetic seeded inputs.
+
+Prints ONE JSON line.  The roofline entry is for the dominant kernel, the fused
+stencil of the middle Taylor terms (k_hpsi, mode 2), timed live with start/stop
+events recorded by its own launches over the timed region.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RT steps/s: learned+mixed vs plain FP64 GPU at ≤1e-6 rel current error"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(sysm, steps: int):
+    """The oracle (C/OpenMP restatement of SALMON's CPU loops) on a bounded sample."""
+    from oracle import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    O = oracle.Oracle(sysm)
+    t0 = time.perf_counter()
+    O.rt_run(nt=steps)
+    dt = time.perf_counter() - t0
+    return {
+        "value": steps / dt,
+        "unit": "RT steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} RT steps of the same workload (oracle/salmon_oracle.c, OpenMP, {threads} threads)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the multi-GPU path is not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from salmon2_amd import system
+    from salmon2_amd.rt import RTPropagator
+
+    s = system.config(args.config)
+    prop = RTPropagator(s)
+    prop.setup_outputs(args.warmup + args.steps)
+    for _ in range(args.warmup):
+        prop.step()
+    torch.cuda.synchronize()
+    prop.h.prof_start(max_launches=12 * args.steps + 16)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        prop.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kern = {}
+    for k in ("hpsi_taylor1", "hpsi_taylor", "hpsi_taylor_last", "proj", "proj_current"):
+        ms, n = prop.h.prof_read(k)
+        kern[k] = {"avg_ms": ms / max(n, 1), "launches": n}
+    prop.h.prof_end()
+    res = prop.results()
+    assert np.all(np.isfinite(res["curr"])), "non-finite current"
+
+    elapsed = t1 - t0
+    steps_per_s = args.steps / elapsed
+    npts = s.nx * s.ny * s.nz
+    norb = s.norb
+    # algorithmic bytes of one middle-term launch: read H^(n-1) psi (16 B), write H^n psi (16 B),
+    # read+write the Taylor accumulator (32 B) per point-orbital, plus V_local once (8 B per point)
+    alg_bytes = 64.0 * npts * norb + 8.0 * npts
+    avg_s = kern["hpsi_taylor"]["avg_ms"] / 1e3
+    achieved = alg_bytes / avg_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "RT steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.config}: Si {s.natom} atoms, {s.nx}x{s.ny}x{s.nz} grid, {s.no} orbitals x {s.nk} k, "
+                        f"{s.nlma} KB projectors, Taylor order {s.n_hamil}, dt {s.dt}, plain FP64",
+            "form": "plain-fp64",
+            "grid_point_orbital_updates_per_s": steps_per_s * npts * norb,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_hpsi<2> (fused stencil + V + k.grad + KB scatter + Taylor accumulate)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "alg_bytes_per_launch": alg_bytes,
+            "avg_launch_ms": kern["hpsi_taylor"]["avg_ms"],
+        },
+        "kernels_ms": {k: round(v["avg_ms"], 5) for k, v in kern.items()},
+    }
+    prop = None
+    torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(s, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,166 @@
+/*
+ * salmon_hip.h — C ABI of the MI355X (gfx950) real-time Kohn-Sham propagation
+ * hot path.  Plain C types only (ints, doubles, raw pointers); callable from
+ * Fortran through iso_c_binding (salmon2_amd/fortran/salmon_hip_mod.f90), from
+ * C/C++, or from Python ctypes.
+ *
+ * Layout contract (identical to SALMON's, so the Fortran driver hands over its
+ * own arrays with plain memory copies):
+ *   orbitals  zwf(ix, iy, iz, ispin, io, ik): complex(8), x unit-stride, y
+ *             leading extent `ly` (ny or ny + 1 with USE_OPT_ARRAY_PADDING),
+ *             z extent nz; src/common/structures.f90:191-196,
+ *             src/common/initialization.f90:606-609.  The padding row is never
+ *             read and never written.
+ *   V_local   (ix, iy, iz, ispin) real(8), unpadded (structures.f90:539-553).
+ *   jxyz      (3, nps, natom) 1-based grid indices, rxyz (3, nps, natom),
+ *   uv        (nps, nlma), ia_tbl (nlma) 1-based, rinv_uvu (nlma)
+ *             (structures.f90:254-319).
+ * Orbital arguments of hpsi/taylor/density/current are DEVICE pointers to the
+ * whole block of nspin*no*nk orbitals; set-up arguments are HOST pointers.
+ * The periodic single-process case is the one supported: no halo, wrap-around
+ * by modulo (idx(j) = mod(j+n-1, n)+1, initialization.f90:574-587), orthogonal
+ * lattice, collinear spin, no DFT+U.
+ *
+ * Every function returns 0 on success or a SALMON_HIP_E* code; the message of
+ * the last failure on the calling thread is salmon_hip_last_error().  Work is
+ * enqueued on the handle's stream; results are visible after
+ * salmon_hip_synchronize() or to later work on the same stream.
+ */
+#ifndef SALMON_HIP_H
+#define SALMON_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SALMON_HIP_OK 0
+#define SALMON_HIP_EINVAL 1       /* invalid argument / shape */
+#define SALMON_HIP_EHIP 2         /* HIP runtime error */
+#define SALMON_HIP_ENOMEM 3       /* device allocation failed */
+#define SALMON_HIP_EUNSUPPORTED 4 /* configuration outside the supported path */
+#define SALMON_HIP_ESTATE 5       /* call order violated (e.g. hpsi before set_vlocal) */
+
+typedef struct salmon_hip_ctx *salmon_hip_t;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *salmon_hip_last_error(void);
+
+/*
+ * Create a handle for one grid and orbital set (replaces the device set-up the
+ * reference does with `!$acc enter data` in src/rt/initialization_rt.f90:234-240).
+ *   coef_lap, coef_nab: s_stencil%coef_lap(4,3), %coef_nab(4,3); coef_lap0: %coef_lap0
+ *   (src/common/initialization.f90:240-255).
+ *   vec_k (3, nk), rocc (no, nk, nspin), wtk (nk): s_dft_system%vec_k/rocc/wtk.
+ *   hgs (3), hvol: grid spacing and volume element (src/common/lattice.f90:40).
+ */
+int salmon_hip_create(int nx, int ny, int nz, int ly, int nspin, int no, int nk,
+                      const double *coef_lap, const double *coef_nab, double coef_lap0,
+                      const double *vec_k, const double *rocc, const double *wtk,
+                      const double *hgs, double hvol, salmon_hip_t *out);
+
+/* Release every device resource of the handle. */
+int salmon_hip_destroy(salmon_hip_t h);
+
+/* Enqueue subsequent work on `stream` (a hipStream_t; NULL = the handle's own stream). */
+int salmon_hip_set_stream(salmon_hip_t h, void *stream);
+
+/* The handle's current stream (a hipStream_t). */
+void *salmon_hip_get_stream(salmon_hip_t h);
+
+/*
+ * Kleinman-Bylander projector tables, s_pp_grid%{mps,jxyz,rxyz,ia_tbl,rinv_uvu,uv}
+ * (set up by init_ps, src/atom/pp/prep_pp.f90:22-527).  Projectors of one atom
+ * must be contiguous in ilma (as set_lma orders them).  natom = 0 disables the
+ * nonlocal term.
+ */
+int salmon_hip_set_nonlocal(salmon_hip_t h, int natom, int nps, const int *mps, const int *jxyz,
+                            const double *rxyz, int nlma, const int *ia_tbl,
+                            const double *rinv_uvu, const double *uv);
+
+/* V_local(ix,iy,iz,ispin), host array (update_vlocal, src/common/hamiltonian.f90:813-843). */
+int salmon_hip_set_vlocal(salmon_hip_t h, const double *vlocal);
+
+/* Same from a device array (e.g. a potential produced on the GPU). */
+int salmon_hip_set_vlocal_device(salmon_hip_t h, const double *vlocal_dev);
+
+/*
+ * Set system%vec_Ac and rebuild the phase table
+ * zekr_uV(j,ilma,ik) = conjg(exp(i (k+A).r_j)) uv(j,ilma) and the per-k stencil
+ * terms k_lap0 = lap0 + |k+A|^2/2, k_nabt(n,d) = (k+A)_d coef_nab(n,d):
+ * update_kvector_nonlocalpt (src/common/hamiltonian.f90:847-903) and
+ * src/common/hamiltonian.f90:146-155.
+ */
+int salmon_hip_update_kvector(salmon_hip_t h, const double *vec_Ac);
+
+/*
+ * htpsi = H tpsi for every orbital (hpsi, src/common/hamiltonian.f90:27-229,424-435:
+ * 25-point stencil + V_local + k+A gradient, then zpseudo,
+ * src/common/nonlocal_potential.f90:175-390).  Device pointers; tpsi != htpsi.
+ */
+int salmon_hip_hpsi(salmon_hip_t h, const void *tpsi, void *htpsi);
+
+/*
+ * Taylor propagation tspsi_out = sum_{n=0..n_hamil} zc(n) H^n tspsi_in with
+ * zc(0) = 1 (taylor, src/rt/taylor.f90:21-121).  zc: host complex(8)[n_hamil]
+ * as interleaved (re, im).  tspsi_in and sshtpsi are work arrays as in the
+ * reference: on return tspsi_in holds H^2 tspsi_in when n_hamil >= 3 (the
+ * reference leaves H^n_hamil there; the RT driver never reads it,
+ * src/rt/main_tddft.f90:91-99), sshtpsi holds an intermediate power.
+ */
+int salmon_hip_taylor(salmon_hip_t h, const double *zc, int n_hamil, void *tspsi_in,
+                      void *tspsi_out, void *sshtpsi);
+
+/*
+ * rho(ix,iy,iz,ispin) = sum_{ik,io} rocc*wtk*|psi|^2 into a DEVICE array
+ * (calc_density, src/common/density_matrix.f90:26-236).
+ */
+int salmon_hip_density(salmon_hip_t h, const void *psi, double *rho_dev);
+
+/*
+ * curr(3, nspin) into a DEVICE array (calc_current, src/common/density_matrix.f90:240-463,
+ * stencil_current :465-530, calc_current_nonlocal :532-591): the kinetic part uses
+ * k + vec_Ac, the nonlocal part the phase table of the last update_kvector call,
+ * exactly as time_evolution_step.f90:243-246 does.  Result divided by N_grid.
+ */
+int salmon_hip_current(salmon_hip_t h, const double *vec_Ac, const void *psi, double *curr_dev);
+
+/*
+ * Electronic dipole and electron count from rho (DEVICE arrays): out[0:3] =
+ * -sum(pos_d * rho) * Hgs_d * Hvol, out[3] = sum(rho) * Hvol (calc_dip and
+ * subdip, src/rt/dip.f90:23-112,114-225).
+ */
+int salmon_hip_dipole(salmon_hip_t h, const double *rho_dev, double *out_dev);
+
+/*
+ * Kernel timing (no counterpart in the reference, whose timers are host
+ * wall-clock regions, src/misc/timer.f90:297-310): between prof_start and
+ * prof_end every launch of the listed kernels is bracketed by start/stop
+ * events recorded by the launch itself (hipExtLaunchKernelGGL), up to
+ * max_launches launches.  prof_stop drains the stream and returns the summed
+ * device time and launch count of one kernel kind.
+ */
+#define SALMON_HIP_KERNEL_HPSI0 0          /* k_hpsi, plain hpsi */
+#define SALMON_HIP_KERNEL_HPSI_TAYLOR1 1   /* k_hpsi, first Taylor term */
+#define SALMON_HIP_KERNEL_HPSI_TAYLOR 2    /* k_hpsi, middle Taylor terms */
+#define SALMON_HIP_KERNEL_HPSI_TAYLOR_LAST 3
+#define SALMON_HIP_KERNEL_PROJ 4           /* k_proj, KB projection */
+#define SALMON_HIP_KERNEL_PROJ_CURRENT 5   /* k_proj, nonlocal-current moments */
+int salmon_hip_prof_start(salmon_hip_t h, int max_launches);
+int salmon_hip_prof_stop(salmon_hip_t h, int kind, double *total_ms, int *count);
+int salmon_hip_prof_end(salmon_hip_t h);
+
+/* Block the host until the handle's stream has drained. */
+int salmon_hip_synchronize(salmon_hip_t h);
+
+/* Device memory helpers for hosts without their own allocator (e.g. Fortran). */
+int salmon_hip_malloc(void **dptr, size_t bytes);
+int salmon_hip_free(void *dptr);
+int salmon_hip_memcpy_htod(salmon_hip_t h, void *dst_dev, const void *src_host, size_t bytes);
+int salmon_hip_memcpy_dtoh(salmon_hip_t h, void *dst_host, const void *src_dev, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SALMON_HIP_H */

@@ -1,0 +1,1159 @@
+// MI355X (gfx950) real-time Kohn-Sham propagation hot path: device state,
+// kernels and the C ABI declared in include/salmon_hip.h.
+//
+// Kernels (one launch each, all deterministic: no floating-point atomics):
+//   k_update_kvector  phase table zekr_uV and per-k stencil terms
+//                     (update_kvector_nonlocalpt, src/common/hamiltonian.f90:847-903)
+//   k_proj<NW>        batched projector x orbital contraction on FP64 MFMA
+//                     (v_mfma_f64_16x16x4f64): uVpsi of zpseudo
+//                     (src/common/nonlocal_potential.f90:360-369) for NW = 1 and the
+//                     four moments of calc_current_nonlocal (density_matrix.f90:532-591)
+//                     for NW = 4
+//   k_hpsi<MODE>      25-point stencil + V_local + k+A gradient
+//                     (zstencil_typical_seq, zstencil_core/typical/seq.f90:17-127),
+//                     the KB scatter of zpseudo (nonlocal_potential.f90:370-378) as a
+//                     deterministic gather over a point->(atom, j) inverse map, and
+//                     the Taylor accumulation of taylor.f90:53-114 in the epilogue
+//   k_density_*       calc_density (density_matrix.f90:150-230)
+//   k_current_*       stencil_current + calc_current (density_matrix.f90:240-530)
+//   k_dipole          calc_dip (src/rt/dip.f90:114-225)
+#include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "salmon_hip.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                            \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) return fail(SALMON_HIP_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCHCHK(name)                                                                          \
+  do {                                                                                           \
+    hipError_t e_ = hipGetLastError();                                                           \
+    if (e_ != hipSuccess) return fail(SALMON_HIP_EHIP, "launch %s: %s", name, hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kBlock = 256;
+constexpr int kProjJC = 16;      // sphere points staged per LDS chunk in k_proj
+constexpr int kProjOrb = 64;     // orbitals per k_proj block (4 waves x 16)
+constexpr int kMaxLmaPerAtom = 16;
+constexpr int kCurSlabs = 8;     // z slabs per orbital in the current kernel
+constexpr int kDensGroup = 32;   // orbitals per density partial
+
+__device__ __forceinline__ int wrapi(int v, int n) { return (v + 8 * n) % n; }
+
+// ---------------------------------------------------------------------------
+// Phase table and per-k stencil coefficients
+// ---------------------------------------------------------------------------
+struct KvecArgs {
+  const double *veck;    // (3, nk)
+  const double *rxyz;    // (3, nps, natom)
+  const double *uv;      // (nps, nlma)
+  const int *mps;        // (natom)
+  const int *lma_atom;   // (nlma) 0-based atom of each projector
+  double2 *zekr;         // (nps, nlma, nk)
+  double *kpar;          // (16, nk): k_lap0, k_nabt(12), kAc(3)
+  double coef_nab[12];
+  double coef_lap0;
+  double ax, ay, az;
+  int nlma, nps, nk;
+};
+
+__global__ __launch_bounds__(kBlock) void k_update_kvector(KvecArgs a) {
+  const int ilma = blockIdx.x;
+  const int ik = blockIdx.y;
+  const double kx = a.veck[3 * ik + 0] + a.ax;
+  const double ky = a.veck[3 * ik + 1] + a.ay;
+  const double kz = a.veck[3 * ik + 2] + a.az;
+  if (ilma == 0 && threadIdx.x == 0) {
+    double *kp = a.kpar + 16 * ik;
+    double ss = kx * kx;
+    ss = ss + ky * ky;
+    ss = ss + kz * kz;
+    kp[0] = a.coef_lap0 + 0.5 * ss;
+    for (int n = 0; n < 4; ++n) {
+      kp[1 + n] = kx * a.coef_nab[n];
+      kp[5 + n] = ky * a.coef_nab[4 + n];
+      kp[9 + n] = kz * a.coef_nab[8 + n];
+    }
+    kp[13] = kx;
+    kp[14] = ky;
+    kp[15] = kz;
+  }
+  if (ilma >= a.nlma) return;
+  const int ia = a.lma_atom[ilma];
+  const int m = a.mps[ia];
+  for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    const double *r = a.rxyz + 3 * ((size_t)j + (size_t)a.nps * ia);
+    const double th = kx * r[0] + ky * r[1] + kz * r[2];
+    double s, c;
+    sincos(th, &s, &c);
+    const double u = a.uv[(size_t)j + (size_t)a.nps * ilma];
+    a.zekr[(size_t)j + (size_t)a.nps * ((size_t)ilma + (size_t)a.nlma * ik)] = make_double2(c * u, -s * u);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Projector x orbital contraction on FP64 MFMA.
+// D[n][o] = sum_K A[n][K] B[K][o] with K = (sphere point j, re/im),
+// o = 16 orbitals per wave, n = (lma, weight w, re/im part).
+//   part re:  Re(conj(Z) r_w psi) = Zr r_w psi_re + Zi r_w psi_im
+//   part im:  Im(conj(Z) r_w psi) = Zr r_w psi_im - Zi r_w psi_re
+// r_0 = 1 (plain projection), r_1..3 = rxyz components (current moments).
+// ---------------------------------------------------------------------------
+struct ProjArgs {
+  const double2 *psi;     // orbital block base
+  const double2 *zekr;    // (nps, nlma, nk)
+  const double *rxyz;     // (3, nps, natom)
+  const int *jidx;        // (nps, natom) offsets within an orbital block
+  const int *mps;         // (natom)
+  const int2 *atom_lma;   // (natom): first ilma, count
+  const double *rinv;     // (nlma)
+  double2 *out;           // NW=1: c(nlma, nb, nk) scaled by rinv; NW=4: (4, nlma, nb, nk) raw
+  size_t osz;
+  int nb, nk, natom, nps, nlma, ngrp;
+};
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
+  constexpr int MAXT = (2 * NW * kMaxLmaPerAtom + 15) / 16;
+  __shared__ double2 sp[kProjJC][kProjOrb];
+  const int grp = blockIdx.x % a.ngrp;
+  const int t = blockIdx.x / a.ngrp;
+  const int ia = t % a.natom;
+  const int ik = t / a.natom;
+  const int2 al = a.atom_lma[ia];
+  const int m = a.mps[ia];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int ncol = 2 * NW * al.y;
+  const int ntile = (ncol + 15) >> 4;
+  const size_t kbase = (size_t)ik * a.nb;
+
+  typedef double __attribute__((ext_vector_type(4))) d4;
+  d4 acc[MAXT];
+#pragma unroll
+  for (int tt = 0; tt < MAXT; ++tt) acc[tt] = d4{0.0, 0.0, 0.0, 0.0};
+
+  const int kq = lane >> 4;      // K row of this lane (0..3)
+  const int comp = kq & 1;       // 0: re, 1: im of psi
+  const int oc = lane & 15;      // orbital column (B) / output row (A)
+
+  for (int j0 = 0; j0 < m; j0 += kProjJC) {
+    // stage psi at kProjJC sphere points for the block's 64 orbitals
+#pragma unroll
+    for (int r = 0; r < (kProjJC * kProjOrb) / kBlock; ++r) {
+      const int e = threadIdx.x + kBlock * r;
+      const int jj = e % kProjJC;
+      const int o = e / kProjJC;
+      const int j = j0 + jj;
+      const int ob = grp * kProjOrb + o;
+      double2 v = make_double2(0.0, 0.0);
+      if (j < m && ob < a.nb) v = a.psi[(kbase + ob) * a.osz + a.jidx[(size_t)a.nps * ia + j]];
+      sp[jj][o] = v;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int ks = 0; ks < kProjJC / 2; ++ks) {
+      const int jp = 2 * ks + (kq >> 1);
+      const int j = j0 + jp;
+      const double2 pv = sp[jp][wave * 16 + oc];
+      const double bval = comp ? pv.y : pv.x;
+      double zr = 0.0, zi = 0.0, rw[3] = {0.0, 0.0, 0.0};
+      const bool jv = j < m;
+      if (jv && NW > 1) {
+        const double *rr = a.rxyz + 3 * ((size_t)j + (size_t)a.nps * ia);
+        rw[0] = rr[0];
+        rw[1] = rr[1];
+        rw[2] = rr[2];
+      }
+#pragma unroll
+      for (int tt = 0; tt < MAXT; ++tt) {
+        if (tt < ntile) {
+          const int n = tt * 16 + oc;
+          double aval = 0.0;
+          if (jv && n < ncol) {
+            const int part = n & 1;
+            const int lw = n >> 1;
+            const int w = lw % NW;
+            const int l = lw / NW;
+            const double2 z = a.zekr[(size_t)j + (size_t)a.nps * ((size_t)(al.x + l) + (size_t)a.nlma * ik)];
+            const double r = (w == 0) ? 1.0 : rw[w - 1];
+            zr = z.x * r;
+            zi = z.y * r;
+            aval = part == 0 ? (comp ? zi : zr) : (comp ? zr : -zi);
+          }
+          acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(aval, bval, acc[tt], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // D layout: lane holds D[4*(lane/16) + r][lane%16], r = 0..3
+  const int ob = grp * kProjOrb + wave * 16 + oc;
+  if (ob >= a.nb) return;
+#pragma unroll
+  for (int tt = 0; tt < MAXT; ++tt) {
+    if (tt >= ntile) break;
+#pragma unroll
+    for (int r = 0; r < 4; r += 2) {
+      const int n = tt * 16 + 4 * kq + r;  // even: re part, n+1: im part
+      if (n >= ncol) continue;
+      const int lw = n >> 1;
+      const int w = lw % NW;
+      const int l = lw / NW;
+      const int ilma = al.x + l;
+      const double re = acc[tt][r], im = acc[tt][r + 1];
+      if (NW == 1) {
+        const double s = a.rinv[ilma];
+        a.out[(kbase + ob) * a.nlma + ilma] = make_double2(re * s, im * s);
+      } else {
+        a.out[((kbase + ob) * a.nlma + ilma) * 4 + w] = make_double2(re, im);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused hpsi: stencil + V + k.grad + KB scatter + Taylor accumulation.
+// Block = full x rows (nx) x TY rows in y, streaming z over [z0, z0+TZ) with a
+// 9-deep register queue along z; x and y neighbours come from an LDS plane
+// (double-buffered) that carries 4 halo rows on either side in y.
+// MODE 0: hout = H in
+// MODE 1: hout = H in;   acc = in + zc * hout        (first Taylor term)
+// MODE 2: hout = H in;   acc = acc + zc * hout       (middle terms)
+// MODE 3:                acc = acc + zc * (H in)     (last term, H^n not stored)
+// ---------------------------------------------------------------------------
+struct HpsiArgs {
+  const double2 *in;
+  double2 *hout;
+  double2 *acc;
+  const double *V;        // (nx, ny, nz, nspin)
+  const double *kpar;     // (16, nk)
+  const double2 *c;       // (nlma, nb, nk) projections (rinv applied)
+  const double2 *zekr;    // (nps, nlma, nk)
+  const int *pt_off;      // (ngrid + 1)
+  const int2 *pt_ent;     // (ilma0 | cnt << 24, j)
+  double lap[12];         // coef_lap(4,3)
+  double2 zc;
+  size_t osz;
+  int nx, ny, nz, ly, nspin, nb, nk, nlma, nps;
+  int TY, TZ, nyt, nzt, nblk;
+  int has_nl;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_hpsi(HpsiArgs a) {
+  extern __shared__ double2 lds[];
+  // XCD-aware block order: blocks dispatched to one XCD (bid % 8) get a
+  // contiguous range of logical tiles, so the y-neighbour tiles of one orbital
+  // share that XCD's L2.
+  int bid = blockIdx.x;
+  const int nfull = (a.nblk / 8) * 8;
+  if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;
+  const int ytile = bid % a.nyt;
+  const int zt = (bid / a.nyt) % a.nzt;
+  const int b = bid / (a.nyt * a.nzt);  // global orbital index (ispin fastest)
+  const int ik = b / a.nb;
+  const int ispin = b % a.nspin;
+
+  const int nx = a.nx, ny = a.ny, nz = a.nz, ly = a.ly;
+  const int TY = a.TY;
+  const int y0 = ytile * TY;
+  const int z0 = zt * a.TZ;
+  const int z1 = min(z0 + a.TZ, nz);
+  const int tid = threadIdx.x;
+  const int tx = tid % nx;
+  const int ty = tid / nx;
+  const bool inTile = ty < TY;
+  const int y = y0 + ty;
+  const bool outY = inTile && y < ny;
+  const int yw = wrapi(y, ny);
+  const int rowW = nx;
+  const int planeL = (TY + 8) * rowW;
+
+  const double2 *in = a.in + (size_t)b * a.osz;
+  const size_t zs = (size_t)nx * ly;
+  const size_t colOff = (size_t)tx + (size_t)nx * yw;
+
+  const double *kp = a.kpar + 16 * ik;
+  const double klap0 = kp[0];
+  double nab[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) nab[i] = kp[1 + i];
+
+  int xo[8];
+#pragma unroll
+  for (int n = 1; n <= 4; ++n) {
+    xo[n - 1] = wrapi(tx + n, nx);
+    xo[3 + n] = wrapi(tx - n, nx);
+  }
+
+  // halo elements of this thread: e = tid + kBlock*h over 8*nx elements
+  const int nhalo = 8 * nx;
+
+  double2 q[9];
+  if (inTile) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) q[i] = in[colOff + zs * wrapi(z0 - 4 + i, nz)];
+  }
+  double2 nxt = make_double2(0.0, 0.0);
+  if (inTile) nxt = in[colOff + zs * wrapi(z0 + 4, nz)];
+  double2 hal[4];
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    const int e = tid + kBlock * h;
+    if (e < nhalo) {
+      const int hr = e / nx, hx = e % nx;
+      const int lr = hr < 4 ? hr : TY + hr;
+      hal[h] = in[(size_t)hx + (size_t)nx * wrapi(y0 - 4 + lr, ny) + zs * wrapi(z0, nz)];
+    }
+  }
+
+  int buf = 0;
+  for (int z = z0; z < z1; ++z) {
+    double2 *P = lds + buf * planeL;
+    if (inTile) {
+      q[8] = nxt;
+      P[(ty + 4) * rowW + tx] = q[4];
+    }
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int e = tid + kBlock * h;
+      if (e < nhalo) {
+        const int hr = e / nx, hx = e % nx;
+        const int lr = hr < 4 ? hr : TY + hr;
+        P[lr * rowW + hx] = hal[h];
+      }
+    }
+    __syncthreads();
+    // prefetch the next plane's queue entry and halo
+    if (z + 1 < z1) {
+      if (inTile) nxt = in[colOff + zs * wrapi(z + 5, nz)];
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int e = tid + kBlock * h;
+        if (e < nhalo) {
+          const int hr = e / nx, hx = e % nx;
+          const int lr = hr < 4 ? hr : TY + hr;
+          hal[h] = in[(size_t)hx + (size_t)nx * wrapi(y0 - 4 + lr, ny) + zs * wrapi(z + 1, nz)];
+        }
+      }
+    }
+    if (outY) {
+      const double2 *row = P + (ty + 4) * rowW;
+      double2 v, w;
+      // x axis
+      {
+        double2 p1 = row[xo[0]], p2 = row[xo[1]], p3 = row[xo[2]], p4 = row[xo[3]];
+        double2 m1 = row[xo[4]], m2 = row[xo[5]], m3 = row[xo[6]], m4 = row[xo[7]];
+        v.x = a.lap[0] * (p1.x + m1.x) + a.lap[1] * (p2.x + m2.x) + a.lap[2] * (p3.x + m3.x) + a.lap[3] * (p4.x + m4.x);
+        v.y = a.lap[0] * (p1.y + m1.y) + a.lap[1] * (p2.y + m2.y) + a.lap[2] * (p3.y + m3.y) + a.lap[3] * (p4.y + m4.y);
+        w.x = nab[0] * (p1.x - m1.x) + nab[1] * (p2.x - m2.x) + nab[2] * (p3.x - m3.x) + nab[3] * (p4.x - m4.x);
+        w.y = nab[0] * (p1.y - m1.y) + nab[1] * (p2.y - m2.y) + nab[2] * (p3.y - m3.y) + nab[3] * (p4.y - m4.y);
+      }
+      // y axis
+      {
+        double2 p1 = P[(ty + 5) * rowW + tx], p2 = P[(ty + 6) * rowW + tx];
+        double2 p3 = P[(ty + 7) * rowW + tx], p4 = P[(ty + 8) * rowW + tx];
+        double2 m1 = P[(ty + 3) * rowW + tx], m2 = P[(ty + 2) * rowW + tx];
+        double2 m3 = P[(ty + 1) * rowW + tx], m4 = P[(ty + 0) * rowW + tx];
+        v.x += a.lap[4] * (p1.x + m1.x) + a.lap[5] * (p2.x + m2.x) + a.lap[6] * (p3.x + m3.x) + a.lap[7] * (p4.x + m4.x);
+        v.y += a.lap[4] * (p1.y + m1.y) + a.lap[5] * (p2.y + m2.y) + a.lap[6] * (p3.y + m3.y) + a.lap[7] * (p4.y + m4.y);
+        w.x += nab[4] * (p1.x - m1.x) + nab[5] * (p2.x - m2.x) + nab[6] * (p3.x - m3.x) + nab[7] * (p4.x - m4.x);
+        w.y += nab[4] * (p1.y - m1.y) + nab[5] * (p2.y - m2.y) + nab[6] * (p3.y - m3.y) + nab[7] * (p4.y - m4.y);
+      }
+      // z axis from the register queue (q[4] is the centre)
+      v.x += a.lap[8] * (q[5].x + q[3].x) + a.lap[9] * (q[6].x + q[2].x) + a.lap[10] * (q[7].x + q[1].x) + a.lap[11] * (q[8].x + q[0].x);
+      v.y += a.lap[8] * (q[5].y + q[3].y) + a.lap[9] * (q[6].y + q[2].y) + a.lap[10] * (q[7].y + q[1].y) + a.lap[11] * (q[8].y + q[0].y);
+      w.x += nab[8] * (q[5].x - q[3].x) + nab[9] * (q[6].x - q[2].x) + nab[10] * (q[7].x - q[1].x) + nab[11] * (q[8].x - q[0].x);
+      w.y += nab[8] * (q[5].y - q[3].y) + nab[9] * (q[6].y - q[2].y) + nab[10] * (q[7].y - q[1].y) + nab[11] * (q[8].y - q[0].y);
+
+      const double2 t = q[4];
+      const size_t g = (size_t)tx + (size_t)nx * ((size_t)y + (size_t)ny * z);
+      const double vl = a.V[g + (size_t)ispin * ((size_t)nx * ny * nz)];
+      // V t + k_lap0 t - v/2 - i w
+      double2 h;
+      h.x = (vl * t.x + klap0 * t.x) - 0.5 * v.x + w.y;
+      h.y = (vl * t.y + klap0 * t.y) - 0.5 * v.y - w.x;
+      if (a.has_nl) {
+        const int e0 = a.pt_off[g], e1 = a.pt_off[g + 1];
+        for (int e = e0; e < e1; ++e) {
+          const int2 en = a.pt_ent[e];
+          const int ilma0 = en.x & 0xffffff;
+          const int cnt = en.x >> 24;
+          const double2 *cc = a.c + ((size_t)ik * a.nb + (b % a.nb)) * a.nlma + ilma0;
+          const double2 *zz = a.zekr + (size_t)en.y + (size_t)a.nps * ((size_t)ilma0 + (size_t)a.nlma * ik);
+          for (int l = 0; l < cnt; ++l) {
+            const double2 cv = cc[l];
+            const double2 zv = zz[(size_t)a.nps * l];
+            h.x += cv.x * zv.x - cv.y * zv.y;
+            h.y += cv.x * zv.y + cv.y * zv.x;
+          }
+        }
+      }
+      const size_t oi = (size_t)b * a.osz + colOff + zs * z;
+      if (MODE != 3) a.hout[oi] = h;
+      if (MODE == 1) {
+        a.acc[oi] = make_double2(t.x + (a.zc.x * h.x - a.zc.y * h.y), t.y + (a.zc.x * h.y + a.zc.y * h.x));
+      } else if (MODE >= 2) {
+        const double2 o = a.acc[oi];
+        a.acc[oi] = make_double2(o.x + (a.zc.x * h.x - a.zc.y * h.y), o.y + (a.zc.x * h.y + a.zc.y * h.x));
+      }
+    }
+    // advance the z queue
+#pragma unroll
+    for (int i = 0; i < 8; ++i) q[i] = q[i + 1];
+    buf ^= 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Density: rho(p) = sum_{ik, io} (|psi|^2 rocc) wtk, orbitals in (ik, io) order,
+// in groups of kDensGroup summed in order, groups combined in order.
+// ---------------------------------------------------------------------------
+struct DensArgs {
+  const double2 *psi;
+  const double *rocc;   // (no, nk, nspin)
+  const double *wtk;
+  double *part;         // (ngrid, nspin, ngrp)
+  double *rho;          // (ngrid, nspin)
+  size_t osz;
+  int nx, ny, nz, ly, nspin, no, nk, ngrp;
+};
+
+__global__ __launch_bounds__(kBlock) void k_density_part(DensArgs a) {
+  const size_t ngrid = (size_t)a.nx * a.ny * a.nz;
+  const size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  const int grp = blockIdx.y;
+  const int ispin = blockIdx.z;
+  if (g >= ngrid) return;
+  const int x = g % a.nx;
+  const int yz = g / a.nx;
+  const int y = yz % a.ny;
+  const int z = yz / a.ny;
+  const size_t off = (size_t)x + (size_t)a.nx * ((size_t)y + (size_t)a.ly * z);
+  const int M = a.no * a.nk;
+  const int m0 = grp * kDensGroup, m1 = min(M, m0 + kDensGroup);
+  double acc = 0.0;
+  for (int mm = m0; mm < m1; ++mm) {
+    const int io = mm % a.no, ik = mm / a.no;
+    const size_t b = (size_t)ispin + (size_t)a.nspin * mm;
+    const double2 p = a.psi[b * a.osz + off];
+    const double w2 = p.x * p.x + p.y * p.y;
+    acc = acc + (w2 * a.rocc[io + (size_t)a.no * (ik + (size_t)a.nk * ispin)]) * a.wtk[ik];
+  }
+  if (a.ngrp == 1)
+    a.rho[g + ngrid * ispin] = acc;
+  else
+    a.part[g + ngrid * ((size_t)ispin + (size_t)a.nspin * grp)] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_density_sum(DensArgs a) {
+  const size_t ngrid = (size_t)a.nx * a.ny * a.nz;
+  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ngrid * a.nspin) return;
+  double acc = 0.0;
+  for (int grp = 0; grp < a.ngrp; ++grp) acc = acc + a.part[i + ngrid * a.nspin * (size_t)grp];
+  a.rho[i] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Current, local part per orbital and z slab: sum |psi|^2 and
+// Im sum_n (nab_n conj(psi)) psi(r + n e_d), forward neighbours only.
+// ---------------------------------------------------------------------------
+struct CurArgs {
+  const double2 *psi;
+  double *part;            // (4, nslab, norb)
+  double coef_nab[12];
+  size_t osz;
+  int nx, ny, nz, ly, nslab;
+};
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_current_local(CurArgs a) {
+  __shared__ double sh[kBlock];
+  const int slab = blockIdx.x % a.nslab;
+  const int b = blockIdx.x / a.nslab;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int zper = (nz + a.nslab - 1) / a.nslab;
+  const int z0 = slab * zper, z1 = min(nz, z0 + zper);
+  const double2 *p = a.psi + (size_t)b * a.osz;
+  const size_t zs = (size_t)nx * a.ly;
+  double r0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+  const int npl = nx * ny;
+  const int cnt = (z1 > z0) ? (z1 - z0) * npl : 0;
+  for (int e = threadIdx.x; e < cnt; e += kBlock) {
+    const int z = z0 + e / npl;
+    const int r = e % npl;
+    const int y = r / nx, x = r % nx;
+    const double2 c = p[(size_t)x + (size_t)nx * y + zs * z];
+    r0 += c.x * c.x + c.y * c.y;
+    const double cr = c.x, ci = -c.y;
+#pragma unroll
+    for (int n = 1; n <= 4; ++n) {
+      const double2 px = p[(size_t)wrapi(x + n, nx) + (size_t)nx * y + zs * z];
+      const double2 py = p[(size_t)x + (size_t)nx * wrapi(y + n, ny) + zs * z];
+      const double2 pz = p[(size_t)x + (size_t)nx * y + zs * wrapi(z + n, nz)];
+      const double ax = a.coef_nab[n - 1] * cr, bx = a.coef_nab[n - 1] * ci;
+      const double ay = a.coef_nab[3 + n] * cr, by = a.coef_nab[3 + n] * ci;
+      const double az = a.coef_nab[7 + n] * cr, bz = a.coef_nab[7 + n] * ci;
+      sx += ax * px.y + bx * px.x;
+      sy += ay * py.y + by * py.x;
+      sz += az * pz.y + bz * pz.x;
+    }
+  }
+  r0 = block_sum(r0, sh);
+  sx = block_sum(sx, sh);
+  sy = block_sum(sy, sh);
+  sz = block_sum(sz, sh);
+  if (threadIdx.x == 0) {
+    double *o = a.part + 4 * ((size_t)slab + (size_t)a.nslab * b);
+    o[0] = r0;
+    o[1] = sx;
+    o[2] = sy;
+    o[3] = sz;
+  }
+}
+
+struct CurFinArgs {
+  const double *part;      // (4, nslab, norb)
+  const double2 *nl;       // (4, nlma, nb, nk) raw moments, or null
+  const double *rinv;      // (nlma)
+  const double *veck;      // (3, nk)
+  const double *rocc;      // (no, nk, nspin)
+  const double *wtk;       // (nk)
+  double *contrib;         // (3, no*nk) scratch
+  double *curr;            // (3, nspin)
+  double ax, ay, az;
+  double inv_ngrid;
+  int nslab, nspin, no, nk, nlma;
+};
+
+__global__ __launch_bounds__(kBlock) void k_current_final(CurFinArgs a, int ispin) {
+  const int M = a.no * a.nk;
+  for (int mm = threadIdx.x; mm < M; mm += kBlock) {
+    const int io = mm % a.no, ik = mm / a.no;
+    const size_t b = (size_t)ispin + (size_t)a.nspin * mm;
+    double rt = 0.0, s[3] = {0.0, 0.0, 0.0};
+    for (int sl = 0; sl < a.nslab; ++sl) {
+      const double *o = a.part + 4 * ((size_t)sl + (size_t)a.nslab * b);
+      rt = rt + o[0];
+      s[0] = s[0] + o[1];
+      s[1] = s[1] + o[2];
+      s[2] = s[2] + o[3];
+    }
+    double jw[3] = {0.0, 0.0, 0.0};
+    if (a.nl) {
+      const int nb = a.no * a.nspin;
+      const size_t ob = (size_t)ik * nb + (size_t)(ispin + a.nspin * io);
+      for (int l = 0; l < a.nlma; ++l) {
+        const double2 *u4 = a.nl + (ob * a.nlma + l) * 4;
+        const double rv = a.rinv[l];
+        const double ur = u4[0].x * rv, ui = u4[0].y * rv;
+        for (int d = 0; d < 3; ++d) {
+          // aimag(conjg(uVpsi_r) * uVpsi)
+          jw[d] = jw[d] + (u4[1 + d].x * ui - u4[1 + d].y * ur);
+        }
+      }
+      for (int d = 0; d < 3; ++d) jw[d] = jw[d] * 2.0;
+    }
+    const double kA[3] = {a.veck[3 * ik] + a.ax, a.veck[3 * ik + 1] + a.ay, a.veck[3 * ik + 2] + a.az};
+    const double occ = a.rocc[io + (size_t)a.no * (ik + (size_t)a.nk * ispin)];
+    for (int d = 0; d < 3; ++d) {
+      const double j1 = kA[d] * rt;
+      const double j2 = s[d] * 2.0;
+      a.contrib[3 * (size_t)mm + d] = (((j1 + j2) + jw[d]) * occ) * a.wtk[ik];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int d = threadIdx.x;
+    double w4 = 0.0;
+    for (int mm = 0; mm < M; ++mm) w4 = w4 + a.contrib[3 * (size_t)mm + d];
+    a.curr[3 * ispin + d] = w4 * a.inv_ngrid;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dipole: out[0:3] = -sum(pos_d rho) Hgs_d Hvol, out[3] = sum(rho) Hvol
+// pos = i (odd n) or i - 1/2 (even n), i the 1-based grid index.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_dipole(const double *rho, double *out, int nx, int ny, int nz,
+                                                 int nspin, double hx, double hy, double hz, double hvol) {
+  __shared__ double sh[4][1024];
+  const size_t ngrid = (size_t)nx * ny * nz;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int is = 0; is < nspin; ++is)
+    for (size_t g = threadIdx.x; g < ngrid; g += 1024) {
+      const int x = g % nx;
+      const int y = (g / nx) % ny;
+      const int z = g / ((size_t)nx * ny);
+      const double r = rho[g + ngrid * is];
+      const double px = (nx % 2 == 1) ? (double)(x + 1) : ((double)(x + 1) - 0.5);
+      const double py = (ny % 2 == 1) ? (double)(y + 1) : ((double)(y + 1) - 0.5);
+      const double pz = (nz % 2 == 1) ? (double)(z + 1) : ((double)(z + 1) - 0.5);
+      acc[0] += px * r;
+      acc[1] += py * r;
+      acc[2] += pz * r;
+      acc[3] += r;
+    }
+  for (int d = 0; d < 4; ++d) sh[d][threadIdx.x] = acc[d];
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int d = 0; d < 4; ++d) sh[d][threadIdx.x] = sh[d][threadIdx.x] + sh[d][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = -sh[0][0] * hx * hvol;
+    out[1] = -sh[1][0] * hy * hvol;
+    out[2] = -sh[2][0] * hz * hvol;
+    out[3] = sh[3][0] * hvol;
+  }
+}
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return 0;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(SALMON_HIP_ENOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+  }
+  return 0;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct salmon_hip_ctx {
+  int nx = 0, ny = 0, nz = 0, ly = 0, nspin = 0, no = 0, nk = 0;
+  int nb = 0, norb = 0;
+  size_t osz = 0, ngrid = 0;
+  double coef_lap[12] = {}, coef_nab[12] = {}, coef_lap0 = 0, hgs[3] = {}, hvol = 0;
+  hipStream_t own = nullptr, stream = nullptr;
+  bool have_V = false, have_k = false;
+  double *d_veck = nullptr, *d_rocc = nullptr, *d_wtk = nullptr, *d_V = nullptr, *d_kpar = nullptr;
+  // nonlocal
+  int natom = 0, nps = 0, nlma = 0, nent = 0;
+  int *d_mps = nullptr, *d_jidx = nullptr, *d_lma_atom = nullptr, *d_pt_off = nullptr;
+  int2 *d_atom_lma = nullptr, *d_pt_ent = nullptr;
+  double *d_rxyz = nullptr, *d_uv = nullptr, *d_rinv = nullptr;
+  double2 *d_zekr = nullptr, *d_c = nullptr, *d_curnl = nullptr;
+  // observables scratch
+  double *d_curpart = nullptr, *d_contrib = nullptr, *d_denspart = nullptr;
+  int dens_ngrp = 1;
+  // stencil launch shape
+  int TY = 1, TZ = 1, nyt = 1, nzt = 1;
+  // kernel timing (salmon_hip_prof_*): start/stop event pairs per launch
+  bool prof = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ev_kind;
+  size_t ev_used = 0;
+};
+
+// Events for one timed launch, or nullptr pair when timing is off / the pool is full.
+static void prof_events(salmon_hip_ctx *c, int kind, hipEvent_t *s, hipEvent_t *e) {
+  *s = nullptr;
+  *e = nullptr;
+  if (!c->prof || 2 * (c->ev_used + 1) > c->ev.size()) return;
+  *s = c->ev[2 * c->ev_used];
+  *e = c->ev[2 * c->ev_used + 1];
+  c->ev_kind[c->ev_used] = kind;
+  c->ev_used += 1;
+}
+
+static int check(salmon_hip_t h) {
+  if (!h) return fail(SALMON_HIP_EINVAL, "null handle");
+  return 0;
+}
+
+extern "C" {
+
+const char *salmon_hip_last_error(void) { return g_err.c_str(); }
+
+int salmon_hip_create(int nx, int ny, int nz, int ly, int nspin, int no, int nk, const double *coef_lap,
+                      const double *coef_nab, double coef_lap0, const double *vec_k, const double *rocc,
+                      const double *wtk, const double *hgs, double hvol, salmon_hip_t *out) {
+  if (!out) return fail(SALMON_HIP_EINVAL, "out is null");
+  *out = nullptr;
+  if (nx < 4 || ny < 4 || nz < 4) return fail(SALMON_HIP_EINVAL, "grid %dx%dx%d below Nd=4", nx, ny, nz);
+  if (nx > kBlock) return fail(SALMON_HIP_EUNSUPPORTED, "nx=%d above %d", nx, kBlock);
+  if (ly < ny) return fail(SALMON_HIP_EINVAL, "ly=%d < ny=%d", ly, ny);
+  if (nspin != 1 && nspin != 2) return fail(SALMON_HIP_EINVAL, "nspin=%d", nspin);
+  if (no < 1 || nk < 1) return fail(SALMON_HIP_EINVAL, "no=%d nk=%d", no, nk);
+  if (!coef_lap || !coef_nab || !vec_k || !rocc || !wtk || !hgs) return fail(SALMON_HIP_EINVAL, "null input");
+  salmon_hip_ctx *c = new salmon_hip_ctx();
+  c->nx = nx; c->ny = ny; c->nz = nz; c->ly = ly;
+  c->nspin = nspin; c->no = no; c->nk = nk;
+  c->nb = nspin * no;
+  c->norb = c->nb * nk;
+  c->osz = (size_t)nx * ly * nz;
+  c->ngrid = (size_t)nx * ny * nz;
+  memcpy(c->coef_lap, coef_lap, sizeof(c->coef_lap));
+  memcpy(c->coef_nab, coef_nab, sizeof(c->coef_nab));
+  c->coef_lap0 = coef_lap0;
+  memcpy(c->hgs, hgs, sizeof(c->hgs));
+  c->hvol = hvol;
+  int rc = 0;
+  auto bail = [&](int code) { salmon_hip_destroy(c); return code; };
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(SALMON_HIP_EHIP, "hipStreamCreate failed"));
+  c->stream = c->own;
+  if ((rc = dalloc(&c->d_veck, 3 * (size_t)nk))) return bail(rc);
+  if ((rc = dalloc(&c->d_rocc, (size_t)no * nk * nspin))) return bail(rc);
+  if ((rc = dalloc(&c->d_wtk, (size_t)nk))) return bail(rc);
+  if ((rc = dalloc(&c->d_V, c->ngrid * nspin))) return bail(rc);
+  if ((rc = dalloc(&c->d_kpar, 16 * (size_t)nk))) return bail(rc);
+  if (hipMemcpy(c->d_veck, vec_k, sizeof(double) * 3 * nk, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_rocc, rocc, sizeof(double) * no * nk * nspin, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_wtk, wtk, sizeof(double) * nk, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(SALMON_HIP_EHIP, "upload of k-point tables failed"));
+  // stencil tile shape: full x rows, TY rows, z chunks so that the launch has
+  // at least ~2048 blocks where the orbital count allows it
+  c->TY = std::min(ny, std::max(1, kBlock / nx));
+  c->nyt = (ny + c->TY - 1) / c->TY;
+  int want = 2048;
+  int nzt = (want + c->norb * c->nyt - 1) / (c->norb * c->nyt);
+  nzt = std::max(1, std::min(nzt, std::max(1, nz / 8)));
+  c->TZ = (nz + nzt - 1) / nzt;
+  c->nzt = (nz + c->TZ - 1) / c->TZ;
+  // observables scratch
+  c->dens_ngrp = (no * nk + kDensGroup - 1) / kDensGroup;
+  if (c->dens_ngrp > 1 && (rc = dalloc(&c->d_denspart, c->ngrid * nspin * c->dens_ngrp))) return bail(rc);
+  if ((rc = dalloc(&c->d_curpart, 4 * (size_t)kCurSlabs * c->norb))) return bail(rc);
+  if ((rc = dalloc(&c->d_contrib, 3 * (size_t)no * nk))) return bail(rc);
+  *out = c;
+  return 0;
+}
+
+int salmon_hip_destroy(salmon_hip_t c) {
+  if (!c) return 0;
+  if (c->own) (void)hipStreamSynchronize(c->own);
+  if (c->stream && c->stream != c->own) (void)hipStreamSynchronize(c->stream);
+  dfree(c->d_veck); dfree(c->d_rocc); dfree(c->d_wtk); dfree(c->d_V); dfree(c->d_kpar);
+  dfree(c->d_mps); dfree(c->d_jidx); dfree(c->d_lma_atom); dfree(c->d_pt_off);
+  dfree(c->d_atom_lma); dfree(c->d_pt_ent); dfree(c->d_rxyz); dfree(c->d_uv); dfree(c->d_rinv);
+  dfree(c->d_zekr); dfree(c->d_c); dfree(c->d_curnl);
+  dfree(c->d_curpart); dfree(c->d_contrib); dfree(c->d_denspart);
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+  return 0;
+}
+
+int salmon_hip_set_stream(salmon_hip_t c, void *stream) {
+  if (int rc = check(c)) return rc;
+  c->stream = stream ? (hipStream_t)stream : c->own;
+  return 0;
+}
+
+void *salmon_hip_get_stream(salmon_hip_t c) { return c ? (void *)c->stream : nullptr; }
+
+int salmon_hip_set_nonlocal(salmon_hip_t c, int natom, int nps, const int *mps, const int *jxyz,
+                            const double *rxyz, int nlma, const int *ia_tbl, const double *rinv_uvu,
+                            const double *uv) {
+  if (int rc = check(c)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  dfree(c->d_mps); dfree(c->d_jidx); dfree(c->d_lma_atom); dfree(c->d_pt_off);
+  dfree(c->d_atom_lma); dfree(c->d_pt_ent); dfree(c->d_rxyz); dfree(c->d_uv); dfree(c->d_rinv);
+  dfree(c->d_zekr); dfree(c->d_c); dfree(c->d_curnl);
+  c->natom = c->nps = c->nlma = c->nent = 0;
+  c->have_k = false;
+  if (natom == 0 || nlma == 0) return 0;
+  if (natom < 0 || nps < 1 || nlma < 0 || !mps || !jxyz || !rxyz || !ia_tbl || !rinv_uvu || !uv)
+    return fail(SALMON_HIP_EINVAL, "invalid projector tables");
+  // validate: sphere sizes and grid indices, projectors contiguous per atom
+  std::vector<int2> atom_lma(natom, make_int2(0, 0));
+  std::vector<int> lma_atom(nlma);
+  for (int l = 0; l < nlma; ++l) {
+    const int ia = ia_tbl[l] - 1;
+    if (ia < 0 || ia >= natom) return fail(SALMON_HIP_EINVAL, "ia_tbl(%d)=%d out of range", l + 1, ia + 1);
+    if (l > 0 && ia < ia_tbl[l - 1] - 1) return fail(SALMON_HIP_EUNSUPPORTED, "ia_tbl not atom-ordered");
+    if (atom_lma[ia].y == 0) atom_lma[ia].x = l;
+    else if (atom_lma[ia].x + atom_lma[ia].y != l)
+      return fail(SALMON_HIP_EUNSUPPORTED, "projectors of atom %d not contiguous", ia + 1);
+    atom_lma[ia].y += 1;
+    lma_atom[l] = ia;
+  }
+  for (int ia = 0; ia < natom; ++ia)
+    if (atom_lma[ia].y > kMaxLmaPerAtom)
+      return fail(SALMON_HIP_EUNSUPPORTED, "atom %d has %d projectors (max %d)", ia + 1, atom_lma[ia].y,
+                  kMaxLmaPerAtom);
+  if (nlma >= (1 << 24)) return fail(SALMON_HIP_EUNSUPPORTED, "nlma too large");
+  std::vector<int> jidx((size_t)natom * nps, 0);
+  std::vector<int> cnt(c->ngrid + 1, 0);
+  for (int ia = 0; ia < natom; ++ia) {
+    if (mps[ia] < 0 || mps[ia] > nps) return fail(SALMON_HIP_EINVAL, "mps(%d)=%d", ia + 1, mps[ia]);
+    for (int j = 0; j < mps[ia]; ++j) {
+      const int *p = jxyz + 3 * ((size_t)j + (size_t)nps * ia);
+      const int x = p[0] - 1, y = p[1] - 1, z = p[2] - 1;
+      if (x < 0 || x >= c->nx || y < 0 || y >= c->ny || z < 0 || z >= c->nz)
+        return fail(SALMON_HIP_EINVAL, "jxyz(:,%d,%d) outside the grid", j + 1, ia + 1);
+      jidx[(size_t)nps * ia + j] = x + c->nx * (y + c->ly * z);
+      const size_t g = (size_t)x + (size_t)c->nx * ((size_t)y + (size_t)c->ny * z);
+      if (atom_lma[ia].y > 0) cnt[g + 1] += 1;
+    }
+  }
+  for (size_t g = 0; g < c->ngrid; ++g) cnt[g + 1] += cnt[g];
+  const int nent = cnt[c->ngrid];
+  std::vector<int2> ent(std::max(nent, 1));
+  std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+  for (int ia = 0; ia < natom; ++ia) {  // atom-major fill keeps entries sorted by ilma per point
+    if (atom_lma[ia].y == 0) continue;
+    for (int j = 0; j < mps[ia]; ++j) {
+      const int *p = jxyz + 3 * ((size_t)j + (size_t)nps * ia);
+      const size_t g = (size_t)(p[0] - 1) + (size_t)c->nx * ((size_t)(p[1] - 1) + (size_t)c->ny * (p[2] - 1));
+      ent[fillp[g]++] = make_int2(atom_lma[ia].x | (atom_lma[ia].y << 24), j);
+    }
+  }
+  int rc = 0;
+  if ((rc = dalloc(&c->d_mps, natom)) || (rc = dalloc(&c->d_jidx, (size_t)natom * nps)) ||
+      (rc = dalloc(&c->d_lma_atom, nlma)) || (rc = dalloc(&c->d_pt_off, c->ngrid + 1)) ||
+      (rc = dalloc(&c->d_atom_lma, natom)) || (rc = dalloc(&c->d_pt_ent, ent.size())) ||
+      (rc = dalloc(&c->d_rxyz, 3 * (size_t)nps * natom)) || (rc = dalloc(&c->d_uv, (size_t)nps * nlma)) ||
+      (rc = dalloc(&c->d_rinv, nlma)) || (rc = dalloc(&c->d_zekr, (size_t)nps * nlma * c->nk)) ||
+      (rc = dalloc(&c->d_c, (size_t)nlma * c->norb)) || (rc = dalloc(&c->d_curnl, 4 * (size_t)nlma * c->norb)))
+    return rc;
+  HIPCHK(hipMemcpy(c->d_mps, mps, sizeof(int) * natom, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_jidx, jidx.data(), sizeof(int) * jidx.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_lma_atom, lma_atom.data(), sizeof(int) * nlma, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_pt_off, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_atom_lma, atom_lma.data(), sizeof(int2) * natom, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_pt_ent, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_rxyz, rxyz, sizeof(double) * 3 * nps * natom, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_uv, uv, sizeof(double) * (size_t)nps * nlma, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_rinv, rinv_uvu, sizeof(double) * nlma, hipMemcpyHostToDevice));
+  c->natom = natom;
+  c->nps = nps;
+  c->nlma = nlma;
+  c->nent = nent;
+  return 0;
+}
+
+int salmon_hip_set_vlocal(salmon_hip_t c, const double *vlocal) {
+  if (int rc = check(c)) return rc;
+  if (!vlocal) return fail(SALMON_HIP_EINVAL, "vlocal is null");
+  HIPCHK(hipMemcpyAsync(c->d_V, vlocal, sizeof(double) * c->ngrid * c->nspin, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_V = true;
+  return 0;
+}
+
+int salmon_hip_set_vlocal_device(salmon_hip_t c, const double *vlocal_dev) {
+  if (int rc = check(c)) return rc;
+  if (!vlocal_dev) return fail(SALMON_HIP_EINVAL, "vlocal is null");
+  HIPCHK(hipMemcpyAsync(c->d_V, vlocal_dev, sizeof(double) * c->ngrid * c->nspin, hipMemcpyDeviceToDevice,
+                        c->stream));
+  c->have_V = true;
+  return 0;
+}
+
+int salmon_hip_update_kvector(salmon_hip_t c, const double *vec_Ac) {
+  if (int rc = check(c)) return rc;
+  if (!vec_Ac) return fail(SALMON_HIP_EINVAL, "vec_Ac is null");
+  KvecArgs a;
+  a.veck = c->d_veck;
+  a.rxyz = c->d_rxyz;
+  a.uv = c->d_uv;
+  a.mps = c->d_mps;
+  a.lma_atom = c->d_lma_atom;
+  a.zekr = c->d_zekr;
+  a.kpar = c->d_kpar;
+  memcpy(a.coef_nab, c->coef_nab, sizeof(a.coef_nab));
+  a.coef_lap0 = c->coef_lap0;
+  a.ax = vec_Ac[0];
+  a.ay = vec_Ac[1];
+  a.az = vec_Ac[2];
+  a.nlma = c->nlma;
+  a.nps = c->nps;
+  a.nk = c->nk;
+  dim3 grid(std::max(1, c->nlma), c->nk);
+  hipLaunchKernelGGL(k_update_kvector, grid, dim3(kBlock), 0, c->stream, a);
+  LAUNCHCHK("k_update_kvector");
+  c->have_k = true;
+  return 0;
+}
+
+static int launch_proj(salmon_hip_t c, const void *psi, int nw) {
+  ProjArgs a;
+  a.psi = (const double2 *)psi;
+  a.zekr = c->d_zekr;
+  a.rxyz = c->d_rxyz;
+  a.jidx = c->d_jidx;
+  a.mps = c->d_mps;
+  a.atom_lma = c->d_atom_lma;
+  a.rinv = c->d_rinv;
+  a.out = nw == 1 ? c->d_c : c->d_curnl;
+  a.osz = c->osz;
+  a.nb = c->nb;
+  a.nk = c->nk;
+  a.natom = c->natom;
+  a.nps = c->nps;
+  a.nlma = c->nlma;
+  a.ngrp = (c->nb + kProjOrb - 1) / kProjOrb;
+  dim3 grid((unsigned)(a.ngrp * c->natom * c->nk));
+  hipEvent_t e0, e1;
+  prof_events(c, nw == 1 ? SALMON_HIP_KERNEL_PROJ : SALMON_HIP_KERNEL_PROJ_CURRENT, &e0, &e1);
+  if (nw == 1)
+    hipExtLaunchKernelGGL(k_proj<1>, grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+  else
+    hipExtLaunchKernelGGL(k_proj<4>, grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+  LAUNCHCHK("k_proj");
+  return 0;
+}
+
+static int launch_hpsi(salmon_hip_t c, int mode, const void *in, void *hout, void *acc, const double *zc) {
+  HpsiArgs a;
+  a.in = (const double2 *)in;
+  a.hout = (double2 *)hout;
+  a.acc = (double2 *)acc;
+  a.V = c->d_V;
+  a.kpar = c->d_kpar;
+  a.c = c->d_c;
+  a.zekr = c->d_zekr;
+  a.pt_off = c->d_pt_off;
+  a.pt_ent = c->d_pt_ent;
+  memcpy(a.lap, c->coef_lap, sizeof(a.lap));
+  a.zc = zc ? make_double2(zc[0], zc[1]) : make_double2(0.0, 0.0);
+  a.osz = c->osz;
+  a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.ly = c->ly;
+  a.nspin = c->nspin; a.nb = c->nb; a.nk = c->nk; a.nlma = c->nlma; a.nps = c->nps;
+  a.TY = c->TY; a.TZ = c->TZ; a.nyt = c->nyt; a.nzt = c->nzt;
+  a.nblk = c->norb * c->nyt * c->nzt;
+  a.has_nl = c->natom > 0 && c->nlma > 0;
+  const size_t shmem = sizeof(double2) * 2 * (size_t)(c->TY + 8) * c->nx;
+  dim3 grid((unsigned)a.nblk);
+  hipEvent_t e0, e1;
+  prof_events(c, SALMON_HIP_KERNEL_HPSI0 + mode, &e0, &e1);
+  switch (mode) {
+    case 0: hipExtLaunchKernelGGL(k_hpsi<0>, grid, dim3(kBlock), shmem, c->stream, e0, e1, 0, a); break;
+    case 1: hipExtLaunchKernelGGL(k_hpsi<1>, grid, dim3(kBlock), shmem, c->stream, e0, e1, 0, a); break;
+    case 2: hipExtLaunchKernelGGL(k_hpsi<2>, grid, dim3(kBlock), shmem, c->stream, e0, e1, 0, a); break;
+    default: hipExtLaunchKernelGGL(k_hpsi<3>, grid, dim3(kBlock), shmem, c->stream, e0, e1, 0, a); break;
+  }
+  LAUNCHCHK("k_hpsi");
+  return 0;
+}
+
+static int ready(salmon_hip_t c) {
+  if (!c->have_V) return fail(SALMON_HIP_ESTATE, "V_local not set");
+  if (!c->have_k) return fail(SALMON_HIP_ESTATE, "update_kvector (vec_Ac) not set");
+  return 0;
+}
+
+int salmon_hip_hpsi(salmon_hip_t c, const void *tpsi, void *htpsi) {
+  if (int rc = check(c)) return rc;
+  if (!tpsi || !htpsi || tpsi == htpsi) return fail(SALMON_HIP_EINVAL, "tpsi/htpsi invalid or aliased");
+  if (int rc = ready(c)) return rc;
+  if (c->natom > 0)
+    if (int rc = launch_proj(c, tpsi, 1)) return rc;
+  return launch_hpsi(c, 0, tpsi, htpsi, nullptr, nullptr);
+}
+
+int salmon_hip_taylor(salmon_hip_t c, const double *zc, int n_hamil, void *tspsi_in, void *tspsi_out,
+                      void *sshtpsi) {
+  if (int rc = check(c)) return rc;
+  if (!zc || n_hamil < 1) return fail(SALMON_HIP_EINVAL, "zc/n_hamil invalid");
+  if (!tspsi_in || !tspsi_out || !sshtpsi || tspsi_in == tspsi_out || tspsi_in == sshtpsi || tspsi_out == sshtpsi)
+    return fail(SALMON_HIP_EINVAL, "taylor buffers invalid or aliased");
+  if (int rc = ready(c)) return rc;
+  for (int nn = 1; nn <= n_hamil; ++nn) {
+    void *src = (nn % 2 == 1) ? tspsi_in : sshtpsi;
+    void *dst = (nn % 2 == 1) ? sshtpsi : tspsi_in;
+    int mode = (nn == 1) ? 1 : 2;
+    if (nn == n_hamil && nn > 1) mode = 3;
+    if (c->natom > 0)
+      if (int rc = launch_proj(c, src, 1)) return rc;
+    if (int rc = launch_hpsi(c, mode, src, dst, tspsi_out, zc + 2 * (nn - 1))) return rc;
+  }
+  return 0;
+}
+
+int salmon_hip_density(salmon_hip_t c, const void *psi, double *rho_dev) {
+  if (int rc = check(c)) return rc;
+  if (!psi || !rho_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+  DensArgs a;
+  a.psi = (const double2 *)psi;
+  a.rocc = c->d_rocc;
+  a.wtk = c->d_wtk;
+  a.part = c->d_denspart;
+  a.rho = rho_dev;
+  a.osz = c->osz;
+  a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.ly = c->ly;
+  a.nspin = c->nspin; a.no = c->no; a.nk = c->nk;
+  a.ngrp = c->dens_ngrp;
+  dim3 grid((unsigned)((c->ngrid + kBlock - 1) / kBlock), a.ngrp, c->nspin);
+  hipLaunchKernelGGL(k_density_part, grid, dim3(kBlock), 0, c->stream, a);
+  LAUNCHCHK("k_density_part");
+  if (a.ngrp > 1) {
+    dim3 g2((unsigned)((c->ngrid * c->nspin + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_density_sum, g2, dim3(kBlock), 0, c->stream, a);
+    LAUNCHCHK("k_density_sum");
+  }
+  return 0;
+}
+
+int salmon_hip_current(salmon_hip_t c, const double *vec_Ac, const void *psi, double *curr_dev) {
+  if (int rc = check(c)) return rc;
+  if (!vec_Ac || !psi || !curr_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+  if (c->natom > 0 && !c->have_k) return fail(SALMON_HIP_ESTATE, "update_kvector not called");
+  CurArgs a;
+  a.psi = (const double2 *)psi;
+  a.part = c->d_curpart;
+  memcpy(a.coef_nab, c->coef_nab, sizeof(a.coef_nab));
+  a.osz = c->osz;
+  a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.ly = c->ly;
+  a.nslab = kCurSlabs;
+  hipLaunchKernelGGL(k_current_local, dim3((unsigned)(c->norb * kCurSlabs)), dim3(kBlock), 0, c->stream, a);
+  LAUNCHCHK("k_current_local");
+  if (c->natom > 0)
+    if (int rc = launch_proj(c, psi, 4)) return rc;
+  CurFinArgs f;
+  f.part = c->d_curpart;
+  f.nl = c->natom > 0 ? c->d_curnl : nullptr;
+  f.rinv = c->d_rinv;
+  f.veck = c->d_veck;
+  f.rocc = c->d_rocc;
+  f.wtk = c->d_wtk;
+  f.contrib = c->d_contrib;
+  f.curr = curr_dev;
+  f.ax = vec_Ac[0];
+  f.ay = vec_Ac[1];
+  f.az = vec_Ac[2];
+  f.inv_ngrid = 1.0 / (double)c->ngrid;
+  f.nslab = kCurSlabs;
+  f.nspin = c->nspin;
+  f.no = c->no;
+  f.nk = c->nk;
+  f.nlma = c->nlma;
+  for (int is = 0; is < c->nspin; ++is) {
+    hipLaunchKernelGGL(k_current_final, dim3(1), dim3(kBlock), 0, c->stream, f, is);
+    LAUNCHCHK("k_current_final");
+  }
+  return 0;
+}
+
+int salmon_hip_dipole(salmon_hip_t c, const double *rho_dev, double *out_dev) {
+  if (int rc = check(c)) return rc;
+  if (!rho_dev || !out_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+  hipLaunchKernelGGL(k_dipole, dim3(1), dim3(1024), 0, c->stream, rho_dev, out_dev, c->nx, c->ny, c->nz,
+                     c->nspin, c->hgs[0], c->hgs[1], c->hgs[2], c->hvol);
+  LAUNCHCHK("k_dipole");
+  return 0;
+}
+
+int salmon_hip_prof_start(salmon_hip_t c, int max_launches) {
+  if (int rc = check(c)) return rc;
+  if (max_launches < 1) return fail(SALMON_HIP_EINVAL, "max_launches=%d", max_launches);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  while (c->ev.size() < 2 * (size_t)max_launches) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c->ev.push_back(e);
+  }
+  c->ev_kind.assign(c->ev.size() / 2, -1);
+  c->ev_used = 0;
+  c->prof = true;
+  return 0;
+}
+
+int salmon_hip_prof_stop(salmon_hip_t c, int kind, double *total_ms, int *count) {
+  if (int rc = check(c)) return rc;
+  if (!total_ms || !count) return fail(SALMON_HIP_EINVAL, "null output");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  int n = 0;
+  for (size_t i = 0; i < c->ev_used; ++i) {
+    if (c->ev_kind[i] != kind) continue;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[2 * i], c->ev[2 * i + 1]));
+    tot += ms;
+    n += 1;
+  }
+  *total_ms = tot;
+  *count = n;
+  return 0;
+}
+
+int salmon_hip_prof_end(salmon_hip_t c) {
+  if (int rc = check(c)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->prof = false;
+  c->ev_used = 0;
+  return 0;
+}
+
+int salmon_hip_synchronize(salmon_hip_t c) {
+  if (int rc = check(c)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int salmon_hip_malloc(void **dptr, size_t bytes) {
+  if (!dptr) return fail(SALMON_HIP_EINVAL, "dptr is null");
+  hipError_t e = hipMalloc(dptr, bytes);
+  if (e != hipSuccess) return fail(SALMON_HIP_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return 0;
+}
+
+int salmon_hip_free(void *dptr) {
+  HIPCHK(hipFree(dptr));
+  return 0;
+}
+
+int salmon_hip_memcpy_htod(salmon_hip_t c, void *dst, const void *src, size_t bytes) {
+  if (int rc = check(c)) return rc;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int salmon_hip_memcpy_dtoh(salmon_hip_t c, void *dst, const void *src, size_t bytes) {
+  if (int rc = check(c)) return rc;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+"""ctypes binding of the C ABI in include/salmon_hip.h.
+
+The library is the in-tree ``salmon2_amd/libsalmon_hip.so`` built by
+``salmon2_amd.build``.  There is no fallback: if the library is missing or a
+call fails, an exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import build as _build
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+EXPORTS = (
+    "salmon_hip_last_error",
+    "salmon_hip_create",
+    "salmon_hip_destroy",
+    "salmon_hip_set_stream",
+    "salmon_hip_get_stream",
+    "salmon_hip_set_nonlocal",
+    "salmon_hip_set_vlocal",
+    "salmon_hip_set_vlocal_device",
+    "salmon_hip_update_kvector",
+    "salmon_hip_hpsi",
+    "salmon_hip_taylor",
+    "salmon_hip_density",
+    "salmon_hip_current",
+    "salmon_hip_dipole",
+    "salmon_hip_synchronize",
+    "salmon_hip_prof_start",
+    "salmon_hip_prof_stop",
+    "salmon_hip_prof_end",
+    "salmon_hip_malloc",
+    "salmon_hip_free",
+    "salmon_hip_memcpy_htod",
+    "salmon_hip_memcpy_dtoh",
+)
+
+_lib = None
+
+
+class SalmonHipError(RuntimeError):
+    pass
+
+
+def lib_path() -> str:
+    return _build.LIB
+
+
+def load():
+    """Load libsalmon_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise SalmonHipError(f"{path} is missing: build it with `python -m salmon2_amd.build`")
+    L = C.CDLL(path)
+    L.salmon_hip_last_error.restype = C.c_char_p
+    L.salmon_hip_create.argtypes = [C.c_int] * 7 + [_dp, _dp, C.c_double, _dp, _dp, _dp, _dp, C.c_double,
+                                                   C.POINTER(C.c_void_p)]
+    L.salmon_hip_destroy.argtypes = [C.c_void_p]
+    L.salmon_hip_set_stream.argtypes = [C.c_void_p, C.c_void_p]
+    L.salmon_hip_get_stream.argtypes = [C.c_void_p]
+    L.salmon_hip_get_stream.restype = C.c_void_p
+    L.salmon_hip_set_nonlocal.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _dp, C.c_int, _ip, _dp, _dp]
+    L.salmon_hip_set_vlocal.argtypes = [C.c_void_p, _dp]
+    L.salmon_hip_set_vlocal_device.argtypes = [C.c_void_p, C.c_void_p]
+    L.salmon_hip_update_kvector.argtypes = [C.c_void_p, _dp]
+    L.salmon_hip_hpsi.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.salmon_hip_taylor.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.salmon_hip_density.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.salmon_hip_current.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p]
+    L.salmon_hip_dipole.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.salmon_hip_synchronize.argtypes = [C.c_void_p]
+    L.salmon_hip_prof_start.argtypes = [C.c_void_p, C.c_int]
+    L.salmon_hip_prof_stop.argtypes = [C.c_void_p, C.c_int, _dp, _ip]
+    L.salmon_hip_prof_end.argtypes = [C.c_void_p]
+    L.salmon_hip_malloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    L.salmon_hip_free.argtypes = [C.c_void_p]
+    L.salmon_hip_memcpy_htod.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    L.salmon_hip_memcpy_dtoh.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    _lib = L
+    return L
+
+
+def _chk(rc: int, what: str):
+    if rc != 0:
+        msg = load().salmon_hip_last_error().decode(errors="replace")
+        raise SalmonHipError(f"{what} failed (code {rc}): {msg}")
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def _ptr(t) -> int:
+    """Device address of a torch tensor (or an int address)."""
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+class Handle:
+    """One grid + orbital set on the GPU (wraps salmon_hip_create .. destroy)."""
+
+    def __init__(self, sysm, stream=None):
+        L = load()
+        self.s = sysm
+        h = C.c_void_p()
+        lap = np.ascontiguousarray(sysm.coef_lap, dtype=np.float64).ravel()
+        nab = np.ascontiguousarray(sysm.coef_nab, dtype=np.float64).ravel()
+        veck = np.ascontiguousarray(sysm.vec_k, dtype=np.float64)
+        rocc = np.ascontiguousarray(sysm.rocc, dtype=np.float64)
+        wtk = np.ascontiguousarray(sysm.wtk, dtype=np.float64)
+        hgs = np.ascontiguousarray(sysm.hgs, dtype=np.float64)
+        _chk(L.salmon_hip_create(sysm.nx, sysm.ny, sysm.nz, sysm.ly, sysm.nspin, sysm.no, sysm.nk, _p(lap),
+                                 _p(nab), float(sysm.coef_lap0), _p(veck), _p(rocc), _p(wtk), _p(hgs),
+                                 float(sysm.hvol), C.byref(h)), "salmon_hip_create")
+        self.h = h
+        if stream is not None:
+            self.set_stream(stream)
+        if sysm.natom > 0:
+            self.set_nonlocal(sysm)
+        self.set_vlocal(sysm.vlocal)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            load().salmon_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream):
+        """stream: torch.cuda.Stream, raw hipStream_t int, or None for the handle's own."""
+        raw = getattr(stream, "cuda_stream", stream)
+        _chk(load().salmon_hip_set_stream(self.h, C.c_void_p(raw or None)), "set_stream")
+
+    def set_nonlocal(self, s):
+        self._nl = [np.ascontiguousarray(s.mps, np.int32), np.ascontiguousarray(s.jxyz, np.int32),
+                    np.ascontiguousarray(s.rxyz, np.float64), np.ascontiguousarray(s.ia_tbl, np.int32),
+                    np.ascontiguousarray(s.rinv_uvu, np.float64), np.ascontiguousarray(s.uv, np.float64)]
+        mps, jxyz, rxyz, ia, rinv, uv = self._nl
+        _chk(load().salmon_hip_set_nonlocal(self.h, s.natom, s.nps, _p(mps, _ip), _p(jxyz, _ip), _p(rxyz), s.nlma,
+                                            _p(ia, _ip), _p(rinv), _p(uv)), "set_nonlocal")
+
+    def set_vlocal(self, V):
+        v = np.ascontiguousarray(V, dtype=np.float64)
+        _chk(load().salmon_hip_set_vlocal(self.h, _p(v)), "set_vlocal")
+
+    def set_vlocal_device(self, Vdev):
+        _chk(load().salmon_hip_set_vlocal_device(self.h, C.c_void_p(_ptr(Vdev))), "set_vlocal_device")
+
+    def update_kvector(self, vec_ac):
+        a = np.ascontiguousarray(vec_ac, dtype=np.float64)
+        _chk(load().salmon_hip_update_kvector(self.h, _p(a)), "update_kvector")
+
+    def hpsi(self, tpsi, htpsi):
+        _chk(load().salmon_hip_hpsi(self.h, C.c_void_p(_ptr(tpsi)), C.c_void_p(_ptr(htpsi))), "hpsi")
+
+    def taylor(self, zc, tin, tout, work):
+        z = np.ascontiguousarray(np.asarray(zc, dtype=np.complex128)).view(np.float64)
+        _chk(load().salmon_hip_taylor(self.h, _p(z), len(z) // 2, C.c_void_p(_ptr(tin)), C.c_void_p(_ptr(tout)),
+                                      C.c_void_p(_ptr(work))), "taylor")
+
+    def density(self, psi, rho):
+        _chk(load().salmon_hip_density(self.h, C.c_void_p(_ptr(psi)), C.c_void_p(_ptr(rho))), "density")
+
+    def current(self, vec_ac, psi, curr):
+        a = np.ascontiguousarray(vec_ac, dtype=np.float64)
+        _chk(load().salmon_hip_current(self.h, _p(a), C.c_void_p(_ptr(psi)), C.c_void_p(_ptr(curr))), "current")
+
+    def dipole(self, rho, out):
+        _chk(load().salmon_hip_dipole(self.h, C.c_void_p(_ptr(rho)), C.c_void_p(_ptr(out))), "dipole")
+
+    KERNELS = {"hpsi": 0, "hpsi_taylor1": 1, "hpsi_taylor": 2, "hpsi_taylor_last": 3, "proj": 4,
+               "proj_current": 5}
+
+    def prof_start(self, max_launches=100000):
+        _chk(load().salmon_hip_prof_start(self.h, int(max_launches)), "prof_start")
+
+    def prof_read(self, kernel):
+        """(total device ms, launch count) of one kernel kind since prof_start."""
+        ms = C.c_double()
+        n = C.c_int()
+        _chk(load().salmon_hip_prof_stop(self.h, self.KERNELS[kernel], C.byref(ms), C.byref(n)), "prof_stop")
+        return ms.value, n.value
+
+    def prof_end(self):
+        _chk(load().salmon_hip_prof_end(self.h), "prof_end")
+
+    def synchronize(self):
+        _chk(load().salmon_hip_synchronize(self.h), "synchronize")
