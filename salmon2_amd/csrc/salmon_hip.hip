@@ -116,12 +116,16 @@ __global__ __launch_bounds__(kBlock) void k_update_kvector(KvecArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Projector x orbital contraction on FP64 MFMA.
+// Projector x orbital contraction on FP64 MFMA (v_mfma_f64_16x16x4f64).
 // D[n][o] = sum_K A[n][K] B[K][o] with K = (sphere point j, re/im),
-// o = 16 orbitals per wave, n = (lma, weight w, re/im part).
+// o = 16 orbitals per wave, n = (lma-weight index lw, re/im part).
 //   part re:  Re(conj(Z) r_w psi) = Zr r_w psi_re + Zi r_w psi_im
 //   part im:  Im(conj(Z) r_w psi) = Zr r_w psi_im - Zi r_w psi_re
 // r_0 = 1 (plain projection), r_1..3 = rxyz components (current moments).
+// Operand layout: lane L holds A[L%16][L/16] and B[L/16][L%16]; the result
+// lands as D[L/16 + 4r][L%16] in register r.  Rows are therefore assigned as
+// n = 8h + 4p + q (q = 0..3, p = part, h = half) <-> lw = 8t + 4h + q, so the
+// re and im rows of one lw sit in registers (2h, 2h+1) of the same lane.
 // ---------------------------------------------------------------------------
 struct ProjArgs {
   const double2 *psi;     // orbital block base
@@ -138,7 +142,7 @@ struct ProjArgs {
 
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
-  constexpr int MAXT = (2 * NW * kMaxLmaPerAtom + 15) / 16;
+  constexpr int MAXT = (NW * kMaxLmaPerAtom + 7) / 8;
   __shared__ double2 sp[kProjJC][kProjOrb];
   const int grp = blockIdx.x % a.ngrp;
   const int t = blockIdx.x / a.ngrp;
@@ -148,8 +152,8 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
   const int m = a.mps[ia];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int ncol = 2 * NW * al.y;
-  const int ntile = (ncol + 15) >> 4;
+  const int nlw = NW * al.y;            // (lma, weight) pairs of this atom
+  const int ntile = (nlw + 7) >> 3;     // 8 lw (16 rows) per MFMA tile
   const size_t kbase = (size_t)ik * a.nb;
 
   typedef double __attribute__((ext_vector_type(4))) d4;
@@ -192,11 +196,10 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
 #pragma unroll
       for (int tt = 0; tt < MAXT; ++tt) {
         if (tt < ntile) {
-          const int n = tt * 16 + oc;
+          const int part = (oc >> 2) & 1;
+          const int lw = tt * 8 + (oc >> 3) * 4 + (oc & 3);
           double aval = 0.0;
-          if (jv && n < ncol) {
-            const int part = n & 1;
-            const int lw = n >> 1;
+          if (jv && lw < nlw) {
             const int w = lw % NW;
             const int l = lw / NW;
             const double2 z = a.zekr[(size_t)j + (size_t)a.nps * ((size_t)(al.x + l) + (size_t)a.nlma * ik)];
@@ -211,21 +214,20 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
     }
     __syncthreads();
   }
-  // D layout: lane holds D[4*(lane/16) + r][lane%16], r = 0..3
+  // lane (kq, oc) holds rows kq + 4r: r = 2h -> re of lw = 8t + 4h + kq, r = 2h+1 -> its im
   const int ob = grp * kProjOrb + wave * 16 + oc;
   if (ob >= a.nb) return;
 #pragma unroll
   for (int tt = 0; tt < MAXT; ++tt) {
     if (tt >= ntile) break;
 #pragma unroll
-    for (int r = 0; r < 4; r += 2) {
-      const int n = tt * 16 + 4 * kq + r;  // even: re part, n+1: im part
-      if (n >= ncol) continue;
-      const int lw = n >> 1;
+    for (int hh = 0; hh < 2; ++hh) {
+      const int lw = tt * 8 + hh * 4 + kq;
+      if (lw >= nlw) continue;
       const int w = lw % NW;
       const int l = lw / NW;
       const int ilma = al.x + l;
-      const double re = acc[tt][r], im = acc[tt][r + 1];
+      const double re = acc[tt][2 * hh], im = acc[tt][2 * hh + 1];
       if (NW == 1) {
         const double s = a.rinv[ilma];
         a.out[(kbase + ob) * a.nlma + ilma] = make_double2(re * s, im * s);
