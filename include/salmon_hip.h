@@ -103,11 +103,13 @@ int salmon_hip_hpsi(salmon_hip_t h, const void *tpsi, void *htpsi);
 
 /*
  * Taylor propagation tspsi_out = sum_{n=0..n_hamil} zc(n) H^n tspsi_in with
- * zc(0) = 1 (taylor, src/rt/taylor.f90:21-121).  zc: host complex(8)[n_hamil]
- * as interleaved (re, im).  tspsi_in and sshtpsi are work arrays as in the
- * reference: on return tspsi_in holds H^2 tspsi_in when n_hamil >= 3 (the
- * reference leaves H^n_hamil there; the RT driver never reads it,
- * src/rt/main_tddft.f90:91-99), sshtpsi holds an intermediate power.
+ * zc(0) = 1 (taylor, src/rt/taylor.f90:21-121), summed in the reference's
+ * order (((psi + zc1 H psi) + zc2 H^2 psi) + ...).  zc: host complex(8)[n_hamil]
+ * as interleaved (re, im); 1 <= n_hamil <= 6.  Buffer contract, stricter than
+ * the reference's (which leaves H^n_hamil psi in tspsi_in; its RT driver never
+ * reads it, src/rt/main_tddft.f90:91-99): tspsi_in is left unchanged and
+ * sshtpsi holds H tspsi_in on return (n_hamil >= 2).  H^2 .. H^(n_hamil-1)
+ * psi live in scratch owned by the handle (allocated on first use).
  */
 int salmon_hip_taylor(salmon_hip_t h, const double *zc, int n_hamil, void *tspsi_in,
                       void *tspsi_out, void *sshtpsi);
@@ -147,6 +149,7 @@ int salmon_hip_dipole(salmon_hip_t h, const double *rho_dev, double *out_dev);
 #define SALMON_HIP_KERNEL_HPSI_TAYLOR_LAST 3
 #define SALMON_HIP_KERNEL_PROJ 4           /* k_proj, KB projection */
 #define SALMON_HIP_KERNEL_PROJ_CURRENT 5   /* k_proj, nonlocal-current moments */
+#define SALMON_HIP_KERNEL_NL_SCATTER 6     /* k_nl_scatter, KB term at sphere points */
 int salmon_hip_prof_start(salmon_hip_t h, int max_launches);
 int salmon_hip_prof_stop(salmon_hip_t h, int kind, double *total_ms, int *count);
 int salmon_hip_prof_end(salmon_hip_t h);

@@ -9,11 +9,12 @@
 //                     (src/common/nonlocal_potential.f90:360-369) for NW = 1 and the
 //                     four moments of calc_current_nonlocal (density_matrix.f90:532-591)
 //                     for NW = 4
-//   k_hpsi<MODE>      25-point stencil + V_local + k+A gradient
-//                     (zstencil_typical_seq, zstencil_core/typical/seq.f90:17-127),
-//                     the KB scatter of zpseudo (nonlocal_potential.f90:370-378) as a
-//                     deterministic gather over a point->(atom, j) inverse map, and
-//                     the Taylor accumulation of taylor.f90:53-114 in the epilogue
+//   k_stencil<HC,NC>  25-point stencil + V_local + k+A gradient
+//                     (zstencil_typical_seq, zstencil_core/typical/seq.f90:17-127) fed by
+//                     direct global->LDS DMA rings; in the last Taylor term it also forms
+//                     the sum of taylor.f90:53-114
+//   k_nl_scatter      the KB scatter of zpseudo (nonlocal_potential.f90:370-378) as a
+//                     deterministic per-point gather over a point->(atom, j) inverse map
 //   k_density_*       calc_density (density_matrix.f90:150-230)
 //   k_current_*       stencil_current + calc_current (density_matrix.f90:240-530)
 //   k_dipole          calc_dip (src/rt/dip.f90:114-225)
@@ -73,12 +74,15 @@ struct KvecArgs {
   const double *uv;      // (nps, nlma)
   const int *mps;        // (natom)
   const int *lma_atom;   // (nlma) 0-based atom of each projector
+  const int2 *atom_lma;  // (natom): first ilma, count
+  const int *entof;      // (nps, natom): CSR entry of each sphere point
   double2 *zekr;         // (nps, nlma, nk)
+  double2 *zent;         // (cmax, nent, nk)
   double *kpar;          // (16, nk): k_lap0, k_nabt(12), kAc(3)
   double coef_nab[12];
   double coef_lap0;
   double ax, ay, az;
-  int nlma, nps, nk;
+  int nlma, nps, nk, nent, cmax;
 };
 
 __global__ __launch_bounds__(kBlock) void k_update_kvector(KvecArgs a) {
@@ -111,7 +115,10 @@ __global__ __launch_bounds__(kBlock) void k_update_kvector(KvecArgs a) {
     double s, c;
     sincos(th, &s, &c);
     const double u = a.uv[(size_t)j + (size_t)a.nps * ilma];
-    a.zekr[(size_t)j + (size_t)a.nps * ((size_t)ilma + (size_t)a.nlma * ik)] = make_double2(c * u, -s * u);
+    const double2 z = make_double2(c * u, -s * u);
+    a.zekr[(size_t)j + (size_t)a.nps * ((size_t)ilma + (size_t)a.nlma * ik)] = z;
+    const int e = a.entof[(size_t)j + (size_t)a.nps * ia];
+    a.zent[(size_t)(ilma - a.atom_lma[ia].x) + (size_t)a.cmax * ((size_t)e + (size_t)a.nent * ik)] = z;
   }
 }
 
@@ -135,7 +142,7 @@ struct ProjArgs {
   const int *mps;         // (natom)
   const int2 *atom_lma;   // (natom): first ilma, count
   const double *rinv;     // (nlma)
-  double2 *out;           // NW=1: c(nlma, nb, nk) scaled by rinv; NW=4: (4, nlma, nb, nk) raw
+  double2 *out;           // NW=1: c(nb, nlma, nk) scaled by rinv; NW=4: (4, nlma, nb, nk) raw
   size_t osz;
   int nb, nk, natom, nps, nlma, ngrp;
 };
@@ -230,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
       const double re = acc[tt][2 * hh], im = acc[tt][2 * hh + 1];
       if (NW == 1) {
         const double s = a.rinv[ilma];
-        a.out[(kbase + ob) * a.nlma + ilma] = make_double2(re * s, im * s);
+        a.out[((size_t)ik * a.nlma + ilma) * a.nb + ob] = make_double2(re * s, im * s);
       } else {
         a.out[((kbase + ob) * a.nlma + ilma) * 4 + w] = make_double2(re, im);
       }
@@ -238,46 +245,84 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
   }
 }
 
+constexpr int kMaxComb = 5;   // Taylor order up to 6
+
 // ---------------------------------------------------------------------------
-// Fused hpsi: stencil + V + k.grad + KB scatter + Taylor accumulation.
-// Block = full x rows (nx) x TY rows in y, streaming z over [z0, z0+TZ) with a
-// 9-deep register queue along z; x and y neighbours come from an LDS plane
-// (double-buffered) that carries 4 halo rows on either side in y.
-// MODE 0: hout = H in
-// MODE 1: hout = H in;   acc = in + zc * hout        (first Taylor term)
-// MODE 2: hout = H in;   acc = acc + zc * hout       (middle terms)
-// MODE 3:                acc = acc + zc * (H in)     (last term, H^n not stored)
+// Local hpsi: 25-point stencil + V + k.grad (zstencil_typical_seq,
+// zstencil_core/typical/seq.f90:17-127), with the Taylor combination folded
+// into the last term.  The KB term is added afterwards by k_nl_scatter.
+//
+// Block = full x rows (nx) x TY rows in y (one point per thread), streaming z
+// over [z0, z0+TZ).  Every per-plane input arrives by direct global->LDS DMA
+// (global_load_lds, issued through inline asm so that only explicit
+// s_waitcnt vmcnt(N) waits gate it) kDmaDepth planes ahead:
+//   tile ring  (kRingT slots): psi rows of planes z .. z+4+D  (x/y neighbours, queue head)
+//   halo ring  (D+1 slots):    the 4+4 y-halo rows of planes z .. z+D
+//   V ring     (D+1 slots):    V_local of planes z .. z+D
+//   comb ring  (2 slots, last term only): psi, h1 .. h_{N-2} of planes z, z+1
+// z neighbours come from a 9-deep register queue fed from the tile ring.  The
+// only compiler-visible global memory operation in the loop is the store, so
+// the in-order vmcnt counter never forces an early wait on the ring.
+// NC < 0:  hout = H_loc in
+// NC >= 0: hout = (((psi + c1 h1) + ... + c_{N-1} in) + c_N H_loc in), N = NC + 1
 // ---------------------------------------------------------------------------
-struct HpsiArgs {
+constexpr int kDmaDepth = 2;
+constexpr int kRingT = 5 + kDmaDepth;
+constexpr int kRingH = kDmaDepth + 1;
+constexpr int kTile = 256;  // tile slot size (elements), = 4 waves x 64 lanes
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ unsigned lds_addr(const void *p) {
+  return (unsigned)(uintptr_t)((lds_void *)p);
+}
+// lane i of the wave writes its element to LDS byte address m0 + i * size
+// (M0 is not used by compiler-generated gfx950 code in these kernels: LDS
+// instructions do not read it)
+__device__ __forceinline__ void dma16(const void *g, unsigned m0) {
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(g) : "memory");
+}
+__device__ __forceinline__ void dma4(const void *g, unsigned m0) {
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dword %1, off" ::"s"(m0), "v"(g) : "memory");
+}
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+struct StencilArgs {
   const double2 *in;
   double2 *hout;
-  double2 *acc;
+  double2 *sink;          // kTile elements absorbing the stores of lanes without a point
+  const double2 *comb[kMaxComb];
+  double2 zc[kMaxComb + 1];
   const double *V;        // (nx, ny, nz, nspin)
   const double *kpar;     // (16, nk)
-  const double2 *c;       // (nlma, nb, nk) projections (rinv applied)
-  const double2 *zekr;    // (nps, nlma, nk)
-  const int *pt_off;      // (ngrid + 1)
-  const int2 *pt_ent;     // (ilma0 | cnt << 24, j)
   double lap[12];         // coef_lap(4,3)
-  double2 zc;
   size_t osz;
-  int nx, ny, nz, ly, nspin, nb, nk, nlma, nps;
+  int nx, ny, nz, ly, nspin, nb, nk;
   int TY, TZ, nyt, nzt, nblk;
-  int has_nl;
 };
 
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_hpsi(HpsiArgs a) {
+template <int HC, int NC>
+__global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
+  constexpr bool FINAL = NC >= 0;
+  constexpr int D = kDmaDepth;
+  // DMA instructions per wave per plane, in issue order: comb (NC), tile, halo (HC), V (2 x dword);
+  // then 1 store
+  constexpr int kOpsAfterComb = 1 + HC + 2 + 1;
   extern __shared__ double2 lds[];
-  // XCD-aware block order: blocks dispatched to one XCD (bid % 8) get a
-  // contiguous range of logical tiles, so the y-neighbour tiles of one orbital
-  // share that XCD's L2.
+  double2 *tileR = lds;                                   // kRingT x kTile
+  double2 *haloR = tileR + kRingT * kTile;                // kRingH x HC*kTile
+  double *vR = (double *)(haloR + kRingH * HC * kTile);   // kRingH x kTile doubles
+  double2 *combR = (double2 *)(vR + kRingH * kTile);      // 2 x NC x kTile
+
   int bid = blockIdx.x;
   const int nfull = (a.nblk / 8) * 8;
-  if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;
+  if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;  // XCD-aware tile order
   const int ytile = bid % a.nyt;
   const int zt = (bid / a.nyt) % a.nzt;
-  const int b = bid / (a.nyt * a.nzt);  // global orbital index (ispin fastest)
+  const int b = bid / (a.nyt * a.nzt);
   const int ik = b / a.nb;
   const int ispin = b % a.nspin;
 
@@ -287,148 +332,252 @@ __global__ __launch_bounds__(kBlock) void k_hpsi(HpsiArgs a) {
   const int z0 = zt * a.TZ;
   const int z1 = min(z0 + a.TZ, nz);
   const int tid = threadIdx.x;
-  const int tx = tid % nx;
-  const int ty = tid / nx;
-  const bool inTile = ty < TY;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int ntile = TY * nx;
+  const bool active = tid < ntile;
+  const int te = active ? tid : 0;          // element this lane loads (dummy lanes repeat element 0)
+  const int tx = te % nx;
+  const int ty = te / nx;
   const int y = y0 + ty;
-  const bool outY = inTile && y < ny;
+  const bool outY = active && y < ny;
   const int yw = wrapi(y, ny);
-  const int rowW = nx;
-  const int planeL = (TY + 8) * rowW;
 
-  const double2 *in = a.in + (size_t)b * a.osz;
   const size_t zs = (size_t)nx * ly;
+  const size_t orbOff = (size_t)b * a.osz;
+  const double2 *in = a.in + orbOff;
   const size_t colOff = (size_t)tx + (size_t)nx * yw;
+  const size_t gplane = (size_t)nx * ny;
+  // V: two dword DMAs per wave, each covering 32 points of the wave's 64 (lane i -> point i/2, half i%2)
+  const double *Vb = a.V + (size_t)ispin * gplane * nz;
+  size_t voff[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int e = 64 * wave + 32 * k + (lane >> 1);
+    const int ec = e < ntile ? e : 0;
+    voff[k] = 2 * ((size_t)(ec % nx) + (size_t)nx * wrapi(y0 + ec / nx, ny)) + (lane & 1);
+  }
 
+  // halo elements of this lane: chunk c = wave + 4k, element 64c + lane
+  size_t hoff[HC];
+#pragma unroll
+  for (int k = 0; k < HC; ++k) {
+    const int e = 64 * (wave + 4 * k) + lane;
+    const int ec = e < 8 * nx ? e : 0;
+    const int hr = ec / nx, hx = ec % nx;
+    const int gy = hr < 4 ? y0 - 4 + hr : y0 + TY + (hr - 4);
+    hoff[k] = (size_t)hx + (size_t)nx * wrapi(gy, ny);
+  }
+  // LDS byte addresses of this wave's chunk in each ring (uniform per wave)
+  const unsigned tileA = __builtin_amdgcn_readfirstlane(lds_addr(tileR) + 16u * 64u * wave);
+  const unsigned haloA = __builtin_amdgcn_readfirstlane(lds_addr(haloR) + 16u * 64u * wave);
+  const unsigned vA = __builtin_amdgcn_readfirstlane(lds_addr(vR) + 8u * 64u * wave);
+  const unsigned combA = __builtin_amdgcn_readfirstlane(lds_addr(combR) + 16u * 64u * wave);
+
+  auto dma_tile = [&](int plane, int slot) {
+    dma16(in + colOff + zs * wrapi(plane, nz), tileA + 16u * kTile * slot);
+  };
+  auto dma_halo = [&](int plane, int slot) {
+#pragma unroll
+    for (int k = 0; k < HC; ++k)
+      dma16(in + hoff[k] + zs * wrapi(plane, nz), haloA + 16u * (HC * kTile * slot + 4 * 64 * k));
+  };
+  auto dma_v = [&](int plane, int slot) {
+    const float *vb = (const float *)Vb + 2 * gplane * wrapi(plane, nz);
+    dma4(vb + voff[0], vA + 8u * kTile * slot);
+    dma4(vb + voff[1], vA + 8u * kTile * slot + 256u);
+  };
+  auto dma_comb = [&](int plane, int slot) {
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+      dma16(a.comb[n] + orbOff + colOff + zs * wrapi(plane, nz), combA + 16u * kTile * (NC * slot + n));
+  };
+
+  // y-neighbour LDS offsets: rows ty+n (n = 1..4, -1..-4) in the tile slot or the halo slot
+  int yoff[8];
+  bool yhal[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int n = i < 4 ? i + 1 : -(i - 3);
+    const int r = ty + n;
+    if (r >= 0 && r < TY) {
+      yoff[i] = r * nx + tx;
+      yhal[i] = false;
+    } else {
+      yoff[i] = (r < 0 ? r + 4 : 4 + r - TY) * nx + tx;
+      yhal[i] = true;
+    }
+  }
+  int xo[8];
+#pragma unroll
+  for (int n = 1; n <= 4; ++n) {
+    xo[n - 1] = ty * nx + wrapi(tx + n, nx);
+    xo[3 + n] = ty * nx + wrapi(tx - n, nx);
+  }
   const double *kp = a.kpar + 16 * ik;
   const double klap0 = kp[0];
   double nab[12];
 #pragma unroll
   for (int i = 0; i < 12; ++i) nab[i] = kp[1 + i];
 
-  int xo[8];
-#pragma unroll
-  for (int n = 1; n <= 4; ++n) {
-    xo[n - 1] = wrapi(tx + n, nx);
-    xo[3 + n] = wrapi(tx - n, nx);
-  }
-
-  // halo elements of this thread: e = tid + kBlock*h over 8*nx elements
-  const int nhalo = 8 * nx;
-
+  // prologue: queue planes z0-4 .. z0+3 (plain loads), rings for planes z0 .. z0+3+D
   double2 q[9];
-  if (inTile) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) q[i] = in[colOff + zs * wrapi(z0 - 4 + i, nz)];
+  for (int i = 0; i < 8; ++i) q[i] = in[colOff + zs * wrapi(z0 - 4 + i, nz)];
+  __builtin_amdgcn_s_waitcnt(0);  // compiler-visible: the queue loads are complete
+  if (FINAL) dma_comb(z0, z0 & 1);
+  for (int p = 0; p < 4 + D; ++p) dma_tile(z0 + p, (z0 + p) % kRingT);
+  for (int p = 0; p < D; ++p) {
+    dma_halo(z0 + p, (z0 + p) % kRingH);
+    dma_v(z0 + p, (z0 + p) % kRingH);
   }
-  double2 nxt = make_double2(0.0, 0.0);
-  if (inTile) nxt = in[colOff + zs * wrapi(z0 + 4, nz)];
-  double2 hal[4];
-#pragma unroll
-  for (int h = 0; h < 4; ++h) {
-    const int e = tid + kBlock * h;
-    if (e < nhalo) {
-      const int hr = e / nx, hx = e % nx;
-      const int lr = hr < 4 ? hr : TY + hr;
-      hal[h] = in[(size_t)hx + (size_t)nx * wrapi(y0 - 4 + lr, ny) + zs * wrapi(z0, nz)];
-    }
-  }
+  wait_vm<0>();
 
-  int buf = 0;
+  int sT = z0 % kRingT, sH = z0 % kRingH;  // ring slots of plane z
   for (int z = z0; z < z1; ++z) {
-    double2 *P = lds + buf * planeL;
-    if (inTile) {
-      q[8] = nxt;
-      P[(ty + 4) * rowW + tx] = q[4];
-    }
-#pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      const int e = tid + kBlock * h;
-      if (e < nhalo) {
-        const int hr = e / nx, hx = e % nx;
-        const int lr = hr < 4 ? hr : TY + hr;
-        P[lr * rowW + hx] = hal[h];
-      }
-    }
+    // planes z (halo, V, comb) and z+4 (tile) were issued >= one full plane ago
+    wait_vm<kOpsAfterComb>();
     __syncthreads();
-    // prefetch the next plane's queue entry and halo
-    if (z + 1 < z1) {
-      if (inTile) nxt = in[colOff + zs * wrapi(z + 5, nz)];
-#pragma unroll
-      for (int h = 0; h < 4; ++h) {
-        const int e = tid + kBlock * h;
-        if (e < nhalo) {
-          const int hr = e / nx, hx = e % nx;
-          const int lr = hr < 4 ? hr : TY + hr;
-          hal[h] = in[(size_t)hx + (size_t)nx * wrapi(y0 - 4 + lr, ny) + zs * wrapi(z + 1, nz)];
-        }
-      }
-    }
-    if (outY) {
-      const double2 *row = P + (ty + 4) * rowW;
-      double2 v, w;
-      // x axis
-      {
-        double2 p1 = row[xo[0]], p2 = row[xo[1]], p3 = row[xo[2]], p4 = row[xo[3]];
-        double2 m1 = row[xo[4]], m2 = row[xo[5]], m3 = row[xo[6]], m4 = row[xo[7]];
-        v.x = a.lap[0] * (p1.x + m1.x) + a.lap[1] * (p2.x + m2.x) + a.lap[2] * (p3.x + m3.x) + a.lap[3] * (p4.x + m4.x);
-        v.y = a.lap[0] * (p1.y + m1.y) + a.lap[1] * (p2.y + m2.y) + a.lap[2] * (p3.y + m3.y) + a.lap[3] * (p4.y + m4.y);
-        w.x = nab[0] * (p1.x - m1.x) + nab[1] * (p2.x - m2.x) + nab[2] * (p3.x - m3.x) + nab[3] * (p4.x - m4.x);
-        w.y = nab[0] * (p1.y - m1.y) + nab[1] * (p2.y - m2.y) + nab[2] * (p3.y - m3.y) + nab[3] * (p4.y - m4.y);
-      }
-      // y axis
-      {
-        double2 p1 = P[(ty + 5) * rowW + tx], p2 = P[(ty + 6) * rowW + tx];
-        double2 p3 = P[(ty + 7) * rowW + tx], p4 = P[(ty + 8) * rowW + tx];
-        double2 m1 = P[(ty + 3) * rowW + tx], m2 = P[(ty + 2) * rowW + tx];
-        double2 m3 = P[(ty + 1) * rowW + tx], m4 = P[(ty + 0) * rowW + tx];
-        v.x += a.lap[4] * (p1.x + m1.x) + a.lap[5] * (p2.x + m2.x) + a.lap[6] * (p3.x + m3.x) + a.lap[7] * (p4.x + m4.x);
-        v.y += a.lap[4] * (p1.y + m1.y) + a.lap[5] * (p2.y + m2.y) + a.lap[6] * (p3.y + m3.y) + a.lap[7] * (p4.y + m4.y);
-        w.x += nab[4] * (p1.x - m1.x) + nab[5] * (p2.x - m2.x) + nab[6] * (p3.x - m3.x) + nab[7] * (p4.x - m4.x);
-        w.y += nab[4] * (p1.y - m1.y) + nab[5] * (p2.y - m2.y) + nab[6] * (p3.y - m3.y) + nab[7] * (p4.y - m4.y);
-      }
-      // z axis from the register queue (q[4] is the centre)
-      v.x += a.lap[8] * (q[5].x + q[3].x) + a.lap[9] * (q[6].x + q[2].x) + a.lap[10] * (q[7].x + q[1].x) + a.lap[11] * (q[8].x + q[0].x);
-      v.y += a.lap[8] * (q[5].y + q[3].y) + a.lap[9] * (q[6].y + q[2].y) + a.lap[10] * (q[7].y + q[1].y) + a.lap[11] * (q[8].y + q[0].y);
-      w.x += nab[8] * (q[5].x - q[3].x) + nab[9] * (q[6].x - q[2].x) + nab[10] * (q[7].x - q[1].x) + nab[11] * (q[8].x - q[0].x);
-      w.y += nab[8] * (q[5].y - q[3].y) + nab[9] * (q[6].y - q[2].y) + nab[10] * (q[7].y - q[1].y) + nab[11] * (q[8].y - q[0].y);
+    const int sTn = (sT + 4 + D) % kRingT;  // slot of plane z+4+D (= slot of z-1)
+    const int sHn = (sH + D) % kRingH;      // slot of plane z+D   (= slot of z-1)
+    if (FINAL) dma_comb(z + 1, (z + 1) & 1);
+    dma_tile(z + 4 + D, sTn);
+    dma_halo(z + D, sHn);
+    dma_v(z + D, sHn);
 
-      const double2 t = q[4];
-      const size_t g = (size_t)tx + (size_t)nx * ((size_t)y + (size_t)ny * z);
-      const double vl = a.V[g + (size_t)ispin * ((size_t)nx * ny * nz)];
-      // V t + k_lap0 t - v/2 - i w
-      double2 h;
-      h.x = (vl * t.x + klap0 * t.x) - 0.5 * v.x + w.y;
-      h.y = (vl * t.y + klap0 * t.y) - 0.5 * v.y - w.x;
-      if (a.has_nl) {
-        const int e0 = a.pt_off[g], e1 = a.pt_off[g + 1];
-        for (int e = e0; e < e1; ++e) {
-          const int2 en = a.pt_ent[e];
-          const int ilma0 = en.x & 0xffffff;
-          const int cnt = en.x >> 24;
-          const double2 *cc = a.c + ((size_t)ik * a.nb + (b % a.nb)) * a.nlma + ilma0;
-          const double2 *zz = a.zekr + (size_t)en.y + (size_t)a.nps * ((size_t)ilma0 + (size_t)a.nlma * ik);
-          for (int l = 0; l < cnt; ++l) {
-            const double2 cv = cc[l];
-            const double2 zv = zz[(size_t)a.nps * l];
-            h.x += cv.x * zv.x - cv.y * zv.y;
-            h.y += cv.x * zv.y + cv.y * zv.x;
-          }
-        }
-      }
-      const size_t oi = (size_t)b * a.osz + colOff + zs * z;
-      if (MODE != 3) a.hout[oi] = h;
-      if (MODE == 1) {
-        a.acc[oi] = make_double2(t.x + (a.zc.x * h.x - a.zc.y * h.y), t.y + (a.zc.x * h.y + a.zc.y * h.x));
-      } else if (MODE >= 2) {
-        const double2 o = a.acc[oi];
-        a.acc[oi] = make_double2(o.x + (a.zc.x * h.x - a.zc.y * h.y), o.y + (a.zc.x * h.y + a.zc.y * h.x));
-      }
+    const double2 *T = tileR + kTile * sT;
+    const double2 *H = haloR + HC * kTile * sH;
+    q[8] = tileR[kTile * ((sT + 4) % kRingT) + te];
+    double2 v, w;
+    {
+      const double2 p1 = T[xo[0]], p2 = T[xo[1]], p3 = T[xo[2]], p4 = T[xo[3]];
+      const double2 m1 = T[xo[4]], m2 = T[xo[5]], m3 = T[xo[6]], m4 = T[xo[7]];
+      v.x = a.lap[0] * (p1.x + m1.x) + a.lap[1] * (p2.x + m2.x) + a.lap[2] * (p3.x + m3.x) + a.lap[3] * (p4.x + m4.x);
+      v.y = a.lap[0] * (p1.y + m1.y) + a.lap[1] * (p2.y + m2.y) + a.lap[2] * (p3.y + m3.y) + a.lap[3] * (p4.y + m4.y);
+      w.x = nab[0] * (p1.x - m1.x) + nab[1] * (p2.x - m2.x) + nab[2] * (p3.x - m3.x) + nab[3] * (p4.x - m4.x);
+      w.y = nab[0] * (p1.y - m1.y) + nab[1] * (p2.y - m2.y) + nab[2] * (p3.y - m3.y) + nab[3] * (p4.y - m4.y);
     }
-    // advance the z queue
+    {
+      const double2 p1 = (yhal[0] ? H : T)[yoff[0]], p2 = (yhal[1] ? H : T)[yoff[1]];
+      const double2 p3 = (yhal[2] ? H : T)[yoff[2]], p4 = (yhal[3] ? H : T)[yoff[3]];
+      const double2 m1 = (yhal[4] ? H : T)[yoff[4]], m2 = (yhal[5] ? H : T)[yoff[5]];
+      const double2 m3 = (yhal[6] ? H : T)[yoff[6]], m4 = (yhal[7] ? H : T)[yoff[7]];
+      v.x += a.lap[4] * (p1.x + m1.x) + a.lap[5] * (p2.x + m2.x) + a.lap[6] * (p3.x + m3.x) + a.lap[7] * (p4.x + m4.x);
+      v.y += a.lap[4] * (p1.y + m1.y) + a.lap[5] * (p2.y + m2.y) + a.lap[6] * (p3.y + m3.y) + a.lap[7] * (p4.y + m4.y);
+      w.x += nab[4] * (p1.x - m1.x) + nab[5] * (p2.x - m2.x) + nab[6] * (p3.x - m3.x) + nab[7] * (p4.x - m4.x);
+      w.y += nab[4] * (p1.y - m1.y) + nab[5] * (p2.y - m2.y) + nab[6] * (p3.y - m3.y) + nab[7] * (p4.y - m4.y);
+    }
+    v.x += a.lap[8] * (q[5].x + q[3].x) + a.lap[9] * (q[6].x + q[2].x) + a.lap[10] * (q[7].x + q[1].x) + a.lap[11] * (q[8].x + q[0].x);
+    v.y += a.lap[8] * (q[5].y + q[3].y) + a.lap[9] * (q[6].y + q[2].y) + a.lap[10] * (q[7].y + q[1].y) + a.lap[11] * (q[8].y + q[0].y);
+    w.x += nab[8] * (q[5].x - q[3].x) + nab[9] * (q[6].x - q[2].x) + nab[10] * (q[7].x - q[1].x) + nab[11] * (q[8].x - q[0].x);
+    w.y += nab[8] * (q[5].y - q[3].y) + nab[9] * (q[6].y - q[2].y) + nab[10] * (q[7].y - q[1].y) + nab[11] * (q[8].y - q[0].y);
+
+    const double2 t = q[4];
+    const double vcur = vR[kTile * sH + te];
+    double2 h;
+    h.x = (vcur * t.x + klap0 * t.x) - 0.5 * v.x + w.y;
+    h.y = (vcur * t.y + klap0 * t.y) - 0.5 * v.y - w.x;
+    double2 o = h;
+    if (FINAL) {
+      constexpr int N = NC + 1;
+      const double2 *C = combR + kTile * (NC * (z & 1));
+      if (N == 1) {
+        o = t;
+      } else {
+        o = C[te];
+#pragma unroll
+        for (int n = 1; n < N - 1; ++n) {
+          const double2 cc = C[kTile * n + te];
+          const double2 zc = a.zc[n - 1];
+          o.x = o.x + (zc.x * cc.x - zc.y * cc.y);
+          o.y = o.y + (zc.x * cc.y + zc.y * cc.x);
+        }
+        const double2 zc = a.zc[N > 1 ? N - 2 : 0];
+        o.x = o.x + (zc.x * t.x - zc.y * t.y);
+        o.y = o.y + (zc.x * t.y + zc.y * t.x);
+      }
+      const double2 zc = a.zc[N - 1];
+      o.x = o.x + (zc.x * h.x - zc.y * h.y);
+      o.y = o.y + (zc.x * h.y + zc.y * h.x);
+    }
+    // the store is the one compiler-visible vm op per plane (counted in kOpsAfterComb):
+    // every lane stores, lanes without a point into the sink, so each wave issues it
+    double2 *dst = outY ? a.hout + orbOff + colOff + zs * z : a.sink + tid;
+    *dst = o;
 #pragma unroll
     for (int i = 0; i < 8; ++i) q[i] = q[i + 1];
-    buf ^= 1;
+    sT = (sT + 1 == kRingT) ? 0 : sT + 1;
+    sH = (sH + 1 == kRingH) ? 0 : sH + 1;
+  }
+  wait_vm<0>();  // drain the look-ahead DMAs before the LDS is released
+}
+
+// ---------------------------------------------------------------------------
+// KB term of hpsi at the sphere points only (zpseudo's scatter,
+// nonlocal_potential.f90:370-378), as a deterministic per-point gather over
+// the point -> (atom, j) inverse map, entries in ascending ilma:
+//   hout(p) += s * sum_e sum_l c(ilma0_e + l) zekr(j_e, ilma0_e + l)
+// with s = 1 (plain / intermediate terms) or s = c_N (last Taylor term).
+// One thread per touched point and kScatG orbitals of one k-point: the entry
+// tables are read once and reused, the orbitals' read-modify-writes overlap.
+// ---------------------------------------------------------------------------
+constexpr int kScatG = 8;
+
+struct ScatterArgs {
+  double2 *hout;
+  const double2 *c;       // (nb, nlma, nk), orbital fastest
+  const double2 *zent;    // (cmax, nent, nk)
+  const int *tp_g;        // (ntp) unpadded index of each touched point
+  const int *tp_off;      // (ntp) orbital-block offset of each touched point
+  const int *pt_off;      // (ngrid + 1)
+  const int *pt_lma;      // (nent) ilma0 | cnt << 24
+  double2 s;
+  size_t osz;
+  int ntp, nb, nlma, nent, cmax, ngrp;
+};
+
+__global__ __launch_bounds__(kBlock) void k_nl_scatter(ScatterArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int ik = blockIdx.y / a.ngrp;
+  const int b0 = (blockIdx.y % a.ngrp) * kScatG;
+  if (i >= a.ntp) return;
+  const int g = a.tp_g[i];
+  const int e0 = a.pt_off[g], e1 = a.pt_off[g + 1];
+  const int nob = min(kScatG, a.nb - b0);
+  const double2 *zb = a.zent + (size_t)ik * a.nent * a.cmax;
+  const double2 *cb = a.c + (size_t)ik * a.nlma * a.nb + b0;
+  double2 acc[kScatG];
+#pragma unroll
+  for (int o = 0; o < kScatG; ++o) acc[o] = make_double2(0.0, 0.0);
+  for (int e = e0; e < e1; ++e) {
+    const int lc = a.pt_lma[e];
+    const int ilma0 = lc & 0xffffff;
+    const int cnt = lc >> 24;
+    for (int l = 0; l < cnt; ++l) {
+      const double2 zv = zb[(size_t)e * a.cmax + l];
+      const double2 *cl = cb + (size_t)(ilma0 + l) * a.nb;
+#pragma unroll
+      for (int o = 0; o < kScatG; ++o) {
+        if (o < nob) {
+          const double2 cv = cl[o];
+          acc[o].x += cv.x * zv.x - cv.y * zv.y;
+          acc[o].y += cv.x * zv.y + cv.y * zv.x;
+        }
+      }
+    }
+  }
+  double2 *p = a.hout + ((size_t)ik * a.nb + b0) * a.osz + a.tp_off[i];
+  double2 old[kScatG];
+#pragma unroll
+  for (int o = 0; o < kScatG; ++o)
+    if (o < nob) old[o] = p[(size_t)o * a.osz];
+#pragma unroll
+  for (int o = 0; o < kScatG; ++o) {
+    if (o < nob) {
+      double2 v = old[o];
+      v.x += a.s.x * acc[o].x - a.s.y * acc[o].y;
+      v.y += a.s.x * acc[o].y + a.s.y * acc[o].x;
+      p[(size_t)o * a.osz] = v;
+    }
   }
 }
 
@@ -565,10 +714,30 @@ struct CurFinArgs {
   int nslab, nspin, no, nk, nlma;
 };
 
-__global__ __launch_bounds__(kBlock) void k_current_final(CurFinArgs a, int ispin) {
+// One wave per orbital (ik, io) of spin ispin: its weighted current
+// ((j1 + j2) + jNL) rocc wtk (calc_current, density_matrix.f90:405-450), with
+// jNL = 2 sum_lma Im(conj(uVpsi_r) uVpsi) (calc_current_nonlocal :532-591)
+// reduced across the wave in a fixed tree.
+__global__ __launch_bounds__(kBlock) void k_current_orb(CurFinArgs a, int ispin) {
   const int M = a.no * a.nk;
-  for (int mm = threadIdx.x; mm < M; mm += kBlock) {
-    const int io = mm % a.no, ik = mm / a.no;
+  const int mm = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (mm >= M) return;
+  const int io = mm % a.no, ik = mm / a.no;
+  double jw[3] = {0.0, 0.0, 0.0};
+  if (a.nl) {
+    const int nb = a.no * a.nspin;
+    const size_t ob = (size_t)ik * nb + (size_t)(ispin + a.nspin * io);
+    for (int l = lane; l < a.nlma; l += 64) {
+      const double2 *u4 = a.nl + (ob * a.nlma + l) * 4;
+      const double rv = a.rinv[l];
+      const double ur = u4[0].x * rv, ui = u4[0].y * rv;
+      for (int d = 0; d < 3; ++d) jw[d] += u4[1 + d].x * ui - u4[1 + d].y * ur;
+    }
+    for (int d = 0; d < 3; ++d)
+      for (int off = 32; off > 0; off >>= 1) jw[d] += __shfl_down(jw[d], off, 64);
+  }
+  if (lane == 0) {
     const size_t b = (size_t)ispin + (size_t)a.nspin * mm;
     double rt = 0.0, s[3] = {0.0, 0.0, 0.0};
     for (int sl = 0; sl < a.nslab; ++sl) {
@@ -578,30 +747,19 @@ __global__ __launch_bounds__(kBlock) void k_current_final(CurFinArgs a, int ispi
       s[1] = s[1] + o[2];
       s[2] = s[2] + o[3];
     }
-    double jw[3] = {0.0, 0.0, 0.0};
-    if (a.nl) {
-      const int nb = a.no * a.nspin;
-      const size_t ob = (size_t)ik * nb + (size_t)(ispin + a.nspin * io);
-      for (int l = 0; l < a.nlma; ++l) {
-        const double2 *u4 = a.nl + (ob * a.nlma + l) * 4;
-        const double rv = a.rinv[l];
-        const double ur = u4[0].x * rv, ui = u4[0].y * rv;
-        for (int d = 0; d < 3; ++d) {
-          // aimag(conjg(uVpsi_r) * uVpsi)
-          jw[d] = jw[d] + (u4[1 + d].x * ui - u4[1 + d].y * ur);
-        }
-      }
-      for (int d = 0; d < 3; ++d) jw[d] = jw[d] * 2.0;
-    }
     const double kA[3] = {a.veck[3 * ik] + a.ax, a.veck[3 * ik + 1] + a.ay, a.veck[3 * ik + 2] + a.az};
     const double occ = a.rocc[io + (size_t)a.no * (ik + (size_t)a.nk * ispin)];
     for (int d = 0; d < 3; ++d) {
       const double j1 = kA[d] * rt;
       const double j2 = s[d] * 2.0;
-      a.contrib[3 * (size_t)mm + d] = (((j1 + j2) + jw[d]) * occ) * a.wtk[ik];
+      a.contrib[3 * (size_t)mm + d] = (((j1 + j2) + jw[d] * 2.0) * occ) * a.wtk[ik];
     }
   }
-  __syncthreads();
+}
+
+// Sum of the orbital contributions in (ik, io) order, divided by N_grid.
+__global__ __launch_bounds__(64) void k_current_sum(CurFinArgs a, int ispin) {
+  const int M = a.no * a.nk;
   if (threadIdx.x < 3) {
     const int d = threadIdx.x;
     double w4 = 0.0;
@@ -678,10 +836,16 @@ struct salmon_hip_ctx {
   double *d_veck = nullptr, *d_rocc = nullptr, *d_wtk = nullptr, *d_V = nullptr, *d_kpar = nullptr;
   // nonlocal
   int natom = 0, nps = 0, nlma = 0, nent = 0;
+  int cmax = 0;
   int *d_mps = nullptr, *d_jidx = nullptr, *d_lma_atom = nullptr, *d_pt_off = nullptr;
-  int2 *d_atom_lma = nullptr, *d_pt_ent = nullptr;
+  int *d_pt_lma = nullptr, *d_entof = nullptr, *d_tp_g = nullptr, *d_tp_off = nullptr;
+  int ntp = 0;
+  double2 *d_sink = nullptr;
+  int2 *d_atom_lma = nullptr;
   double *d_rxyz = nullptr, *d_uv = nullptr, *d_rinv = nullptr;
-  double2 *d_zekr = nullptr, *d_c = nullptr, *d_curnl = nullptr;
+  double2 *d_zekr = nullptr, *d_zent = nullptr, *d_c = nullptr, *d_curnl = nullptr;
+  // Taylor scratch: H^2 psi .. H^(N-2) psi (allocated on first use)
+  std::vector<double2 *> tscr;
   // observables scratch
   double *d_curpart = nullptr, *d_contrib = nullptr, *d_denspart = nullptr;
   int dens_ngrp = 1;
@@ -765,6 +929,7 @@ int salmon_hip_create(int nx, int ny, int nz, int ly, int nspin, int no, int nk,
   if (c->dens_ngrp > 1 && (rc = dalloc(&c->d_denspart, c->ngrid * nspin * c->dens_ngrp))) return bail(rc);
   if ((rc = dalloc(&c->d_curpart, 4 * (size_t)kCurSlabs * c->norb))) return bail(rc);
   if ((rc = dalloc(&c->d_contrib, 3 * (size_t)no * nk))) return bail(rc);
+  if ((rc = dalloc(&c->d_sink, kTile))) return bail(rc);
   *out = c;
   return 0;
 }
@@ -775,9 +940,11 @@ int salmon_hip_destroy(salmon_hip_t c) {
   if (c->stream && c->stream != c->own) (void)hipStreamSynchronize(c->stream);
   dfree(c->d_veck); dfree(c->d_rocc); dfree(c->d_wtk); dfree(c->d_V); dfree(c->d_kpar);
   dfree(c->d_mps); dfree(c->d_jidx); dfree(c->d_lma_atom); dfree(c->d_pt_off);
-  dfree(c->d_atom_lma); dfree(c->d_pt_ent); dfree(c->d_rxyz); dfree(c->d_uv); dfree(c->d_rinv);
-  dfree(c->d_zekr); dfree(c->d_c); dfree(c->d_curnl);
+  dfree(c->d_atom_lma); dfree(c->d_pt_lma); dfree(c->d_entof); dfree(c->d_rxyz); dfree(c->d_uv);
+  dfree(c->d_rinv); dfree(c->d_zekr); dfree(c->d_zent); dfree(c->d_c); dfree(c->d_curnl);
+  dfree(c->d_tp_g); dfree(c->d_tp_off); dfree(c->d_sink);
   dfree(c->d_curpart); dfree(c->d_contrib); dfree(c->d_denspart);
+  for (double2 *&p : c->tscr) dfree(p);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
@@ -798,9 +965,10 @@ int salmon_hip_set_nonlocal(salmon_hip_t c, int natom, int nps, const int *mps, 
   if (int rc = check(c)) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   dfree(c->d_mps); dfree(c->d_jidx); dfree(c->d_lma_atom); dfree(c->d_pt_off);
-  dfree(c->d_atom_lma); dfree(c->d_pt_ent); dfree(c->d_rxyz); dfree(c->d_uv); dfree(c->d_rinv);
-  dfree(c->d_zekr); dfree(c->d_c); dfree(c->d_curnl);
-  c->natom = c->nps = c->nlma = c->nent = 0;
+  dfree(c->d_atom_lma); dfree(c->d_pt_lma); dfree(c->d_entof); dfree(c->d_rxyz); dfree(c->d_uv);
+  dfree(c->d_rinv); dfree(c->d_zekr); dfree(c->d_zent); dfree(c->d_c); dfree(c->d_curnl);
+  dfree(c->d_tp_g); dfree(c->d_tp_off);
+  c->natom = c->nps = c->nlma = c->nent = c->cmax = c->ntp = 0;
   c->have_k = false;
   if (natom == 0 || nlma == 0) return 0;
   if (natom < 0 || nps < 1 || nlma < 0 || !mps || !jxyz || !rxyz || !ia_tbl || !rinv_uvu || !uv)
@@ -839,22 +1007,48 @@ int salmon_hip_set_nonlocal(salmon_hip_t c, int natom, int nps, const int *mps, 
   }
   for (size_t g = 0; g < c->ngrid; ++g) cnt[g + 1] += cnt[g];
   const int nent = cnt[c->ngrid];
-  std::vector<int2> ent(std::max(nent, 1));
+  int cmax = 1;
+  for (int ia = 0; ia < natom; ++ia) cmax = std::max(cmax, atom_lma[ia].y);
+  // CSR point -> (atom, j) entries; atom-major fill keeps each point's entries
+  // in ascending ilma, the order zpseudo adds them in
+  std::vector<int> pt_lma(std::max(nent, 1), 0);
+  std::vector<int> entof((size_t)natom * nps, 0);
   std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
-  for (int ia = 0; ia < natom; ++ia) {  // atom-major fill keeps entries sorted by ilma per point
+  for (int ia = 0; ia < natom; ++ia) {
     if (atom_lma[ia].y == 0) continue;
     for (int j = 0; j < mps[ia]; ++j) {
       const int *p = jxyz + 3 * ((size_t)j + (size_t)nps * ia);
       const size_t g = (size_t)(p[0] - 1) + (size_t)c->nx * ((size_t)(p[1] - 1) + (size_t)c->ny * (p[2] - 1));
-      ent[fillp[g]++] = make_int2(atom_lma[ia].x | (atom_lma[ia].y << 24), j);
+      const int e = fillp[g]++;
+      pt_lma[e] = atom_lma[ia].x | (atom_lma[ia].y << 24);
+      entof[(size_t)nps * ia + j] = e;
     }
   }
+  std::vector<int> tp_g, tp_off;
+  for (int z = 0; z < c->nz; ++z)
+    for (int y = 0; y < c->ny; ++y)
+      for (int x = 0; x < c->nx; ++x) {
+        const size_t g = (size_t)x + (size_t)c->nx * ((size_t)y + (size_t)c->ny * z);
+        if (cnt[g + 1] > cnt[g]) {
+          tp_g.push_back((int)g);
+          tp_off.push_back(x + c->nx * (y + c->ly * z));
+        }
+      }
   int rc = 0;
+  if (!tp_g.empty() &&
+      ((rc = dalloc(&c->d_tp_g, tp_g.size())) || (rc = dalloc(&c->d_tp_off, tp_off.size()))))
+    return rc;
+  if (!tp_g.empty()) {
+    HIPCHK(hipMemcpy(c->d_tp_g, tp_g.data(), sizeof(int) * tp_g.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_tp_off, tp_off.data(), sizeof(int) * tp_off.size(), hipMemcpyHostToDevice));
+  }
   if ((rc = dalloc(&c->d_mps, natom)) || (rc = dalloc(&c->d_jidx, (size_t)natom * nps)) ||
       (rc = dalloc(&c->d_lma_atom, nlma)) || (rc = dalloc(&c->d_pt_off, c->ngrid + 1)) ||
-      (rc = dalloc(&c->d_atom_lma, natom)) || (rc = dalloc(&c->d_pt_ent, ent.size())) ||
+      (rc = dalloc(&c->d_atom_lma, natom)) || (rc = dalloc(&c->d_pt_lma, pt_lma.size())) ||
+      (rc = dalloc(&c->d_entof, entof.size())) ||
       (rc = dalloc(&c->d_rxyz, 3 * (size_t)nps * natom)) || (rc = dalloc(&c->d_uv, (size_t)nps * nlma)) ||
       (rc = dalloc(&c->d_rinv, nlma)) || (rc = dalloc(&c->d_zekr, (size_t)nps * nlma * c->nk)) ||
+      (rc = dalloc(&c->d_zent, (size_t)cmax * std::max(nent, 1) * c->nk)) ||
       (rc = dalloc(&c->d_c, (size_t)nlma * c->norb)) || (rc = dalloc(&c->d_curnl, 4 * (size_t)nlma * c->norb)))
     return rc;
   HIPCHK(hipMemcpy(c->d_mps, mps, sizeof(int) * natom, hipMemcpyHostToDevice));
@@ -862,7 +1056,8 @@ int salmon_hip_set_nonlocal(salmon_hip_t c, int natom, int nps, const int *mps, 
   HIPCHK(hipMemcpy(c->d_lma_atom, lma_atom.data(), sizeof(int) * nlma, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_pt_off, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_atom_lma, atom_lma.data(), sizeof(int2) * natom, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_pt_ent, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_pt_lma, pt_lma.data(), sizeof(int) * pt_lma.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_entof, entof.data(), sizeof(int) * entof.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_rxyz, rxyz, sizeof(double) * 3 * nps * natom, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_uv, uv, sizeof(double) * (size_t)nps * nlma, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_rinv, rinv_uvu, sizeof(double) * nlma, hipMemcpyHostToDevice));
@@ -870,6 +1065,8 @@ int salmon_hip_set_nonlocal(salmon_hip_t c, int natom, int nps, const int *mps, 
   c->nps = nps;
   c->nlma = nlma;
   c->nent = nent;
+  c->cmax = cmax;
+  c->ntp = (int)tp_g.size();
   return 0;
 }
 
@@ -900,7 +1097,12 @@ int salmon_hip_update_kvector(salmon_hip_t c, const double *vec_Ac) {
   a.uv = c->d_uv;
   a.mps = c->d_mps;
   a.lma_atom = c->d_lma_atom;
+  a.atom_lma = c->d_atom_lma;
+  a.entof = c->d_entof;
   a.zekr = c->d_zekr;
+  a.zent = c->d_zent;
+  a.nent = c->nent;
+  a.cmax = c->cmax;
   a.kpar = c->d_kpar;
   memcpy(a.coef_nab, c->coef_nab, sizeof(a.coef_nab));
   a.coef_lap0 = c->coef_lap0;
@@ -945,36 +1147,104 @@ static int launch_proj(salmon_hip_t c, const void *psi, int nw) {
   return 0;
 }
 
-static int launch_hpsi(salmon_hip_t c, int mode, const void *in, void *hout, void *acc, const double *zc) {
-  HpsiArgs a;
+}  // extern "C"
+
+template <int HC, int NC>
+static size_t stencil_lds_bytes() {
+  return sizeof(double2) * (kRingT * kTile + kRingH * HC * kTile + (NC >= 0 ? 2 * NC * kTile : 0)) +
+         sizeof(double) * kRingH * kTile;
+}
+
+template <int HC, int NC>
+static int launch_stencil_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
+  const size_t shmem = stencil_lds_bytes<HC, NC>();
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencil<HC, NC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)shmem));
+    attr_set = true;
+  }
+  hipExtLaunchKernelGGL(k_stencil<HC, NC>, grid, dim3(kBlock), shmem, c->stream, e0, e1, 0, a);
+  return 0;
+}
+
+template <int HC>
+static int launch_stencil_hc(salmon_hip_ctx *c, int nc, dim3 grid, hipEvent_t e0, hipEvent_t e1,
+                             const StencilArgs &a) {
+  switch (nc) {
+    case -1: return launch_stencil_t<HC, -1>(c, grid, e0, e1, a);
+    case 0: return launch_stencil_t<HC, 0>(c, grid, e0, e1, a);
+    case 1: return launch_stencil_t<HC, 1>(c, grid, e0, e1, a);
+    case 2: return launch_stencil_t<HC, 2>(c, grid, e0, e1, a);
+    case 3: return launch_stencil_t<HC, 3>(c, grid, e0, e1, a);
+    case 4: return launch_stencil_t<HC, 4>(c, grid, e0, e1, a);
+    default: return launch_stencil_t<HC, 5>(c, grid, e0, e1, a);
+  }
+}
+
+extern "C" {
+
+static int launch_scatter(salmon_hip_t c, void *hout, double2 s) {
+  if (c->ntp == 0) return 0;
+  ScatterArgs a;
+  a.hout = (double2 *)hout;
+  a.c = c->d_c;
+  a.zent = c->d_zent;
+  a.tp_g = c->d_tp_g;
+  a.tp_off = c->d_tp_off;
+  a.pt_off = c->d_pt_off;
+  a.pt_lma = c->d_pt_lma;
+  a.s = s;
+  a.osz = c->osz;
+  a.ntp = c->ntp;
+  a.nb = c->nb;
+  a.nlma = c->nlma;
+  a.nent = c->nent;
+  a.cmax = c->cmax;
+  a.ngrp = (c->nb + kScatG - 1) / kScatG;
+  hipEvent_t e0, e1;
+  prof_events(c, SALMON_HIP_KERNEL_NL_SCATTER, &e0, &e1);
+  dim3 grid((unsigned)((c->ntp + kBlock - 1) / kBlock), (unsigned)(c->nk * a.ngrp));
+  hipExtLaunchKernelGGL(k_nl_scatter, grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+  LAUNCHCHK("k_nl_scatter");
+  return 0;
+}
+
+// One H application: hout = H in (fin = false), or the last Taylor term
+// hout = psi + c1 h1 + ... + c_N H in (fin = true, comb = [psi, h1 .. h_{N-2}]).
+// The projections c of `in` must be current (launch_proj).
+static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, bool fin, void *const *comb, int ncomb,
+                       const double *zc) {
+  StencilArgs a;
+  memset(&a, 0, sizeof(a));
   a.in = (const double2 *)in;
   a.hout = (double2 *)hout;
-  a.acc = (double2 *)acc;
+  a.sink = c->d_sink;
+  const int nc = fin ? ncomb : -1;
+  for (int n = 0; n < (fin ? ncomb : 0); ++n) a.comb[n] = (const double2 *)comb[n];
+  if (fin)
+    for (int n = 0; n <= ncomb; ++n) a.zc[n] = make_double2(zc[2 * n], zc[2 * n + 1]);
   a.V = c->d_V;
   a.kpar = c->d_kpar;
-  a.c = c->d_c;
-  a.zekr = c->d_zekr;
-  a.pt_off = c->d_pt_off;
-  a.pt_ent = c->d_pt_ent;
   memcpy(a.lap, c->coef_lap, sizeof(a.lap));
-  a.zc = zc ? make_double2(zc[0], zc[1]) : make_double2(0.0, 0.0);
   a.osz = c->osz;
   a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.ly = c->ly;
-  a.nspin = c->nspin; a.nb = c->nb; a.nk = c->nk; a.nlma = c->nlma; a.nps = c->nps;
+  a.nspin = c->nspin; a.nb = c->nb; a.nk = c->nk;
   a.TY = c->TY; a.TZ = c->TZ; a.nyt = c->nyt; a.nzt = c->nzt;
   a.nblk = c->norb * c->nyt * c->nzt;
-  a.has_nl = c->natom > 0 && c->nlma > 0;
-  const size_t shmem = sizeof(double2) * 2 * (size_t)(c->TY + 8) * c->nx;
   dim3 grid((unsigned)a.nblk);
   hipEvent_t e0, e1;
-  prof_events(c, SALMON_HIP_KERNEL_HPSI0 + mode, &e0, &e1);
-  switch (mode) {
-    case 0: hipExtLaunchKernelGGL(k_hpsi<0>, grid, dim3(kBlock), shmem, c->stream, e0, e1, 0, a); break;
-    case 1: hipExtLaunchKernelGGL(k_hpsi<1>, grid, dim3(kBlock), shmem, c->stream, e0, e1, 0, a); break;
-    case 2: hipExtLaunchKernelGGL(k_hpsi<2>, grid, dim3(kBlock), shmem, c->stream, e0, e1, 0, a); break;
-    default: hipExtLaunchKernelGGL(k_hpsi<3>, grid, dim3(kBlock), shmem, c->stream, e0, e1, 0, a); break;
-  }
-  LAUNCHCHK("k_hpsi");
+  prof_events(c, kind, &e0, &e1);
+  const int hc = (8 * c->nx + kTile - 1) / kTile;
+  int rc = 0;
+  if (hc <= 1) rc = launch_stencil_hc<1>(c, nc, grid, e0, e1, a);
+  else if (hc <= 2) rc = launch_stencil_hc<2>(c, nc, grid, e0, e1, a);
+  else if (hc <= 4) rc = launch_stencil_hc<4>(c, nc, grid, e0, e1, a);
+  else rc = launch_stencil_hc<8>(c, nc, grid, e0, e1, a);
+  if (rc) return rc;
+  LAUNCHCHK("k_stencil");
+  if (c->natom > 0 && c->nlma > 0)
+    return launch_scatter(c, hout, fin ? a.zc[ncomb] : make_double2(1.0, 0.0));
   return 0;
 }
 
@@ -990,26 +1260,37 @@ int salmon_hip_hpsi(salmon_hip_t c, const void *tpsi, void *htpsi) {
   if (int rc = ready(c)) return rc;
   if (c->natom > 0)
     if (int rc = launch_proj(c, tpsi, 1)) return rc;
-  return launch_hpsi(c, 0, tpsi, htpsi, nullptr, nullptr);
+  return launch_hpsi(c, SALMON_HIP_KERNEL_HPSI0, tpsi, htpsi, false, nullptr, 0, nullptr);
 }
 
 int salmon_hip_taylor(salmon_hip_t c, const double *zc, int n_hamil, void *tspsi_in, void *tspsi_out,
                       void *sshtpsi) {
   if (int rc = check(c)) return rc;
-  if (!zc || n_hamil < 1) return fail(SALMON_HIP_EINVAL, "zc/n_hamil invalid");
+  if (!zc || n_hamil < 1 || n_hamil > kMaxComb + 1)
+    return fail(SALMON_HIP_EINVAL, "zc null or n_hamil=%d outside 1..%d", n_hamil, kMaxComb + 1);
   if (!tspsi_in || !tspsi_out || !sshtpsi || tspsi_in == tspsi_out || tspsi_in == sshtpsi || tspsi_out == sshtpsi)
     return fail(SALMON_HIP_EINVAL, "taylor buffers invalid or aliased");
   if (int rc = ready(c)) return rc;
-  for (int nn = 1; nn <= n_hamil; ++nn) {
-    void *src = (nn % 2 == 1) ? tspsi_in : sshtpsi;
-    void *dst = (nn % 2 == 1) ? sshtpsi : tspsi_in;
-    int mode = (nn == 1) ? 1 : 2;
-    if (nn == n_hamil && nn > 1) mode = 3;
-    if (c->natom > 0)
-      if (int rc = launch_proj(c, src, 1)) return rc;
-    if (int rc = launch_hpsi(c, mode, src, dst, tspsi_out, zc + 2 * (nn - 1))) return rc;
+  // h_1 = H psi -> sshtpsi, h_2 .. h_{N-1} -> internal scratch
+  const int nscr = std::max(0, n_hamil - 2);
+  while ((int)c->tscr.size() < nscr) {
+    double2 *p = nullptr;
+    if (int rc = dalloc(&p, c->osz * c->norb)) return rc;
+    c->tscr.push_back(p);
   }
-  return 0;
+  void *h[kMaxComb + 1];
+  h[0] = tspsi_in;
+  if (n_hamil >= 2) h[1] = sshtpsi;
+  for (int n = 2; n < n_hamil; ++n) h[n] = c->tscr[n - 2];
+  for (int n = 1; n < n_hamil; ++n) {
+    if (c->natom > 0)
+      if (int rc = launch_proj(c, h[n - 1], 1)) return rc;
+    const int kind = n == 1 ? SALMON_HIP_KERNEL_HPSI_TAYLOR1 : SALMON_HIP_KERNEL_HPSI_TAYLOR;
+    if (int rc = launch_hpsi(c, kind, h[n - 1], h[n], false, nullptr, 0, nullptr)) return rc;
+  }
+  if (c->natom > 0)
+    if (int rc = launch_proj(c, h[n_hamil - 1], 1)) return rc;
+  return launch_hpsi(c, SALMON_HIP_KERNEL_HPSI_TAYLOR_LAST, h[n_hamil - 1], tspsi_out, true, h, n_hamil - 1, zc);
 }
 
 int salmon_hip_density(salmon_hip_t c, const void *psi, double *rho_dev) {
@@ -1069,9 +1350,12 @@ int salmon_hip_current(salmon_hip_t c, const double *vec_Ac, const void *psi, do
   f.no = c->no;
   f.nk = c->nk;
   f.nlma = c->nlma;
+  const int M = c->no * c->nk;
   for (int is = 0; is < c->nspin; ++is) {
-    hipLaunchKernelGGL(k_current_final, dim3(1), dim3(kBlock), 0, c->stream, f, is);
-    LAUNCHCHK("k_current_final");
+    hipLaunchKernelGGL(k_current_orb, dim3((unsigned)((M + 3) / 4)), dim3(kBlock), 0, c->stream, f, is);
+    LAUNCHCHK("k_current_orb");
+    hipLaunchKernelGGL(k_current_sum, dim3(1), dim3(64), 0, c->stream, f, is);
+    LAUNCHCHK("k_current_sum");
   }
   return 0;
 }

@@ -183,7 +183,7 @@ class Handle:
         _chk(load().salmon_hip_dipole(self.h, C.c_void_p(_ptr(rho)), C.c_void_p(_ptr(out))), "dipole")
 
     KERNELS = {"hpsi": 0, "hpsi_taylor1": 1, "hpsi_taylor": 2, "hpsi_taylor_last": 3, "proj": 4,
-               "proj_current": 5}
+               "proj_current": 5, "nl_scatter": 6}
 
     def prof_start(self, max_launches=100000):
         _chk(load().salmon_hip_prof_start(self.h, int(max_launches)), "prof_start")
