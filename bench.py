@@ -45,6 +45,19 @@ def cpu_baseline(sysm, steps: int):
     }
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (profiles/<workload>_traffic.json; counter values scaled by a
+    1 GiB copy kernel), or None when no summary for this kernel exists."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f).get("per_dispatch_bytes", {})
+    e = d.get(kernel)
+    return None if e is None else e["read_bytes"] + e["write_bytes"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,7 +137,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.config, "k_stencil<1, -1>") if s.nx <= 32 else None,
             "alg_bytes_per_launch": alg_bytes,
             "avg_launch_ms": kern["hpsi_taylor"]["avg_ms"],
         },

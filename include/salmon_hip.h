@@ -95,6 +95,14 @@ int salmon_hip_set_vlocal_device(salmon_hip_t h, const double *vlocal_dev);
 int salmon_hip_update_kvector(salmon_hip_t h, const double *vec_Ac);
 
 /*
+ * Set system%vec_Ac for the stencil terms only (k_lap0, k_nabt), leaving the
+ * phase table of the last update_kvector call in place: hpsi reads vec_Ac at
+ * every call (src/common/hamiltonian.f90:146-155) while zekr_uV changes only in
+ * update_kvector_nonlocalpt.
+ */
+int salmon_hip_set_vec_ac(salmon_hip_t h, const double *vec_Ac);
+
+/*
  * htpsi = H tpsi for every orbital (hpsi, src/common/hamiltonian.f90:27-229,424-435:
  * 25-point stencil + V_local + k+A gradient, then zpseudo,
  * src/common/nonlocal_potential.f90:175-390).  Device pointers; tpsi != htpsi.

@@ -44,5 +44,38 @@ def build_hip(verbose: bool = False) -> str:
     return LIB
 
 
+FDIR = os.path.join(PKG, "fortran")
+FBUILD = os.path.join(FDIR, "_build")
+FORTRAN_SRCS = ["salmon_hip_mod.f90", "salmon_hip_structures_subset.f90", "salmon_hip_dropin.f90",
+                "test_dropin.f90"]
+FORTRAN_EXE = os.path.join(FDIR, "test_dropin")
+
+
+def flang() -> str:
+    for cand in (os.environ.get("FC_FLANG"), "/opt/rocm/lib/llvm/bin/flang", "flang"):
+        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+            return cand
+    return "flang"
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """iso_c_binding layer + SALMON-signature wrappers + their test driver, linked to libsalmon_hip.so."""
+    os.makedirs(FBUILD, exist_ok=True)
+    objs = []
+    for src in FORTRAN_SRCS:
+        obj = os.path.join(FBUILD, src.replace(".f90", ".o"))
+        cmd = [flang(), "-O2", "-fPIC", "-module-dir", FBUILD, "-c", os.path.join(FDIR, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("flang failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+        objs.append(obj)
+    cmd = [flang(), "-o", FORTRAN_EXE, *objs, "-L", PKG, "-lsalmon_hip", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("flang link failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return FORTRAN_EXE
+
+
 if __name__ == "__main__":
     print(build_hip(verbose=True))
+    print(build_fortran(verbose=True))

@@ -60,7 +60,6 @@ constexpr int kBlock = 256;
 constexpr int kProjJC = 16;      // sphere points staged per LDS chunk in k_proj
 constexpr int kProjOrb = 64;     // orbitals per k_proj block (4 waves x 16)
 constexpr int kMaxLmaPerAtom = 16;
-constexpr int kCurSlabs = 8;     // z slabs per orbital in the current kernel
 constexpr int kDensGroup = 32;   // orbitals per density partial
 
 __device__ __forceinline__ int wrapi(int v, int n) { return (v + 8 * n) % n; }
@@ -120,6 +119,29 @@ __global__ __launch_bounds__(kBlock) void k_update_kvector(KvecArgs a) {
     const int e = a.entof[(size_t)j + (size_t)a.nps * ia];
     a.zent[(size_t)(ilma - a.atom_lma[ia].x) + (size_t)a.cmax * ((size_t)e + (size_t)a.nent * ik)] = z;
   }
+}
+
+// Per-k stencil terms only (hpsi reads system%vec_Ac at every call,
+// src/common/hamiltonian.f90:146-155; the phase table is left untouched).
+__global__ void k_set_kpar(KvecArgs a) {
+  const int ik = threadIdx.x;
+  if (ik >= a.nk) return;
+  const double kx = a.veck[3 * ik + 0] + a.ax;
+  const double ky = a.veck[3 * ik + 1] + a.ay;
+  const double kz = a.veck[3 * ik + 2] + a.az;
+  double *kp = a.kpar + 16 * ik;
+  double ss = kx * kx;
+  ss = ss + ky * ky;
+  ss = ss + kz * kz;
+  kp[0] = a.coef_lap0 + 0.5 * ss;
+  for (int n = 0; n < 4; ++n) {
+    kp[1 + n] = kx * a.coef_nab[n];
+    kp[5 + n] = ky * a.coef_nab[4 + n];
+    kp[9 + n] = kz * a.coef_nab[8 + n];
+  }
+  kp[13] = kx;
+  kp[14] = ky;
+  kp[15] = kz;
 }
 
 // ---------------------------------------------------------------------------
@@ -632,15 +654,19 @@ __global__ __launch_bounds__(kBlock) void k_density_sum(DensArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Current, local part per orbital and z slab: sum |psi|^2 and
-// Im sum_n (nab_n conj(psi)) psi(r + n e_d), forward neighbours only.
+// Current, local part (stencil_current, density_matrix.f90:465-530): per
+// orbital sum |psi|^2 and Im sum_n (nab_n conj(psi)) psi(r + n e_d) over
+// forward neighbours.  Block = full x rows x TY rows, streaming z over the
+// whole column: z-forward neighbours from a 5-deep register queue, x and
+// y-forward neighbours from an LDS plane carrying 4 extra rows below the tile.
+// Each block writes its 4 partial sums (fixed-order tree) to part(4, tile, orbital).
 // ---------------------------------------------------------------------------
 struct CurArgs {
   const double2 *psi;
-  double *part;            // (4, nslab, norb)
+  double *part;            // (4, nyt, norb)
   double coef_nab[12];
   size_t osz;
-  int nx, ny, nz, ly, nslab;
+  int nx, ny, nz, ly, TY, nyt;
 };
 
 __device__ __forceinline__ double block_sum(double v, double *sh) {
@@ -656,43 +682,65 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_current_local(CurArgs a) {
+  extern __shared__ double2 lds[];
   __shared__ double sh[kBlock];
-  const int slab = blockIdx.x % a.nslab;
-  const int b = blockIdx.x / a.nslab;
-  const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int zper = (nz + a.nslab - 1) / a.nslab;
-  const int z0 = slab * zper, z1 = min(nz, z0 + zper);
-  const double2 *p = a.psi + (size_t)b * a.osz;
+  const int ytile = blockIdx.x % a.nyt;
+  const int b = blockIdx.x / a.nyt;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, TY = a.TY;
+  const int y0 = ytile * TY;
+  const int tid = threadIdx.x;
+  const bool active = tid < TY * nx;
+  const int te = active ? tid : 0;
+  const int tx = te % nx, ty = te / nx;
+  const int y = y0 + ty;
+  const bool own = active && y < ny;  // this lane's point belongs to the tile
+  const int yw = wrapi(y, ny);
   const size_t zs = (size_t)nx * a.ly;
-  double r0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-  const int npl = nx * ny;
-  const int cnt = (z1 > z0) ? (z1 - z0) * npl : 0;
-  for (int e = threadIdx.x; e < cnt; e += kBlock) {
-    const int z = z0 + e / npl;
-    const int r = e % npl;
-    const int y = r / nx, x = r % nx;
-    const double2 c = p[(size_t)x + (size_t)nx * y + zs * z];
-    r0 += c.x * c.x + c.y * c.y;
-    const double cr = c.x, ci = -c.y;
+  const double2 *p = a.psi + (size_t)b * a.osz;
+  const size_t col = (size_t)tx + (size_t)nx * yw;
+  const int planeL = (TY + 4) * nx;
+  // halo: rows y0+TY .. y0+TY+3, 4*nx elements, loaded by threads e < 4*nx (looping)
+  double2 q[5];
 #pragma unroll
-    for (int n = 1; n <= 4; ++n) {
-      const double2 px = p[(size_t)wrapi(x + n, nx) + (size_t)nx * y + zs * z];
-      const double2 py = p[(size_t)x + (size_t)nx * wrapi(y + n, ny) + zs * z];
-      const double2 pz = p[(size_t)x + (size_t)nx * y + zs * wrapi(z + n, nz)];
-      const double ax = a.coef_nab[n - 1] * cr, bx = a.coef_nab[n - 1] * ci;
-      const double ay = a.coef_nab[3 + n] * cr, by = a.coef_nab[3 + n] * ci;
-      const double az = a.coef_nab[7 + n] * cr, bz = a.coef_nab[7 + n] * ci;
-      sx += ax * px.y + bx * px.x;
-      sy += ay * py.y + by * py.x;
-      sz += az * pz.y + bz * pz.x;
+  for (int i = 0; i < 4; ++i) q[i] = p[col + zs * i];
+  double r0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+  int buf = 0;
+  for (int z = 0; z < nz; ++z) {
+    q[4] = p[col + zs * wrapi(z + 4, nz)];
+    double2 *P = lds + buf * planeL;
+    if (active) P[ty * nx + tx] = q[0];
+    for (int e = tid; e < 4 * nx; e += kBlock) {
+      const int hr = e / nx, hx = e % nx;
+      P[(TY + hr) * nx + hx] = p[(size_t)hx + (size_t)nx * wrapi(y0 + TY + hr, ny) + zs * z];
     }
+    __syncthreads();
+    if (own) {
+      const double2 c = q[0];
+      r0 += c.x * c.x + c.y * c.y;
+      const double cr = c.x, ci = -c.y;
+#pragma unroll
+      for (int n = 1; n <= 4; ++n) {
+        const double2 px = P[ty * nx + wrapi(tx + n, nx)];
+        const double2 py = P[(ty + n) * nx + tx];
+        const double2 pz = q[n];
+        const double ax = a.coef_nab[n - 1] * cr, bx = a.coef_nab[n - 1] * ci;
+        const double ay = a.coef_nab[3 + n] * cr, by = a.coef_nab[3 + n] * ci;
+        const double az = a.coef_nab[7 + n] * cr, bz = a.coef_nab[7 + n] * ci;
+        sx += ax * px.y + bx * px.x;
+        sy += ay * py.y + by * py.x;
+        sz += az * pz.y + bz * pz.x;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) q[i] = q[i + 1];
+    buf ^= 1;
   }
   r0 = block_sum(r0, sh);
   sx = block_sum(sx, sh);
   sy = block_sum(sy, sh);
   sz = block_sum(sz, sh);
-  if (threadIdx.x == 0) {
-    double *o = a.part + 4 * ((size_t)slab + (size_t)a.nslab * b);
+  if (tid == 0) {
+    double *o = a.part + 4 * ((size_t)ytile + (size_t)a.nyt * b);
     o[0] = r0;
     o[1] = sx;
     o[2] = sy;
@@ -757,15 +805,21 @@ __global__ __launch_bounds__(kBlock) void k_current_orb(CurFinArgs a, int ispin)
   }
 }
 
-// Sum of the orbital contributions in (ik, io) order, divided by N_grid.
-__global__ __launch_bounds__(64) void k_current_sum(CurFinArgs a, int ispin) {
+// Sum of the orbital contributions in (ik, io) order, divided by N_grid
+// (contributions staged through LDS in chunks so the serial sum reads LDS).
+__global__ __launch_bounds__(kBlock) void k_current_sum(CurFinArgs a, int ispin) {
+  __shared__ double sh[3 * 1024];
   const int M = a.no * a.nk;
-  if (threadIdx.x < 3) {
-    const int d = threadIdx.x;
-    double w4 = 0.0;
-    for (int mm = 0; mm < M; ++mm) w4 = w4 + a.contrib[3 * (size_t)mm + d];
-    a.curr[3 * ispin + d] = w4 * a.inv_ngrid;
+  double w4 = 0.0;
+  for (int m0 = 0; m0 < M; m0 += 1024) {
+    const int n = min(1024, M - m0);
+    for (int i = threadIdx.x; i < 3 * n; i += kBlock) sh[i] = a.contrib[3 * (size_t)m0 + i];
+    __syncthreads();
+    if (threadIdx.x < 3)
+      for (int mm = 0; mm < n; ++mm) w4 = w4 + sh[3 * mm + threadIdx.x];
+    __syncthreads();
   }
+  if (threadIdx.x < 3) a.curr[3 * ispin + threadIdx.x] = w4 * a.inv_ngrid;
 }
 
 // ---------------------------------------------------------------------------
@@ -927,7 +981,7 @@ int salmon_hip_create(int nx, int ny, int nz, int ly, int nspin, int no, int nk,
   // observables scratch
   c->dens_ngrp = (no * nk + kDensGroup - 1) / kDensGroup;
   if (c->dens_ngrp > 1 && (rc = dalloc(&c->d_denspart, c->ngrid * nspin * c->dens_ngrp))) return bail(rc);
-  if ((rc = dalloc(&c->d_curpart, 4 * (size_t)kCurSlabs * c->norb))) return bail(rc);
+  if ((rc = dalloc(&c->d_curpart, 4 * (size_t)c->nyt * c->norb))) return bail(rc);
   if ((rc = dalloc(&c->d_contrib, 3 * (size_t)no * nk))) return bail(rc);
   if ((rc = dalloc(&c->d_sink, kTile))) return bail(rc);
   *out = c;
@@ -1116,6 +1170,27 @@ int salmon_hip_update_kvector(salmon_hip_t c, const double *vec_Ac) {
   hipLaunchKernelGGL(k_update_kvector, grid, dim3(kBlock), 0, c->stream, a);
   LAUNCHCHK("k_update_kvector");
   c->have_k = true;
+  return 0;
+}
+
+int salmon_hip_set_vec_ac(salmon_hip_t c, const double *vec_Ac) {
+  if (int rc = check(c)) return rc;
+  if (!vec_Ac) return fail(SALMON_HIP_EINVAL, "vec_Ac is null");
+  if (c->nk > 1024) return fail(SALMON_HIP_EUNSUPPORTED, "nk=%d above 1024", c->nk);
+  if (c->natom > 0 && !c->have_k) return fail(SALMON_HIP_ESTATE, "update_kvector not called yet");
+  KvecArgs a;
+  memset(&a, 0, sizeof(a));
+  a.veck = c->d_veck;
+  a.kpar = c->d_kpar;
+  memcpy(a.coef_nab, c->coef_nab, sizeof(a.coef_nab));
+  a.coef_lap0 = c->coef_lap0;
+  a.ax = vec_Ac[0];
+  a.ay = vec_Ac[1];
+  a.az = vec_Ac[2];
+  a.nk = c->nk;
+  hipLaunchKernelGGL(k_set_kpar, dim3(1), dim3(1024), 0, c->stream, a);
+  LAUNCHCHK("k_set_kpar");
+  if (c->natom == 0) c->have_k = true;
   return 0;
 }
 
@@ -1327,8 +1402,10 @@ int salmon_hip_current(salmon_hip_t c, const double *vec_Ac, const void *psi, do
   memcpy(a.coef_nab, c->coef_nab, sizeof(a.coef_nab));
   a.osz = c->osz;
   a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.ly = c->ly;
-  a.nslab = kCurSlabs;
-  hipLaunchKernelGGL(k_current_local, dim3((unsigned)(c->norb * kCurSlabs)), dim3(kBlock), 0, c->stream, a);
+  a.TY = c->TY;
+  a.nyt = c->nyt;
+  const size_t shm = sizeof(double2) * 2 * (size_t)(c->TY + 4) * c->nx;
+  hipLaunchKernelGGL(k_current_local, dim3((unsigned)(c->norb * c->nyt)), dim3(kBlock), shm, c->stream, a);
   LAUNCHCHK("k_current_local");
   if (c->natom > 0)
     if (int rc = launch_proj(c, psi, 4)) return rc;
@@ -1345,7 +1422,7 @@ int salmon_hip_current(salmon_hip_t c, const double *vec_Ac, const void *psi, do
   f.ay = vec_Ac[1];
   f.az = vec_Ac[2];
   f.inv_ngrid = 1.0 / (double)c->ngrid;
-  f.nslab = kCurSlabs;
+  f.nslab = c->nyt;
   f.nspin = c->nspin;
   f.no = c->no;
   f.nk = c->nk;
@@ -1354,7 +1431,7 @@ int salmon_hip_current(salmon_hip_t c, const double *vec_Ac, const void *psi, do
   for (int is = 0; is < c->nspin; ++is) {
     hipLaunchKernelGGL(k_current_orb, dim3((unsigned)((M + 3) / 4)), dim3(kBlock), 0, c->stream, f, is);
     LAUNCHCHK("k_current_orb");
-    hipLaunchKernelGGL(k_current_sum, dim3(1), dim3(64), 0, c->stream, f, is);
+    hipLaunchKernelGGL(k_current_sum, dim3(1), dim3(kBlock), 0, c->stream, f, is);
     LAUNCHCHK("k_current_sum");
   }
   return 0;

@@ -26,6 +26,7 @@ EXPORTS = (
     "salmon_hip_set_vlocal",
     "salmon_hip_set_vlocal_device",
     "salmon_hip_update_kvector",
+    "salmon_hip_set_vec_ac",
     "salmon_hip_hpsi",
     "salmon_hip_taylor",
     "salmon_hip_density",
@@ -72,6 +73,7 @@ def load():
     L.salmon_hip_set_vlocal.argtypes = [C.c_void_p, _dp]
     L.salmon_hip_set_vlocal_device.argtypes = [C.c_void_p, C.c_void_p]
     L.salmon_hip_update_kvector.argtypes = [C.c_void_p, _dp]
+    L.salmon_hip_set_vec_ac.argtypes = [C.c_void_p, _dp]
     L.salmon_hip_hpsi.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_taylor.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_density.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -163,6 +165,10 @@ class Handle:
     def update_kvector(self, vec_ac):
         a = np.ascontiguousarray(vec_ac, dtype=np.float64)
         _chk(load().salmon_hip_update_kvector(self.h, _p(a)), "update_kvector")
+
+    def set_vec_ac(self, vec_ac):
+        a = np.ascontiguousarray(vec_ac, dtype=np.float64)
+        _chk(load().salmon_hip_set_vec_ac(self.h, _p(a)), "set_vec_ac")
 
     def hpsi(self, tpsi, htpsi):
         _chk(load().salmon_hip_hpsi(self.h, C.c_void_p(_ptr(tpsi)), C.c_void_p(_ptr(htpsi))), "hpsi")
