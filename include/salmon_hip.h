@@ -144,6 +144,47 @@ int salmon_hip_current(salmon_hip_t h, const double *vec_Ac, const void *psi, do
 int salmon_hip_dipole(salmon_hip_t h, const double *rho_dev, double *out_dev);
 
 /*
+ * Reciprocal-space tables of the periodic Poisson solver for an orthogonal cell
+ * of lengths al[3] (bohr): the DFT phase tables egx/egy/egz and
+ * coef = 4 pi/|G|^2 with G = 0 removed (init_reciprocal_grid,
+ * src/common/initialization.f90:640-784).  Required before hartree and
+ * update_potential; may be called again when the cell changes.
+ */
+int salmon_hip_set_poisson(salmon_hip_t h, const double *al);
+
+/*
+ * Hartree potential Vh(nx,ny,nz) of the total density rho(nx,ny,nz), both DEVICE
+ * arrays (hartree -> poisson_ft, src/poisson/hartree.f90, poisson_periodic.f90:23-242):
+ * separable forward DFT z, y, x with 1/N, multiply by coef, inverse DFT, real part.
+ */
+int salmon_hip_hartree(salmon_hip_t h, const double *rho_dev, double *vh_dev);
+
+/*
+ * PZ81 LDA for nspin = 1 (exchange_correlation with xc = 'PZ', src/xc/salmon_xc.f90:58-343,
+ * builtin_pz.f90:23-83): vxc(nx,ny,nz) and, if exc_dev is not null, the energy
+ * density per electron.  DEVICE arrays.
+ */
+int salmon_hip_xc_pz(salmon_hip_t h, const double *rho_dev, double *vxc_dev, double *exc_dev);
+
+/*
+ * V_local = Vpsl + Vh + Vxc into the handle's local potential (update_vlocal,
+ * src/common/hamiltonian.f90:813-843), nspin = 1.  With vmix_dev not null the
+ * result is the predictor-corrector average (vmix + V)/2
+ * (src/rt/time_evolution_step.f90:402-405).  If vout_dev is not null the result
+ * is also written there.  All arrays on the DEVICE, (nx,ny,nz).
+ */
+int salmon_hip_update_vlocal(salmon_hip_t h, const double *vpsl_dev, const double *vh_dev,
+                             const double *vxc_dev, const double *vmix_dev, double *vout_dev);
+
+/*
+ * hartree + exchange_correlation (PZ) + update_vlocal in one call from the density,
+ * as time_evolution_step.f90:182-207 does every step with yn_fix_func = 'n'
+ * (Vh kept in handle scratch, Vxc formed in the update kernel).
+ */
+int salmon_hip_update_potential(salmon_hip_t h, const double *rho_dev, const double *vpsl_dev,
+                                const double *vmix_dev, double *vout_dev);
+
+/*
  * Kernel timing (no counterpart in the reference, whose timers are host
  * wall-clock regions, src/misc/timer.f90:297-310): between prof_start and
  * prof_end every launch of the listed kernels is bracketed by start/stop

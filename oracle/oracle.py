@@ -69,6 +69,11 @@ def lib():
         L.orc_ac_asin2cos.argtypes = [C.c_double] * 4 + [_dp, C.c_double, C.c_int, _dp]
         L.orc_rt_run.argtypes = [S, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_rt_run.restype = C.c_int
+        L.orc_hartree.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_xc_pz.argtypes = [C.c_long, _dp, _dp, _dp]
+        L.orc_rt_run_scf.argtypes = [S, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
+                                     _dp, _dp]
+        L.orc_rt_run_scf.restype = C.c_int
         _lib = L
     return _lib
 
@@ -184,6 +189,48 @@ class Oracle:
         if rc != 0:
             raise MemoryError("oracle rt_run allocation failed")
         return p, cur, dip, rne
+
+
+    def rt_run_scf(self, pc: bool, nt=None, psi=None):
+        """RT loop with V_local = Vpsl + Vh[rho] + Vxc[rho] rebuilt every step (system.vlocal is Vpsl),
+        optionally with the predictor-corrector; returns final psi, J(t), dipole(t), rNe(t), final V_local."""
+        s = self.s
+        nt = s.nt if nt is None else nt
+        ac = np.ascontiguousarray(s.pulse.table(s.dt, nt) if s.pulse else np.zeros((nt + 2, 3)))
+        p = np.array(s.psi if psi is None else psi, dtype=np.complex128, copy=True)
+        vpsl = np.ascontiguousarray(s.vlocal[0], dtype=np.float64)
+        vloc = np.zeros_like(vpsl)
+        al = np.ascontiguousarray(s.al, dtype=np.float64)
+        cur = np.zeros((nt, 3))
+        dip = np.zeros((nt, 3))
+        rne = np.zeros(nt)
+        saved = self.o.vlocal
+        try:
+            rc = lib().orc_rt_run_scf(self.ref, _p(vpsl), _p(vloc), _p(al), int(bool(pc)), s.dt, s.n_hamil, nt,
+                                      _p(ac), _p(p.view(np.float64)), _p(cur), _p(dip), _p(rne))
+        finally:
+            self.o.vlocal = saved
+        if rc != 0:
+            raise RuntimeError(f"oracle rt_run_scf failed ({rc})")
+        return p, cur, dip, rne, vloc
+
+
+def hartree(rho, al):
+    """Hartree potential of a periodic orthogonal cell; rho is (nz, ny, nx)."""
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    nz, ny, nx = r.shape
+    v = np.zeros_like(r)
+    lib().orc_hartree(nx, ny, nz, _p(np.ascontiguousarray(al, dtype=np.float64)), _p(r), _p(v))
+    return v
+
+
+def xc_pz(rho):
+    """PZ81 LDA (unpolarised): returns (vxc, exc) per point."""
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    v = np.zeros_like(r)
+    e = np.zeros_like(r)
+    lib().orc_xc_pz(r.size, _p(r), _p(v), _p(e))
+    return v, e
 
 
 def zstencil(nx, ny, nz, ly, tpsi, V, lap0, lapt, nabt, mode="seq"):

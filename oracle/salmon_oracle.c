@@ -27,6 +27,12 @@
  *   orc_dipole          closed form on analytic densities.
  *   orc_ac_asin2cos     closed form of em_field.f90:177-187.
  *   orc_rt_run          composition of the above (time_evolution_step.f90).
+ *   orc_hartree         closed form 4 pi/|G|^2 cos(G.r) on cosine densities; the
+ *                       periodic Poisson equation through the FD-free spectral
+ *                       Laplacian (G^2 V(G) = 4 pi rho(G)); G = 0 removed.
+ *   orc_xc_pz           v = d(rho e)/d rho by finite differences of rho*exc; the
+ *                       published PZ81 values at rs = 1, 2, 5.
+ *   orc_rt_run_scf      composition (time_evolution_step.f90:160-207, 354-423).
  */
 #include "salmon_oracle.h"
 
@@ -556,5 +562,195 @@ int orc_rt_run(orc_system *s, double dt, int n_hamil, int nt, const double *ac_t
   free(w);
   free(rho);
   free(zc);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Twiddle table e(k, j) = exp(i 2 pi (j k) / n), stored [k*n + j]
+   (src/common/initialization.f90:761-784: egx(kx,ix), egxc = conjg) */
+static void twiddles(int n, double *er, double *ei) {
+  const double pi = 3.141592653589793238462643383279502884;
+  for (int j = 0; j < n; ++j)
+    for (int k = 0; k < n; ++k) {
+      const double ang = 2.0 * pi * (double)((long)j * k) / (double)n;
+      er[(size_t)k * n + j] = cos(ang);
+      ei[(size_t)k * n + j] = sin(ang);
+    }
+}
+
+/* out(k along axis d) = sum_j conj?(e(k, j)) in(j along axis d) for every line of axis d */
+static void dft_axis(const int n[3], int d, int conj_tw, const double *er, const double *ei, const double *inr,
+                     const double *ini, double *outr, double *outi) {
+  const int m = n[d];
+  const size_t stride = d == 0 ? 1 : (d == 1 ? (size_t)n[0] : (size_t)n[0] * n[1]);
+  const size_t nlines = (size_t)n[0] * n[1] * n[2] / (size_t)m;
+  const double sg = conj_tw ? -1.0 : 1.0;
+#pragma omp parallel for schedule(static)
+  for (long l = 0; l < (long)nlines; ++l) {
+    /* line start: the point with index 0 along d */
+    size_t base;
+    if (d == 0) base = (size_t)l * m;
+    else if (d == 1) base = (size_t)(l % n[0]) + (size_t)(l / n[0]) * (size_t)n[0] * n[1];
+    else base = (size_t)l;
+    for (int k = 0; k < m; ++k) {
+      double sr = 0.0, si = 0.0;
+      for (int j = 0; j < m; ++j) {
+        const double c = er[(size_t)k * m + j], t = sg * ei[(size_t)k * m + j];
+        const double xr = inr[base + stride * j], xi = ini[base + stride * j];
+        sr = sr + (c * xr - t * xi);
+        si = si + (c * xi + t * xr);
+      }
+      outr[base + stride * k] = sr;
+      outi[base + stride * k] = si;
+    }
+  }
+}
+
+/* Hartree potential of a periodic orthogonal cell: poisson_ft
+   (src/poisson/poisson_periodic.f90:23-242) as separable discrete Fourier sums:
+   forward z, y, x with egzc/egyc/egxc and 1/N at the x stage, V(G) = coef(G) rho(G)
+   with coef = 4 pi/|G|^2, 0 at G = 0, G index i -> i - n for i >= (n+1)/2
+   (src/common/initialization.f90:705-724), inverse z, y, x with egz/egy/egx, Vh = Re. */
+void orc_hartree(int nx, int ny, int nz, const double al[3], const double *rho, double *vh) {
+  const double pi = 3.141592653589793238462643383279502884;
+  const int n[3] = {nx, ny, nz};
+  const size_t N = (size_t)nx * ny * nz;
+  double *ar = (double *)malloc(sizeof(double) * N), *ai = (double *)malloc(sizeof(double) * N);
+  double *br = (double *)malloc(sizeof(double) * N), *bi = (double *)malloc(sizeof(double) * N);
+  double *er[3], *ei[3];
+  for (int d = 0; d < 3; ++d) {
+    er[d] = (double *)malloc(sizeof(double) * n[d] * n[d]);
+    ei[d] = (double *)malloc(sizeof(double) * n[d] * n[d]);
+    twiddles(n[d], er[d], ei[d]);
+  }
+  for (size_t i = 0; i < N; ++i) { ar[i] = rho[i]; ai[i] = 0.0; }
+  dft_axis(n, 2, 1, er[2], ei[2], ar, ai, br, bi);
+  dft_axis(n, 1, 1, er[1], ei[1], br, bi, ar, ai);
+  dft_axis(n, 0, 1, er[0], ei[0], ar, ai, br, bi);
+  for (int kz = 0; kz < nz; ++kz)
+    for (int ky = 0; ky < ny; ++ky)
+      for (int kx = 0; kx < nx; ++kx) {
+        const int ik[3] = {kx, ky, kz};
+        double g2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          const int ii = ik[d] >= (n[d] + 1) / 2 ? ik[d] - n[d] : ik[d];
+          const double g = (double)ii * (2.0 * pi / al[d]);
+          g2 = g2 + g * g;
+        }
+        const size_t i = (size_t)kx + (size_t)nx * ((size_t)ky + (size_t)ny * kz);
+        const double coef = (kx == 0 && ky == 0 && kz == 0) ? 0.0 : 4.0 * pi / g2;
+        br[i] = coef * (br[i] / (double)N);
+        bi[i] = coef * (bi[i] / (double)N);
+      }
+  dft_axis(n, 2, 0, er[2], ei[2], br, bi, ar, ai);
+  dft_axis(n, 1, 0, er[1], ei[1], ar, ai, br, bi);
+  dft_axis(n, 0, 0, er[0], ei[0], br, bi, ar, ai);
+  for (size_t i = 0; i < N; ++i) vh[i] = ar[i];
+  for (int d = 0; d < 3; ++d) { free(er[d]); free(ei[d]); }
+  free(ar); free(ai); free(br); free(bi);
+}
+
+/* PZ81 LDA, unpolarised: exc_cor_pz / PZxc (src/xc/builtin_pz.f90:23-83) with
+   trho = rho (rho_s = rho/2 then trho = 2 rho_s, salmon_xc.f90:778-795). */
+void orc_xc_pz(long n, const double *rho, double *vxc, double *exc_out) {
+  const double Pi = 3.141592653589793;
+  const double gammaU = -0.1423, beta1U = 1.0529, beta2U = 0.3334, AU = 0.0311, BU = -0.048;
+  const double CU = 0.002, DU = -0.0116;
+  const double cst = 0.75 * pow(3.0 / (2.0 * Pi), 2.0 / 3.0);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    const double rho_s = rho[i] * 0.5;
+    const double trho = 2 * rho_s;
+    const double ttrho = trho + 1e-10;
+    const double rs = pow(3.0 / (4 * Pi * ttrho), 1.0 / 3.0);
+    double exc = -cst / rs;
+    double dexc = exc / (3 * ttrho);
+    if (rs > 1.0) {
+      const double rssq = sqrt(rs);
+      exc = exc + gammaU / (1.0 + beta1U * rssq + beta2U * rs);
+      dexc = dexc + gammaU * (0.5 * beta1U * rssq + beta2U * rs) / (3 * ttrho) /
+                        ((1 + beta1U * rssq + beta2U * rs) * (1 + beta1U * rssq + beta2U * rs));
+    } else {
+      const double rsln = log(rs);
+      exc = exc + AU * rsln + BU + CU * rs * rsln + DU * rs;
+      dexc = dexc - rs / (3 * ttrho) * (AU / rs + CU * (rsln + 1) + DU);
+    }
+    vxc[i] = exc + trho * dexc;
+    if (exc_out) exc_out[i] = exc;
+  }
+}
+
+/* V_local = Vpsl + Vh + Vxc (update_vlocal, src/common/hamiltonian.f90:813-843), one spin */
+static void vlocal_from_rho(const orc_system *s, const double *vpsl, const double al[3], const double *rho,
+                            double *vloc) {
+  const size_t g = grid_size(s);
+  double *vh = (double *)malloc(sizeof(double) * g), *vxc = (double *)malloc(sizeof(double) * g);
+  orc_hartree(s->nx, s->ny, s->nz, al, rho, vh);
+  orc_xc_pz((long)g, rho, vxc, NULL);
+  for (size_t i = 0; i < g; ++i) vloc[i] = (vpsl[i] + vh[i]) + vxc[i];
+  free(vh);
+  free(vxc);
+}
+
+/*
+ * RT loop with the potential updated from the density every step (yn_fix_func = 'n'):
+ * src/rt/time_evolution_step.f90:141-148 (optional predictor_corrector, :354-423),
+ * :160-166 (density), :182-207 (Hartree, XC, update_vlocal), :243-253 (current);
+ * the initial V_local is built from the initial density (src/rt/initialization_rt.f90).
+ * vpsl is the fixed local pseudopotential; vloc receives V_local (s->vlocal is set to it).
+ */
+int orc_rt_run_scf(orc_system *s, const double *vpsl, double *vloc, const double al[3], int pc, double dt,
+                   int n_hamil, int nt, const double *ac_tot, double *psi, double *curr_t, double *dip_t,
+                   double *rne_t) {
+  if (s->nspin != 1) return -2;
+  const size_t tot = C2(orb_size(s) * (size_t)s->nspin * s->no * s->nk);
+  const size_t g = grid_size(s);
+  double *a = psi;
+  double *b = (double *)calloc(tot, sizeof(double));
+  double *w = (double *)calloc(tot, sizeof(double));
+  double *psi_tmp = pc ? (double *)malloc(sizeof(double) * tot) : NULL;
+  double *rho = (double *)malloc(sizeof(double) * g);
+  double *v_tmp = (double *)malloc(sizeof(double) * g);
+  double *zc = (double *)malloc(sizeof(double) * 2 * n_hamil);
+  if (!b || !w || !rho || !v_tmp || !zc || (pc && !psi_tmp)) return -1;
+  s->vlocal = vloc;
+  orc_taylor_coefs(dt, n_hamil, 0, zc);
+  orc_density(s, a, rho);
+  vlocal_from_rho(s, vpsl, al, rho, vloc);
+  for (int itt = 1; itt <= nt; ++itt) {
+    double A[3];
+    for (int d = 0; d < 3; ++d) A[d] = 0.5 * (ac_tot[3 * itt + d] + ac_tot[3 * (itt - 1) + d]);
+    if (s->natom > 0) orc_update_kvector(s, A);
+    if (pc) {
+      memcpy(v_tmp, vloc, sizeof(double) * g);
+      memcpy(psi_tmp, a, sizeof(double) * tot);
+      orc_taylor(s, A, zc, n_hamil, a, b, w);
+      orc_density(s, b, rho);
+      vlocal_from_rho(s, vpsl, al, rho, vloc);
+      for (size_t i = 0; i < g; ++i) vloc[i] = 0.5 * (v_tmp[i] + vloc[i]);
+      memcpy(a, psi_tmp, sizeof(double) * tot);
+    }
+    orc_taylor(s, A, zc, n_hamil, a, b, w);
+    orc_density(s, b, rho);
+    vlocal_from_rho(s, vpsl, al, rho, vloc);
+    double Ac[3] = {ac_tot[3 * itt], ac_tot[3 * itt + 1], ac_tot[3 * itt + 2]};
+    double cur[3];
+    orc_current(s, Ac, b, cur);
+    for (int d = 0; d < 3; ++d) curr_t[3 * (itt - 1) + d] = cur[d];
+    double dp[4];
+    orc_dipole(s, rho, dp);
+    for (int d = 0; d < 3; ++d) dip_t[3 * (itt - 1) + d] = dp[d];
+    rne_t[itt - 1] = dp[3];
+    double *t = a;
+    a = b;
+    b = t;
+  }
+  if (a != psi) {
+    memcpy(psi, a, sizeof(double) * tot);
+    free(a);
+  } else {
+    free(b);
+  }
+  free(w); free(psi_tmp); free(rho); free(v_tmp); free(zc);
   return 0;
 }

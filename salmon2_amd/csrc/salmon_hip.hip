@@ -18,6 +18,10 @@
 //   k_density_*       calc_density (density_matrix.f90:150-230)
 //   k_current_*       stencil_current + calc_current (density_matrix.f90:240-530)
 //   k_dipole          calc_dip (src/rt/dip.f90:114-225)
+//   k_dft_axis        Hartree potential (poisson_ft, src/poisson/poisson_periodic.f90:23-242)
+//   k_xc_pz           PZ81 LDA (src/xc/builtin_pz.f90:23-83)
+//   k_update_vlocal   update_vlocal (src/common/hamiltonian.f90:813-843) with the
+//                     predictor-corrector average (src/rt/time_evolution_step.f90:404)
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
@@ -860,6 +864,135 @@ __global__ __launch_bounds__(1024) void k_dipole(const double *rho, double *out,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Hartree potential: poisson_ft (src/poisson/poisson_periodic.f90:23-242) as six
+// separable DFT passes (forward z, y, x with conj twiddles, 1/N and the 4 pi/|G|^2
+// table folded into the x pass; inverse z, y, x, real part out).  Per pass the grid
+// is viewed as (outer, n, inner), inner = stride of the transformed axis; a block
+// owns `cols` consecutive columns c = (o, i), stages the [n][cols] tile in LDS and
+// forms out(k) = sum_j tw(k, j) in(j) with the n x n table tw = exp(i 2 pi j k / n)
+// (src/common/initialization.f90:761-784).  Reads and writes are unit-stride along
+// c for the y and z passes and along j for the x pass.
+// ---------------------------------------------------------------------------
+constexpr int kDftMaxN = 256;
+
+struct DftArgs {
+  const double2 *in;  // complex input, or null with rin
+  const double *rin;  // real input (first forward pass)
+  double2 *out;       // complex output, or null with rout
+  double *rout;       // real part output (last inverse pass)
+  const double2 *tw;  // n x n, tw[k * n + j]
+  const double *coef; // per-point multiplier after the pass (x forward pass), or null
+  double nrm;         // divisor applied before coef
+  long long ncols;    // outer * inner
+  int n, inner, cols;
+  double sg;          // -1: conj(tw) (forward), +1: tw (inverse)
+};
+
+__global__ __launch_bounds__(256) void k_dft_axis(DftArgs a) {
+  extern __shared__ double2 dft_tile[];  // [n][cols]
+  const int n = a.n, inner = a.inner, cols = a.cols;
+  const long long c0 = (long long)blockIdx.x * cols;
+  const int ncl = (int)min((long long)cols, a.ncols - c0);
+  const int tot = n * cols;
+  for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+    int j, cl;
+    if (inner == 1) { cl = t / n; j = t - cl * n; } else { j = t / cols; cl = t - j * cols; }
+    double2 v = make_double2(0.0, 0.0);
+    if (cl < ncl) {
+      const long long c = c0 + cl;
+      const long long o = c / inner, i = c - o * inner;
+      const size_t g = ((size_t)o * n + j) * inner + i;
+      v = a.rin ? make_double2(a.rin[g], 0.0) : a.in[g];
+    }
+    dft_tile[j * cols + cl] = v;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+    int k, cl;
+    if (inner == 1) { cl = t / n; k = t - cl * n; } else { k = t / cols; cl = t - k * cols; }
+    if (cl >= ncl) continue;
+    const double2 *w = a.tw + (size_t)k * n;
+    double sr = 0.0, si = 0.0;
+    for (int j = 0; j < n; ++j) {
+      const double2 e = w[j];
+      const double2 x = dft_tile[j * cols + cl];
+      const double ei = a.sg * e.y;
+      sr = sr + (e.x * x.x - ei * x.y);
+      si = si + (e.x * x.y + ei * x.x);
+    }
+    const long long c = c0 + cl;
+    const long long o = c / inner, i = c - o * inner;
+    const size_t g = ((size_t)o * n + k) * inner + i;
+    if (a.coef) {
+      const double cf = a.coef[g];
+      sr = cf * (sr / a.nrm);
+      si = cf * (si / a.nrm);
+    }
+    if (a.rout) a.rout[g] = sr;
+    else a.out[g] = make_double2(sr, si);
+  }
+}
+
+// PZ81 LDA, unpolarised (exc_cor_pz / PZxc, src/xc/builtin_pz.f90:23-83), trho = rho.
+__device__ __forceinline__ void pz_point(double rho, double &vxc, double &exc) {
+  const double Pi = 3.141592653589793;
+  const double gammaU = -0.1423, beta1U = 1.0529, beta2U = 0.3334, AU = 0.0311, BU = -0.048;
+  const double CU = 0.002, DU = -0.0116;
+  const double cst = 0.75 * pow(3.0 / (2.0 * Pi), 2.0 / 3.0);
+  const double trho = 2.0 * (rho * 0.5);
+  const double ttrho = trho + 1e-10;
+  const double rs = pow(3.0 / (4 * Pi * ttrho), 1.0 / 3.0);
+  double e = -cst / rs;
+  double de = e / (3 * ttrho);
+  if (rs > 1.0) {
+    const double rssq = sqrt(rs);
+    const double den = 1.0 + beta1U * rssq + beta2U * rs;
+    e = e + gammaU / den;
+    de = de + gammaU * (0.5 * beta1U * rssq + beta2U * rs) / (3 * ttrho) / (den * den);
+  } else {
+    const double rsln = log(rs);
+    e = e + AU * rsln + BU + CU * rs * rsln + DU * rs;
+    de = de - rs / (3 * ttrho) * (AU / rs + CU * (rsln + 1) + DU);
+  }
+  vxc = e + trho * de;
+  exc = e;
+}
+
+__global__ __launch_bounds__(256) void k_xc_pz(const double *rho, double *vxc, double *exc, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v, e;
+  pz_point(rho[i], v, e);
+  vxc[i] = v;
+  if (exc) exc[i] = e;
+}
+
+// update_vlocal (src/common/hamiltonian.f90:813-843): V = Vpsl + Vh + Vxc, with Vxc
+// either given or formed from rho on the fly; optionally the predictor-corrector
+// average V = (Vmix + V)/2 (src/rt/time_evolution_step.f90:404).
+struct VlocArgs {
+  const double *vpsl, *vh, *vxc, *rho, *vmix;
+  double *V, *vout;
+  size_t n;
+};
+
+__global__ __launch_bounds__(256) void k_update_vlocal(VlocArgs a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double vx;
+  if (a.rho) {
+    double e;
+    pz_point(a.rho[i], vx, e);
+  } else {
+    vx = a.vxc[i];
+  }
+  double v = a.vpsl[i] + a.vh[i] + vx;
+  if (a.vmix) v = 0.5 * (a.vmix[i] + v);
+  a.V[i] = v;
+  if (a.vout) a.vout[i] = v;
+}
+
 template <typename T>
 int dalloc(T **p, size_t n) {
   *p = nullptr;
@@ -903,6 +1036,11 @@ struct salmon_hip_ctx {
   // observables scratch
   double *d_curpart = nullptr, *d_contrib = nullptr, *d_denspart = nullptr;
   int dens_ngrp = 1;
+  // Hartree (salmon_hip_set_poisson): twiddle tables per axis, 4 pi/|G|^2 table, work
+  bool have_cell = false;
+  double2 *d_tw[3] = {nullptr, nullptr, nullptr};
+  double *d_gcoef = nullptr, *d_vh = nullptr;
+  double2 *d_fft[2] = {nullptr, nullptr};
   // stencil launch shape
   int TY = 1, TZ = 1, nyt = 1, nzt = 1;
   // kernel timing (salmon_hip_prof_*): start/stop event pairs per launch
@@ -998,6 +1136,8 @@ int salmon_hip_destroy(salmon_hip_t c) {
   dfree(c->d_rinv); dfree(c->d_zekr); dfree(c->d_zent); dfree(c->d_c); dfree(c->d_curnl);
   dfree(c->d_tp_g); dfree(c->d_tp_off); dfree(c->d_sink);
   dfree(c->d_curpart); dfree(c->d_contrib); dfree(c->d_denspart);
+  for (int d = 0; d < 3; ++d) dfree(c->d_tw[d]);
+  dfree(c->d_gcoef); dfree(c->d_vh); dfree(c->d_fft[0]); dfree(c->d_fft[1]);
   for (double2 *&p : c->tscr) dfree(p);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->own) (void)hipStreamDestroy(c->own);
@@ -1444,6 +1584,128 @@ int salmon_hip_dipole(salmon_hip_t c, const double *rho_dev, double *out_dev) {
                      c->nspin, c->hgs[0], c->hgs[1], c->hgs[2], c->hvol);
   LAUNCHCHK("k_dipole");
   return 0;
+}
+
+int salmon_hip_set_poisson(salmon_hip_t c, const double *al) {
+  if (int rc = check(c)) return rc;
+  if (!al) return fail(SALMON_HIP_EINVAL, "null pointer");
+  for (int d = 0; d < 3; ++d)
+    if (!(al[d] > 0.0)) return fail(SALMON_HIP_EINVAL, "cell length al[%d]=%g", d, al[d]);
+  const int n[3] = {c->nx, c->ny, c->nz};
+  for (int d = 0; d < 3; ++d)
+    if (n[d] > kDftMaxN) return fail(SALMON_HIP_EUNSUPPORTED, "grid extent %d above %d", n[d], kDftMaxN);
+  const double pi = 3.141592653589793238462643383279502884;
+  int rc = 0;
+  for (int d = 0; d < 3; ++d) {
+    std::vector<double2> tw((size_t)n[d] * n[d]);
+    for (int j = 0; j < n[d]; ++j)
+      for (int k = 0; k < n[d]; ++k) {
+        const double ang = 2.0 * pi * (double)((long)j * k) / (double)n[d];
+        tw[(size_t)k * n[d] + j] = make_double2(cos(ang), sin(ang));
+      }
+    if (!c->d_tw[d] && (rc = dalloc(&c->d_tw[d], tw.size()))) return rc;
+    HIPCHK(hipMemcpy(c->d_tw[d], tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
+  }
+  // fg%coef (src/common/initialization.f90:705-724): index i -> i - n for i >= (n+1)/2
+  std::vector<double> coef(c->ngrid);
+  for (int kz = 0; kz < c->nz; ++kz)
+    for (int ky = 0; ky < c->ny; ++ky)
+      for (int kx = 0; kx < c->nx; ++kx) {
+        const int ik[3] = {kx, ky, kz};
+        double g2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          const int ii = ik[d] >= (n[d] + 1) / 2 ? ik[d] - n[d] : ik[d];
+          const double g = (double)ii * (2.0 * pi / al[d]);
+          g2 = g2 + g * g;
+        }
+        coef[(size_t)kx + (size_t)c->nx * ((size_t)ky + (size_t)c->ny * kz)] =
+            (kx == 0 && ky == 0 && kz == 0) ? 0.0 : 4.0 * pi / g2;
+      }
+  if (!c->d_gcoef && (rc = dalloc(&c->d_gcoef, c->ngrid))) return rc;
+  HIPCHK(hipMemcpy(c->d_gcoef, coef.data(), sizeof(double) * c->ngrid, hipMemcpyHostToDevice));
+  if (!c->d_fft[0] && (rc = dalloc(&c->d_fft[0], c->ngrid))) return rc;
+  if (!c->d_fft[1] && (rc = dalloc(&c->d_fft[1], c->ngrid))) return rc;
+  if (!c->d_vh && (rc = dalloc(&c->d_vh, c->ngrid))) return rc;
+  c->have_cell = true;
+  return 0;
+}
+
+static int launch_dft(salmon_hip_t c, int axis, double sg, const double *rin, const double2 *in, double2 *out,
+                      double *rout, const double *coef) {
+  const int n[3] = {c->nx, c->ny, c->nz};
+  const int inner = axis == 0 ? 1 : (axis == 1 ? c->nx : c->nx * c->ny);
+  DftArgs a;
+  a.in = in; a.rin = rin; a.out = out; a.rout = rout;
+  a.tw = c->d_tw[axis];
+  a.coef = coef;
+  a.nrm = (double)c->ngrid;
+  a.n = n[axis];
+  a.inner = inner;
+  a.ncols = (long long)(c->ngrid / (size_t)n[axis]);
+  a.sg = sg;
+  // about one output per thread; columns per block a power of two in [1, 32]
+  int cols = 1;
+  while (cols < 32 && 2 * cols * a.n <= 256) cols *= 2;
+  a.cols = cols;
+  const unsigned nblk = (unsigned)((a.ncols + cols - 1) / cols);
+  const size_t shm = sizeof(double2) * (size_t)a.n * cols;
+  hipLaunchKernelGGL(k_dft_axis, dim3(nblk), dim3(256), shm, c->stream, a);
+  LAUNCHCHK("k_dft_axis");
+  return 0;
+}
+
+int salmon_hip_hartree(salmon_hip_t c, const double *rho_dev, double *vh_dev) {
+  if (int rc = check(c)) return rc;
+  if (!rho_dev || !vh_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+  if (!c->have_cell) return fail(SALMON_HIP_ESTATE, "set_poisson not called");
+  double2 *b0 = c->d_fft[0], *b1 = c->d_fft[1];
+  int rc;
+  if ((rc = launch_dft(c, 2, -1.0, rho_dev, nullptr, b0, nullptr, nullptr))) return rc;
+  if ((rc = launch_dft(c, 1, -1.0, nullptr, b0, b1, nullptr, nullptr))) return rc;
+  if ((rc = launch_dft(c, 0, -1.0, nullptr, b1, b0, nullptr, c->d_gcoef))) return rc;
+  if ((rc = launch_dft(c, 2, 1.0, nullptr, b0, b1, nullptr, nullptr))) return rc;
+  if ((rc = launch_dft(c, 1, 1.0, nullptr, b1, b0, nullptr, nullptr))) return rc;
+  return launch_dft(c, 0, 1.0, nullptr, b0, nullptr, vh_dev, nullptr);
+}
+
+int salmon_hip_xc_pz(salmon_hip_t c, const double *rho_dev, double *vxc_dev, double *exc_dev) {
+  if (int rc = check(c)) return rc;
+  if (!rho_dev || !vxc_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+  if (c->nspin != 1) return fail(SALMON_HIP_EUNSUPPORTED, "spin-polarised PZ not implemented");
+  hipLaunchKernelGGL(k_xc_pz, dim3((unsigned)((c->ngrid + 255) / 256)), dim3(256), 0, c->stream, rho_dev,
+                     vxc_dev, exc_dev, c->ngrid);
+  LAUNCHCHK("k_xc_pz");
+  return 0;
+}
+
+static int launch_vlocal(salmon_hip_t c, const double *vpsl, const double *vh, const double *vxc,
+                         const double *rho, const double *vmix, double *vout) {
+  VlocArgs a;
+  a.vpsl = vpsl; a.vh = vh; a.vxc = vxc; a.rho = rho; a.vmix = vmix;
+  a.V = c->d_V;
+  a.vout = vout;
+  a.n = c->ngrid;
+  hipLaunchKernelGGL(k_update_vlocal, dim3((unsigned)((c->ngrid + 255) / 256)), dim3(256), 0, c->stream, a);
+  LAUNCHCHK("k_update_vlocal");
+  c->have_V = true;
+  return 0;
+}
+
+int salmon_hip_update_vlocal(salmon_hip_t c, const double *vpsl_dev, const double *vh_dev, const double *vxc_dev,
+                             const double *vmix_dev, double *vout_dev) {
+  if (int rc = check(c)) return rc;
+  if (!vpsl_dev || !vh_dev || !vxc_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+  if (c->nspin != 1) return fail(SALMON_HIP_EUNSUPPORTED, "nspin=2 potential update not implemented");
+  return launch_vlocal(c, vpsl_dev, vh_dev, vxc_dev, nullptr, vmix_dev, vout_dev);
+}
+
+int salmon_hip_update_potential(salmon_hip_t c, const double *rho_dev, const double *vpsl_dev,
+                                const double *vmix_dev, double *vout_dev) {
+  if (int rc = check(c)) return rc;
+  if (!rho_dev || !vpsl_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+  if (c->nspin != 1) return fail(SALMON_HIP_EUNSUPPORTED, "nspin=2 potential update not implemented");
+  if (int rc = salmon_hip_hartree(c, rho_dev, c->d_vh)) return rc;
+  return launch_vlocal(c, vpsl_dev, c->d_vh, nullptr, rho_dev, vmix_dev, vout_dev);
 }
 
 int salmon_hip_prof_start(salmon_hip_t c, int max_launches) {

@@ -32,6 +32,11 @@ EXPORTS = (
     "salmon_hip_density",
     "salmon_hip_current",
     "salmon_hip_dipole",
+    "salmon_hip_set_poisson",
+    "salmon_hip_hartree",
+    "salmon_hip_xc_pz",
+    "salmon_hip_update_vlocal",
+    "salmon_hip_update_potential",
     "salmon_hip_synchronize",
     "salmon_hip_prof_start",
     "salmon_hip_prof_stop",
@@ -79,6 +84,11 @@ def load():
     L.salmon_hip_density.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_current.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p]
     L.salmon_hip_dipole.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.salmon_hip_set_poisson.argtypes = [C.c_void_p, _dp]
+    L.salmon_hip_hartree.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.salmon_hip_xc_pz.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.salmon_hip_update_vlocal.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+    L.salmon_hip_update_potential.argtypes = [C.c_void_p] + [C.c_void_p] * 4
     L.salmon_hip_synchronize.argtypes = [C.c_void_p]
     L.salmon_hip_prof_start.argtypes = [C.c_void_p, C.c_int]
     L.salmon_hip_prof_stop.argtypes = [C.c_void_p, C.c_int, _dp, _ip]
@@ -187,6 +197,27 @@ class Handle:
 
     def dipole(self, rho, out):
         _chk(load().salmon_hip_dipole(self.h, C.c_void_p(_ptr(rho)), C.c_void_p(_ptr(out))), "dipole")
+
+    def set_poisson(self, al):
+        a = np.ascontiguousarray(al, dtype=np.float64)
+        _chk(load().salmon_hip_set_poisson(self.h, _p(a)), "set_poisson")
+
+    def hartree(self, rho, vh):
+        _chk(load().salmon_hip_hartree(self.h, C.c_void_p(_ptr(rho)), C.c_void_p(_ptr(vh))), "hartree")
+
+    def xc_pz(self, rho, vxc, exc=None):
+        _chk(load().salmon_hip_xc_pz(self.h, C.c_void_p(_ptr(rho)), C.c_void_p(_ptr(vxc)),
+                                     C.c_void_p(_ptr(exc) if exc is not None else None)), "xc_pz")
+
+    def update_vlocal(self, vpsl, vh, vxc, vmix=None, vout=None):
+        opt = [C.c_void_p(_ptr(x) if x is not None else None) for x in (vmix, vout)]
+        _chk(load().salmon_hip_update_vlocal(self.h, C.c_void_p(_ptr(vpsl)), C.c_void_p(_ptr(vh)),
+                                             C.c_void_p(_ptr(vxc)), *opt), "update_vlocal")
+
+    def update_potential(self, rho, vpsl, vmix=None, vout=None):
+        opt = [C.c_void_p(_ptr(x) if x is not None else None) for x in (vmix, vout)]
+        _chk(load().salmon_hip_update_potential(self.h, C.c_void_p(_ptr(rho)), C.c_void_p(_ptr(vpsl)), *opt),
+             "update_potential")
 
     KERNELS = {"hpsi": 0, "hpsi_taylor1": 1, "hpsi_taylor": 2, "hpsi_taylor_last": 3, "proj": 4,
                "proj_current": 5, "nl_scatter": 6}

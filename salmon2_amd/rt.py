@@ -10,6 +10,16 @@ branch of ``time_evolution_step`` (src/rt/time_evolution_step.f90:116-127,
   calc_density(spsi_out) ; calc_current with vec_Ac = Ac_tot(itt) ; subdip
   swap(spsi_in, spsi_out)
 
+With ``scf=True`` (``yn_fix_func = 'n'``) the local potential is rebuilt
+from the new density every step, V_local = Vpsl + Vh[rho] + Vxc[rho]
+(time_evolution_step.f90:182-207), starting from the potential of the initial
+orbitals; ``system.vlocal`` is then Vpsl.  With ``pc=True`` the step is the
+predictor-corrector (time_evolution_step.f90:141-143, 354-423): a predictor
+Taylor step with V(t), V' from its density, V = (V(t) + V')/2, then the
+corrector Taylor step from the same input orbitals.  The library's taylor
+leaves its input untouched, so the orbital copy of the reference
+(psi_tmp, :370-372, 410-412) is not needed.
+
 Orbitals stay resident in HBM for the whole run; per-step observables are
 written to device arrays and read back once at the end.  torch is used only
 for device allocation and the stream.
@@ -23,7 +33,7 @@ from .hip import Handle
 
 
 class RTPropagator:
-    def __init__(self, sysm, device="cuda:0", psi=None):
+    def __init__(self, sysm, device="cuda:0", psi=None, scf=False, pc=False):
         self.s = sysm
         self.dev = torch.device(device)
         torch.cuda.set_device(self.dev)
@@ -36,6 +46,16 @@ class RTPropagator:
         self.rho = torch.zeros((sysm.nspin, sysm.nz, sysm.ny, sysm.nx), dtype=torch.float64, device=self.dev)
         self.zc = sysm.zc()
         self.itt = 0
+        self.scf = bool(scf)
+        self.pc = bool(pc) and self.scf  # with a fixed potential the corrector average is the identity
+        if self.scf:
+            if sysm.nspin != 1:
+                raise ValueError("the self-consistent potential update supports nspin = 1")
+            self.h.set_poisson(sysm.al)
+            self.vpsl = torch.from_numpy(np.ascontiguousarray(sysm.vlocal[0])).to(self.dev)
+            self.vcur = torch.zeros_like(self.vpsl)
+            self.h.density(self.psi_a, self.rho)
+            self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
 
     def setup_outputs(self, nt):
         s = self.s
@@ -50,8 +70,14 @@ class RTPropagator:
         itt = self.itt
         A = 0.5 * (self.ac[itt] + self.ac[itt - 1])
         self.h.update_kvector(A)
+        if self.pc:
+            self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
+            self.h.density(self.psi_b, self.rho)
+            self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur)
         self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
         self.h.density(self.psi_b, self.rho)
+        if self.scf:
+            self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
         self.h.current(self.ac[itt], self.psi_b, self.curr_t[itt - 1])
         self.h.dipole(self.rho, self.dip_t[itt - 1])
         self.psi_a, self.psi_b = self.psi_b, self.psi_a

@@ -1,0 +1,145 @@
+"""Pins the oracle's Hartree, PZ-LDA and self-consistent RT loop.
+
+* Hartree: closed form 4 pi/|G|^2 cos(G.r) for cosine densities (including the
+  Nyquist mode and odd grids), numpy's FFT solution of the periodic Poisson
+  equation G^2 V(G) = 4 pi rho(G) with G = 0 removed, zero for constant rho;
+* PZ81: exchange -0.458165/rs and the correlation values of the published
+  parametrisation at rs = 0.5, 1, 2, 5; v_xc = d(rho e_xc)/d rho by central
+  differences on both branches of the parametrisation;
+* SCF loop: every step rebuilt from dense-matrix Taylor polynomials, numpy
+  densities and the two pinned functionals, with and without the
+  predictor-corrector; electron number never grows.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from tests import dense
+
+
+def _grid(n, al):
+    x = [np.arange(m) * al[d] / m for d, m in enumerate(n)]
+    return np.meshgrid(x[2], x[1], x[0], indexing="ij")  # Z, Y, X with shape (nz, ny, nx)
+
+
+@pytest.mark.parametrize("n", [(16, 18, 20), (15, 12, 9)])
+def test_hartree_cosine_closed_form(oracle_lib, n):
+    al = np.array([10.0, 11.0, 12.5])
+    Z, Y, X = _grid(n, al)
+    rho = np.full(Z.shape, 0.3)
+    ref = np.zeros_like(rho)
+    modes = [((2, -3, 1), 1.0, np.cos), ((0, 1, 0), 0.5, np.sin), ((n[0] // 2, 0, 1), 0.25, np.cos)]
+    for m, amp, f in modes:
+        G = 2 * np.pi * np.array(m) / al
+        ph = G[0] * X + G[1] * Y + G[2] * Z
+        rho = rho + amp * f(ph)
+        ref = ref + amp * 4 * np.pi / G.dot(G) * f(ph)
+    v = oracle_lib.hartree(rho, al)
+    assert np.abs(v - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_hartree_matches_fft_poisson(oracle_lib):
+    rng = np.random.default_rng(3)
+    n = (12, 10, 14)
+    al = np.array([9.0, 8.0, 10.0])
+    rho = rng.random((n[2], n[1], n[0]))
+    v = oracle_lib.hartree(rho, al)
+    g = [2 * np.pi * np.fft.fftfreq(n[d], d=al[d] / n[d]) for d in range(3)]
+    G2 = g[2][:, None, None] ** 2 + g[1][None, :, None] ** 2 + g[0][None, None, :] ** 2
+    G2[0, 0, 0] = 1.0
+    rg = np.fft.fftn(rho)
+    rg[0, 0, 0] = 0.0
+    ref = np.fft.ifftn(4 * np.pi * rg / G2).real
+    assert np.abs(v - ref).max() < 1e-12 * np.abs(ref).max()
+    # a constant density carries no Hartree potential (G = 0 removed)
+    assert np.abs(oracle_lib.hartree(np.full((n[2], n[1], n[0]), 0.7), al)).max() < 1e-13
+
+
+def _rho_of_rs(rs):
+    return 3.0 / (4.0 * np.pi * np.asarray(rs, dtype=float) ** 3)
+
+
+def test_pz_published_values(oracle_lib):
+    rs = np.array([0.5, 1.0, 2.0, 5.0])
+    v, e = oracle_lib.xc_pz(_rho_of_rs(rs))
+    ex = -0.4581652932831429 / rs  # -(3/4)(9/(4 pi^2))^(1/3)/rs
+    # Perdew-Zunger 1981 correlation (Hartree): high density A ln rs + B + C rs ln rs + D rs,
+    # low density gamma/(1 + beta1 sqrt(rs) + beta2 rs)
+    ec_pub = np.array([-0.0760500, -0.0596000, -0.0450912, -0.0283390])
+    assert np.abs(e - ex - ec_pub).max() < 2e-7
+
+
+@pytest.mark.parametrize("rs", [0.3, 0.7, 0.95, 1.05, 1.5, 3.0, 8.0])
+def test_pz_potential_is_energy_derivative(oracle_lib, rs):
+    rho = float(_rho_of_rs(rs))
+    h = 1e-5 * rho
+    r = np.array([rho - h, rho + h, rho])
+    v, e = oracle_lib.xc_pz(r)
+    dfd = (r[1] * e[1] - r[0] * e[0]) / (2 * h)
+    assert abs(v[2] - dfd) < 1e-8 * abs(v[2])
+
+
+def _dense_density(s, psi):
+    rho = np.zeros((s.nz, s.ny, s.nx))
+    for ik in range(s.nk):
+        for io in range(s.no):
+            rho += np.abs(psi[ik, io, 0][:, : s.ny, :]) ** 2 * s.rocc[0, ik, io] * s.wtk[ik]
+    return rho
+
+
+def _dense_step(s, H0, V, psi, dt, n_hamil):
+    out = np.zeros_like(psi)
+    for ik in range(s.nk):
+        H = H0[ik] + np.diag(V.ravel())
+        for io in range(s.no):
+            v = dense.interior(psi[ik, io, 0], s)
+            acc = v.copy()
+            term = v.copy()
+            for n in range(1, n_hamil + 1):
+                term = (-1j * dt / n) * (H @ term)
+                acc = acc + term
+            out[ik, io, 0][:, : s.ny, :] = acc.reshape(s.nz, s.ny, s.nx)
+    return out
+
+
+@pytest.mark.parametrize("pc", [False, True])
+def test_scf_loop_dense(oracle_lib, tiny, pc):
+    s = tiny
+    nt = 3
+    O = oracle_lib.Oracle(s)
+    p_or, cur, dip, rne, vloc = O.rt_run_scf(pc, nt=nt)
+
+    vpsl = s.vlocal[0]
+    ac = s.pulse.table(s.dt, nt)
+    s0 = dataclasses.replace(s, vlocal=np.zeros_like(s.vlocal))
+
+    def potential(rho):
+        return vpsl + oracle_lib.hartree(rho, s.al) + oracle_lib.xc_pz(rho)[0]
+
+    psi = np.array(s.psi)
+    V = potential(_dense_density(s, psi))
+    for itt in range(1, nt + 1):
+        A = 0.5 * (ac[itt] + ac[itt - 1])
+        H0 = [dense.full_hamiltonian(s0, s.vec_k[ik] + A) for ik in range(s.nk)]
+        if pc:
+            pred = _dense_step(s, H0, V, psi, s.dt, s.n_hamil)
+            V = 0.5 * (V + potential(_dense_density(s, pred)))
+        psi = _dense_step(s, H0, V, psi, s.dt, s.n_hamil)
+        rho = _dense_density(s, psi)
+        V = potential(rho)
+        assert abs(rne[itt - 1] - rho.sum() * s.hvol) < 1e-12 * rne[itt - 1]
+    scale = np.abs(psi).max()
+    assert np.abs(p_or - psi).max() < 1e-12 * scale
+    assert np.abs(vloc - V).max() < 1e-12 * np.abs(V).max()
+    # the truncated Taylor series contracts the high-frequency part of random orbitals
+    # (|1 - ix - x^2/2 + i x^3/6 + x^4/24| < 1 for 0 < x < 2 sqrt 2), so rNe never grows
+    assert np.all(np.diff(rne) <= 1e-12 * rne[0])
+
+
+def test_scf_pc_differs_from_fixed_potential(oracle_lib, tiny):
+    """The potential update changes the trajectory (the test above would pass on a no-op otherwise)."""
+    O = oracle_lib.Oracle(tiny)
+    _, c_fix, _, _ = O.rt_run(nt=3)
+    _, c_scf, _, _, _ = O.rt_run_scf(True, nt=3)
+    assert np.abs(c_fix - c_scf).max() > 1e-6 * np.abs(c_fix).max()
