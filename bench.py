@@ -3,13 +3,22 @@
 Workload (default): 64-atom Si supercell (2x2x2 diamond cells, a = 10.26 bohr),
 32^3 grid, 128 complex128 orbitals x 8 k-points (2x2x2 Monkhorst-Pack), s+p
 Gaussian KB projectors, 4th-order Taylor, dt = 0.08 au, sin^2 A(t) pulse of
-amplitude 0.005 au.  One step = one full RT step (phase table, 4 H applications
-with the Taylor accumulation, density, current, dipole) on HBM-resident
-orbitals.  Synthetic seeded inputs.
+amplitude 0.005 au.  Synthetic seeded inputs, orbitals HBM-resident.
+
+Forms timed on the same initial state (each for warmup + steps RT steps):
+  fixedv-fp64  V_local fixed (yn_fix_func = 'y'): phase table, Taylor step
+               (4 H applications + sum), density, current, dipole
+  pc-fp64      the reference's self-consistent step with the predictor-corrector
+               (yn_fix_func = 'n', yn_predictor_corrector = 'y'): two Taylor
+               steps, two Hartree + PZ-LDA potential updates per step; plain FP64,
+               the baseline of the metric
+  pc-mixed     the same step with the Taylor chain in complex FP32 (sums FP64)
+Each form's current, dipole and electron count are compared with pc-fp64's
+over the whole run.  `value` is the steps/s of the headline form.
 
 Prints ONE JSON line.  The roofline entry is for the dominant kernel, the
-stencil of the middle Taylor terms (k_stencil), timed live with start/stop
-events recorded by its own launches over the timed region.
+stencil of the middle Taylor terms (k_stencil) in pc-fp64, timed live with
+start/stop events recorded by its own launches over the timed region.
 """
 from __future__ import annotations
 
@@ -22,26 +31,36 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = "RT steps/s: learned+mixed vs plain FP64 GPU at ≤1e-6 rel current error"
+METRIC = "RT steps/s: learned+mixed vs plain FP64 GPU at \u22641e-6 rel current error"
 HBM_PEAK_GBS = 8000.0
+
+FORMS = {
+    "fixedv-fp64": dict(scf=False, pc=False, precision="fp64"),
+    "pc-fp64": dict(scf=True, pc=True, precision="fp64"),
+    "pc-mixed": dict(scf=True, pc=True, precision="mixed"),
+}
+BASE_FORM = "pc-fp64"
+HEADLINE_FORM = "pc-mixed"
 
 
 def cpu_baseline(sysm, steps: int):
-    """The oracle (C/OpenMP restatement of SALMON's CPU loops) on a bounded sample."""
+    """The oracle (C/OpenMP restatement of SALMON's CPU loops) on a bounded sample of the
+    baseline form (self-consistent potential, predictor-corrector)."""
     from oracle import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     O = oracle.Oracle(sysm)
     t0 = time.perf_counter()
-    O.rt_run(nt=steps)
+    O.rt_run_scf(True, nt=steps)
     dt = time.perf_counter() - t0
     return {
         "value": steps / dt,
         "unit": "RT steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{steps} RT steps of the same workload (oracle/salmon_oracle.c, OpenMP, {threads} threads)",
+        "sample": f"{steps} predictor-corrector RT step(s) of the same workload, initial potential included "
+                  f"(oracle/salmon_oracle.c, OpenMP, {threads} threads)",
     }
 
 
@@ -58,13 +77,48 @@ def measured_traffic(workload: str, kernel: str):
     return None if e is None else e["read_bytes"] + e["write_bytes"]
 
 
+def run_form(s, form, steps, warmup, prof):
+    import numpy as np
+    import torch
+
+    from salmon2_amd.rt import RTPropagator
+
+    prop = RTPropagator(s, **FORMS[form])
+    prop.setup_outputs(warmup + steps)
+    for _ in range(warmup):
+        prop.step()
+    torch.cuda.synchronize()
+    if prof:
+        prop.h.prof_start(max_launches=32 * steps + 32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prop.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kern = {}
+    if prof:
+        for k in ("hpsi_taylor1", "hpsi_taylor", "hpsi_taylor_last", "proj", "proj_current", "nl_scatter"):
+            ms, n = prop.h.prof_read(k)
+            kern[k] = {"avg_ms": ms / max(n, 1), "launches": n}
+        prop.h.prof_end()
+    res = prop.results()
+    res["psi"] = None
+    assert np.all(np.isfinite(res["curr"])), f"non-finite current in {form}"
+    prop.h.close()
+    del prop
+    torch.cuda.empty_cache()
+    return dict(elapsed=t1 - t0, kern=kern, **res)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--forms", default=",".join(FORMS))
+    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the run, copy 1 GiB device-to-device (HBM-traffic counter calibration)")
@@ -79,33 +133,40 @@ def main():
     import torch
 
     from salmon2_amd import system
-    from salmon2_amd.rt import RTPropagator
 
     s = system.config(args.config)
-    prop = RTPropagator(s)
-    prop.setup_outputs(args.warmup + args.steps)
-    for _ in range(args.warmup):
-        prop.step()
-    torch.cuda.synchronize()
-    prop.h.prof_start(max_launches=16 * args.steps + 16)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        prop.step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    kern = {}
-    for k in ("hpsi_taylor1", "hpsi_taylor", "hpsi_taylor_last", "proj", "proj_current", "nl_scatter"):
-        ms, n = prop.h.prof_read(k)
-        kern[k] = {"avg_ms": ms / max(n, 1), "launches": n}
-    prop.h.prof_end()
-    res = prop.results()
-    assert np.all(np.isfinite(res["curr"])), "non-finite current"
+    forms = [f for f in args.forms.split(",") if f]
+    if BASE_FORM not in forms:
+        forms.insert(0, BASE_FORM)
+    runs = {}
+    for f in forms:
+        runs[f] = run_form(s, f, args.steps, args.warmup, prof=(f == BASE_FORM))
+        print(f"[bench] {f}: {args.steps / runs[f]['elapsed']:.2f} steps/s", file=sys.stderr, flush=True)
 
-    elapsed = t1 - t0
-    steps_per_s = args.steps / elapsed
     npts = s.nx * s.ny * s.nz
     norb = s.norb
+    base = runs[BASE_FORM]
+
+    def rel(a, b):
+        return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
+
+    form_out = {}
+    for f, r in runs.items():
+        sps = args.steps / r["elapsed"]
+        form_out[f] = {
+            "steps_per_s": sps,
+            "ms_per_step": 1e3 * r["elapsed"] / args.steps,
+            "speedup_vs_" + BASE_FORM: sps / (args.steps / base["elapsed"]),
+            "grid_point_orbital_updates_per_s": sps * npts * norb,
+        }
+        if f != BASE_FORM and FORMS[f]["scf"]:
+            form_out[f]["rel_err_vs_" + BASE_FORM] = {
+                "current": rel(r["curr"], base["curr"]),
+                "dipole": rel(r["dip"], base["dip"]),
+                "electron_count": float(np.abs(r["rne"] - base["rne"]).max() / base["rne"][0]),
+            }
+    head = HEADLINE_FORM if HEADLINE_FORM in runs else BASE_FORM
+    kern = base["kern"]
     # algorithmic bytes of one middle-term launch: read H^(n-1) psi (16 B) and write H^n psi (16 B)
     # per point-orbital, plus V_local once (8 B per point)
     alg_bytes = 32.0 * npts * norb + 8.0 * npts
@@ -113,26 +174,28 @@ def main():
     achieved = alg_bytes / avg_s / 1e9
     out = {
         "metric": METRIC,
-        "value": steps_per_s,
+        "value": form_out[head]["steps_per_s"],
         "unit": "RT steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": form_out[head]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if FORMS[head]["precision"] == "fp64" else "c64 chain / f64 sums",
         "data": "synthetic",
         "config": {
             "workload": f"{args.config}: Si {s.natom} atoms, {s.nx}x{s.ny}x{s.nz} grid, {s.no} orbitals x {s.nk} k, "
-                        f"{s.nlma} KB projectors, Taylor order {s.n_hamil}, dt {s.dt}, plain FP64",
-            "form": "plain-fp64",
-            "grid_point_orbital_updates_per_s": steps_per_s * npts * norb,
+                        f"{s.nlma} KB projectors, Taylor order {s.n_hamil}, dt {s.dt}",
+            "form": head,
+            "baseline_form": BASE_FORM,
+            "grid_point_orbital_updates_per_s": form_out[head]["grid_point_orbital_updates_per_s"],
         },
+        "forms": form_out,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_stencil (middle Taylor terms: 25-point stencil + V + k.grad, LDS-DMA fed)",
+            "kernel": "k_stencil (middle Taylor terms of pc-fp64: 25-point stencil + V + k.grad, LDS-DMA fed)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -143,7 +206,7 @@ def main():
         },
         "kernels_ms": {k: round(v["avg_ms"], 5) for k, v in kern.items()},
     }
-    prop = None
+    runs = None
     torch.cuda.empty_cache()
     if args.calib_copy:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")

@@ -123,6 +123,20 @@ int salmon_hip_taylor(salmon_hip_t h, const double *zc, int n_hamil, void *tspsi
                       void *tspsi_out, void *sshtpsi);
 
 /*
+ * Arithmetic of salmon_hip_taylor (no counterpart in the reference, which is
+ * FP64 throughout).  SALMON_HIP_PREC_FP64 (default): as above.
+ * SALMON_HIP_PREC_MIXED: H psi, H^2 psi, ... are formed and kept as complex FP32
+ * (25-point stencil, V_local and k+A products in FP32), while the projections
+ * <p|H^n psi>, the KB term, the Taylor sum psi + sum zc(n) H^n psi and the
+ * orbitals themselves stay complex FP64; needs n_hamil >= 2 and leaves sshtpsi
+ * unwritten (the FP32 chain lives in handle scratch).  hpsi, density and
+ * current are FP64 in both modes.
+ */
+#define SALMON_HIP_PREC_FP64 0
+#define SALMON_HIP_PREC_MIXED 1
+int salmon_hip_set_precision(salmon_hip_t h, int mode);
+
+/*
  * rho(ix,iy,iz,ispin) = sum_{ik,io} rocc*wtk*|psi|^2 into a DEVICE array
  * (calc_density, src/common/density_matrix.f90:26-236).
  */

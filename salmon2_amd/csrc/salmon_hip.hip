@@ -68,6 +68,24 @@ constexpr int kDensGroup = 32;   // orbitals per density partial
 
 __device__ __forceinline__ int wrapi(int v, int n) { return (v + 8 * n) % n; }
 
+// Complex element types: double2 (FP64) or float2 (FP32, mixed-precision chain).
+// A float2 element is moved by two dword DMAs per 64-point wave chunk (lane i ->
+// point 32k + i/2, half i%2), a double2 element by one dwordx4 DMA (lane i -> point i).
+template <typename R> struct Cx;
+template <> struct Cx<double> { using T = double2; };
+template <> struct Cx<float> { using T = float2; };
+template <typename E> struct DmaW;
+template <> struct DmaW<double2> { static constexpr int W = 1; };
+template <> struct DmaW<float2> { static constexpr int W = 2; };
+
+template <typename D, typename S>
+__device__ __forceinline__ D cvt2(const S &v) {
+  D r;
+  r.x = v.x;
+  r.y = v.y;
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // Phase table and per-k stencil coefficients
 // ---------------------------------------------------------------------------
@@ -161,7 +179,7 @@ __global__ void k_set_kpar(KvecArgs a) {
 // re and im rows of one lw sit in registers (2h, 2h+1) of the same lane.
 // ---------------------------------------------------------------------------
 struct ProjArgs {
-  const double2 *psi;     // orbital block base
+  const void *psi;        // orbital block base (double2, or float2 in the mixed-precision chain)
   const double2 *zekr;    // (nps, nlma, nk)
   const double *rxyz;     // (3, nps, natom)
   const int *jidx;        // (nps, natom) offsets within an orbital block
@@ -173,7 +191,7 @@ struct ProjArgs {
   int nb, nk, natom, nps, nlma, ngrp;
 };
 
-template <int NW>
+template <int NW, typename TP>
 __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
   constexpr int MAXT = (NW * kMaxLmaPerAtom + 7) / 8;
   __shared__ double2 sp[kProjJC][kProjOrb];
@@ -208,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
       const int j = j0 + jj;
       const int ob = grp * kProjOrb + o;
       double2 v = make_double2(0.0, 0.0);
-      if (j < m && ob < a.nb) v = a.psi[(kbase + ob) * a.osz + a.jidx[(size_t)a.nps * ia + j]];
+      if (j < m && ob < a.nb) v = cvt2<double2>(((const TP *)a.psi)[(kbase + ob) * a.osz + a.jidx[(size_t)a.nps * ia + j]]);
       sp[jj][o] = v;
     }
     __syncthreads();
@@ -316,11 +334,24 @@ __device__ __forceinline__ void wait_vm() {
   asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
 
+
+// One element chunk of 64 points (this wave's) into LDS at byte address m0:
+// W = 1: off[0] = the lane's point; W = 2: off[k] = point 32k + lane/2.
+template <typename E>
+__device__ __forceinline__ void dma_chunk(const E *base, const size_t *off, unsigned m0, int lane) {
+  if constexpr (DmaW<E>::W == 1) {
+    dma16(base + off[0], m0);
+  } else {
+    dma4((const float *)(base + off[0]) + (lane & 1), m0);
+    dma4((const float *)(base + off[1]) + (lane & 1), m0 + 256u);
+  }
+}
+
 struct StencilArgs {
-  const double2 *in;
-  double2 *hout;
+  const void *in;
+  void *hout;
   double2 *sink;          // kTile elements absorbing the stores of lanes without a point
-  const double2 *comb[kMaxComb];
+  const void *comb[kMaxComb];  // comb[0] double2 (psi); comb[n >= 1] the chain type
   double2 zc[kMaxComb + 1];
   const double *V;        // (nx, ny, nz, nspin)
   const double *kpar;     // (16, nk)
@@ -330,18 +361,27 @@ struct StencilArgs {
   int TY, TZ, nyt, nzt, nblk;
 };
 
-template <int HC, int NC>
+// TI: element type of `in`; R: arithmetic of the stencil and local-potential
+// products; TO: element type of `hout`.  Plain FP64: (double2, double, double2).
+// Mixed form: (double2, float, float2) for the first term, (float2, float, float2)
+// for the middle ones and (float2, float, double2) for the last, whose Taylor sum
+// is formed in FP64 from psi (FP64) and h_1 .. h_N (FP32).
+template <typename TI, typename R, typename TO, int HC, int NC>
 __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
+  using CH = typename Cx<R>::T;
   constexpr bool FINAL = NC >= 0;
   constexpr int D = kDmaDepth;
-  // DMA instructions per wave per plane, in issue order: comb (NC), tile, halo (HC), V (2 x dword);
+  constexpr int WI = DmaW<TI>::W;
+  constexpr int WC = DmaW<CH>::W;
+  constexpr unsigned EB = sizeof(TI);
+  // DMA instructions per wave per plane, in issue order: comb, tile (WI), halo (HC*WI), V (2 x dword);
   // then 1 store
-  constexpr int kOpsAfterComb = 1 + HC + 2 + 1;
+  constexpr int kOpsAfterComb = WI + HC * WI + 2 + 1;
   extern __shared__ double2 lds[];
-  double2 *tileR = lds;                                   // kRingT x kTile
-  double2 *haloR = tileR + kRingT * kTile;                // kRingH x HC*kTile
+  TI *tileR = (TI *)lds;                                  // kRingT x kTile
+  TI *haloR = tileR + kRingT * kTile;                     // kRingH x HC*kTile
   double *vR = (double *)(haloR + kRingH * HC * kTile);   // kRingH x kTile doubles
-  double2 *combR = (double2 *)(vR + kRingH * kTile);      // 2 x NC x kTile
+  char *combR = (char *)(vR + kRingH * kTile);            // 2 x NC x kTile x 16 bytes
 
   int bid = blockIdx.x;
   const int nfull = (a.nblk / 8) * 8;
@@ -361,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
   const int lane = tid & 63, wave = tid >> 6;
   const int ntile = TY * nx;
   const bool active = tid < ntile;
-  const int te = active ? tid : 0;          // element this lane loads (dummy lanes repeat element 0)
+  const int te = active ? tid : 0;          // element this lane computes (dummy lanes repeat element 0)
   const int tx = te % nx;
   const int ty = te / nx;
   const int y = y0 + ty;
@@ -370,9 +410,24 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
 
   const size_t zs = (size_t)nx * ly;
   const size_t orbOff = (size_t)b * a.osz;
-  const double2 *in = a.in + orbOff;
+  const TI *in = (const TI *)a.in + orbOff;
   const size_t colOff = (size_t)tx + (size_t)nx * yw;
   const size_t gplane = (size_t)nx * ny;
+  // tile points this lane moves (W = 1: its own; W = 2: points 32k + lane/2 of the wave's chunk)
+  auto tile_off = [&](int W, size_t *off) {
+    if (W == 1) {
+      off[0] = colOff;
+      return;
+    }
+    for (int k = 0; k < 2; ++k) {
+      const int e = 64 * wave + 32 * k + (lane >> 1);
+      const int ec = e < ntile ? e : 0;
+      off[k] = (size_t)(ec % nx) + (size_t)nx * wrapi(y0 + ec / nx, ny);
+    }
+  };
+  size_t toffI[2], toffC[2];
+  tile_off(WI, toffI);
+  tile_off(WC, toffC);
   // V: two dword DMAs per wave, each covering 32 points of the wave's 64 (lane i -> point i/2, half i%2)
   const double *Vb = a.V + (size_t)ispin * gplane * nz;
   size_t voff[2];
@@ -383,29 +438,32 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
     voff[k] = 2 * ((size_t)(ec % nx) + (size_t)nx * wrapi(y0 + ec / nx, ny)) + (lane & 1);
   }
 
-  // halo elements of this lane: chunk c = wave + 4k, element 64c + lane
-  size_t hoff[HC];
+  // halo points of this lane: chunk c = wave + 4k, points 64c + (lane | 32kk + lane/2)
+  size_t hoff[HC][2];
 #pragma unroll
-  for (int k = 0; k < HC; ++k) {
-    const int e = 64 * (wave + 4 * k) + lane;
-    const int ec = e < 8 * nx ? e : 0;
-    const int hr = ec / nx, hx = ec % nx;
-    const int gy = hr < 4 ? y0 - 4 + hr : y0 + TY + (hr - 4);
-    hoff[k] = (size_t)hx + (size_t)nx * wrapi(gy, ny);
-  }
+  for (int k = 0; k < HC; ++k)
+#pragma unroll
+    for (int kk = 0; kk < WI; ++kk) {
+      const int e = 64 * (wave + 4 * k) + (WI == 1 ? lane : 32 * kk + (lane >> 1));
+      const int ec = e < 8 * nx ? e : 0;
+      const int hr = ec / nx, hx = ec % nx;
+      const int gy = hr < 4 ? y0 - 4 + hr : y0 + TY + (hr - 4);
+      hoff[k][kk] = (size_t)hx + (size_t)nx * wrapi(gy, ny);
+    }
   // LDS byte addresses of this wave's chunk in each ring (uniform per wave)
-  const unsigned tileA = __builtin_amdgcn_readfirstlane(lds_addr(tileR) + 16u * 64u * wave);
-  const unsigned haloA = __builtin_amdgcn_readfirstlane(lds_addr(haloR) + 16u * 64u * wave);
+  const unsigned tileA = __builtin_amdgcn_readfirstlane(lds_addr(tileR) + EB * 64u * wave);
+  const unsigned haloA = __builtin_amdgcn_readfirstlane(lds_addr(haloR) + EB * 64u * wave);
   const unsigned vA = __builtin_amdgcn_readfirstlane(lds_addr(vR) + 8u * 64u * wave);
   const unsigned combA = __builtin_amdgcn_readfirstlane(lds_addr(combR) + 16u * 64u * wave);
+  const unsigned combAC = __builtin_amdgcn_readfirstlane(lds_addr(combR) + (unsigned)sizeof(CH) * 64u * wave);
 
   auto dma_tile = [&](int plane, int slot) {
-    dma16(in + colOff + zs * wrapi(plane, nz), tileA + 16u * kTile * slot);
+    dma_chunk<TI>(in + zs * wrapi(plane, nz), toffI, tileA + EB * kTile * slot, lane);
   };
   auto dma_halo = [&](int plane, int slot) {
 #pragma unroll
     for (int k = 0; k < HC; ++k)
-      dma16(in + hoff[k] + zs * wrapi(plane, nz), haloA + 16u * (HC * kTile * slot + 4 * 64 * k));
+      dma_chunk<TI>(in + zs * wrapi(plane, nz), hoff[k], haloA + EB * (HC * kTile * slot + 4 * 64 * k), lane);
   };
   auto dma_v = [&](int plane, int slot) {
     const float *vb = (const float *)Vb + 2 * gplane * wrapi(plane, nz);
@@ -413,9 +471,18 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
     dma4(vb + voff[1], vA + 8u * kTile * slot + 256u);
   };
   auto dma_comb = [&](int plane, int slot) {
+    const size_t pl = orbOff + zs * wrapi(plane, nz);
 #pragma unroll
-    for (int n = 0; n < NC; ++n)
-      dma16(a.comb[n] + orbOff + colOff + zs * wrapi(plane, nz), combA + 16u * kTile * (NC * slot + n));
+    for (int n = 0; n < (NC > 0 ? NC : 1); ++n) {
+      if (n >= NC) break;
+      // every comb array owns kTile x 16 bytes per slot; psi is double2, the chain arrays CH
+      if (n == 0) {
+        size_t o0[1] = {colOff};
+        dma_chunk<double2>((const double2 *)a.comb[0] + pl, o0, combA + 16u * kTile * (NC * slot), lane);
+      } else {
+        dma_chunk<CH>((const CH *)a.comb[n] + pl, toffC, combAC + 16u * kTile * (NC * slot + n), lane);
+      }
+    }
   };
 
   // y-neighbour LDS offsets: rows ty+n (n = 1..4, -1..-4) in the tile slot or the halo slot
@@ -440,15 +507,18 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
     xo[3 + n] = ty * nx + wrapi(tx - n, nx);
   }
   const double *kp = a.kpar + 16 * ik;
-  const double klap0 = kp[0];
-  double nab[12];
+  const R klap0 = (R)kp[0];
+  R nab[12], lap[12];
 #pragma unroll
-  for (int i = 0; i < 12; ++i) nab[i] = kp[1 + i];
+  for (int i = 0; i < 12; ++i) {
+    nab[i] = (R)kp[1 + i];
+    lap[i] = (R)a.lap[i];
+  }
 
   // prologue: queue planes z0-4 .. z0+3 (plain loads), rings for planes z0 .. z0+3+D
-  double2 q[9];
+  CH q[9];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) q[i] = in[colOff + zs * wrapi(z0 - 4 + i, nz)];
+  for (int i = 0; i < 8; ++i) q[i] = cvt2<CH>(in[colOff + zs * wrapi(z0 - 4 + i, nz)]);
   __builtin_amdgcn_s_waitcnt(0);  // compiler-visible: the queue loads are complete
   if (FINAL) dma_comb(z0, z0 & 1);
   for (int p = 0; p < 4 + D; ++p) dma_tile(z0 + p, (z0 + p) % kRingT);
@@ -470,64 +540,69 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
     dma_halo(z + D, sHn);
     dma_v(z + D, sHn);
 
-    const double2 *T = tileR + kTile * sT;
-    const double2 *H = haloR + HC * kTile * sH;
-    q[8] = tileR[kTile * ((sT + 4) % kRingT) + te];
-    double2 v, w;
+    const TI *T = tileR + kTile * sT;
+    const TI *H = haloR + HC * kTile * sH;
+    q[8] = cvt2<CH>(tileR[kTile * ((sT + 4) % kRingT) + te]);
+    CH v, w;
     {
-      const double2 p1 = T[xo[0]], p2 = T[xo[1]], p3 = T[xo[2]], p4 = T[xo[3]];
-      const double2 m1 = T[xo[4]], m2 = T[xo[5]], m3 = T[xo[6]], m4 = T[xo[7]];
-      v.x = a.lap[0] * (p1.x + m1.x) + a.lap[1] * (p2.x + m2.x) + a.lap[2] * (p3.x + m3.x) + a.lap[3] * (p4.x + m4.x);
-      v.y = a.lap[0] * (p1.y + m1.y) + a.lap[1] * (p2.y + m2.y) + a.lap[2] * (p3.y + m3.y) + a.lap[3] * (p4.y + m4.y);
+      const CH p1 = cvt2<CH>(T[xo[0]]), p2 = cvt2<CH>(T[xo[1]]), p3 = cvt2<CH>(T[xo[2]]), p4 = cvt2<CH>(T[xo[3]]);
+      const CH m1 = cvt2<CH>(T[xo[4]]), m2 = cvt2<CH>(T[xo[5]]), m3 = cvt2<CH>(T[xo[6]]), m4 = cvt2<CH>(T[xo[7]]);
+      v.x = lap[0] * (p1.x + m1.x) + lap[1] * (p2.x + m2.x) + lap[2] * (p3.x + m3.x) + lap[3] * (p4.x + m4.x);
+      v.y = lap[0] * (p1.y + m1.y) + lap[1] * (p2.y + m2.y) + lap[2] * (p3.y + m3.y) + lap[3] * (p4.y + m4.y);
       w.x = nab[0] * (p1.x - m1.x) + nab[1] * (p2.x - m2.x) + nab[2] * (p3.x - m3.x) + nab[3] * (p4.x - m4.x);
       w.y = nab[0] * (p1.y - m1.y) + nab[1] * (p2.y - m2.y) + nab[2] * (p3.y - m3.y) + nab[3] * (p4.y - m4.y);
     }
     {
-      const double2 p1 = (yhal[0] ? H : T)[yoff[0]], p2 = (yhal[1] ? H : T)[yoff[1]];
-      const double2 p3 = (yhal[2] ? H : T)[yoff[2]], p4 = (yhal[3] ? H : T)[yoff[3]];
-      const double2 m1 = (yhal[4] ? H : T)[yoff[4]], m2 = (yhal[5] ? H : T)[yoff[5]];
-      const double2 m3 = (yhal[6] ? H : T)[yoff[6]], m4 = (yhal[7] ? H : T)[yoff[7]];
-      v.x += a.lap[4] * (p1.x + m1.x) + a.lap[5] * (p2.x + m2.x) + a.lap[6] * (p3.x + m3.x) + a.lap[7] * (p4.x + m4.x);
-      v.y += a.lap[4] * (p1.y + m1.y) + a.lap[5] * (p2.y + m2.y) + a.lap[6] * (p3.y + m3.y) + a.lap[7] * (p4.y + m4.y);
+      const CH p1 = cvt2<CH>((yhal[0] ? H : T)[yoff[0]]), p2 = cvt2<CH>((yhal[1] ? H : T)[yoff[1]]);
+      const CH p3 = cvt2<CH>((yhal[2] ? H : T)[yoff[2]]), p4 = cvt2<CH>((yhal[3] ? H : T)[yoff[3]]);
+      const CH m1 = cvt2<CH>((yhal[4] ? H : T)[yoff[4]]), m2 = cvt2<CH>((yhal[5] ? H : T)[yoff[5]]);
+      const CH m3 = cvt2<CH>((yhal[6] ? H : T)[yoff[6]]), m4 = cvt2<CH>((yhal[7] ? H : T)[yoff[7]]);
+      v.x += lap[4] * (p1.x + m1.x) + lap[5] * (p2.x + m2.x) + lap[6] * (p3.x + m3.x) + lap[7] * (p4.x + m4.x);
+      v.y += lap[4] * (p1.y + m1.y) + lap[5] * (p2.y + m2.y) + lap[6] * (p3.y + m3.y) + lap[7] * (p4.y + m4.y);
       w.x += nab[4] * (p1.x - m1.x) + nab[5] * (p2.x - m2.x) + nab[6] * (p3.x - m3.x) + nab[7] * (p4.x - m4.x);
       w.y += nab[4] * (p1.y - m1.y) + nab[5] * (p2.y - m2.y) + nab[6] * (p3.y - m3.y) + nab[7] * (p4.y - m4.y);
     }
-    v.x += a.lap[8] * (q[5].x + q[3].x) + a.lap[9] * (q[6].x + q[2].x) + a.lap[10] * (q[7].x + q[1].x) + a.lap[11] * (q[8].x + q[0].x);
-    v.y += a.lap[8] * (q[5].y + q[3].y) + a.lap[9] * (q[6].y + q[2].y) + a.lap[10] * (q[7].y + q[1].y) + a.lap[11] * (q[8].y + q[0].y);
+    v.x += lap[8] * (q[5].x + q[3].x) + lap[9] * (q[6].x + q[2].x) + lap[10] * (q[7].x + q[1].x) + lap[11] * (q[8].x + q[0].x);
+    v.y += lap[8] * (q[5].y + q[3].y) + lap[9] * (q[6].y + q[2].y) + lap[10] * (q[7].y + q[1].y) + lap[11] * (q[8].y + q[0].y);
     w.x += nab[8] * (q[5].x - q[3].x) + nab[9] * (q[6].x - q[2].x) + nab[10] * (q[7].x - q[1].x) + nab[11] * (q[8].x - q[0].x);
     w.y += nab[8] * (q[5].y - q[3].y) + nab[9] * (q[6].y - q[2].y) + nab[10] * (q[7].y - q[1].y) + nab[11] * (q[8].y - q[0].y);
 
-    const double2 t = q[4];
-    const double vcur = vR[kTile * sH + te];
-    double2 h;
-    h.x = (vcur * t.x + klap0 * t.x) - 0.5 * v.x + w.y;
-    h.y = (vcur * t.y + klap0 * t.y) - 0.5 * v.y - w.x;
-    double2 o = h;
-    if (FINAL) {
+    const CH t = q[4];
+    const R vcur = (R)vR[kTile * sH + te];
+    CH hh;
+    hh.x = (vcur * t.x + klap0 * t.x) - (R)0.5 * v.x + w.y;
+    hh.y = (vcur * t.y + klap0 * t.y) - (R)0.5 * v.y - w.x;
+    TO o;
+    if constexpr (FINAL) {
       constexpr int N = NC + 1;
-      const double2 *C = combR + kTile * (NC * (z & 1));
+      const double2 td = cvt2<double2>(t), h = cvt2<double2>(hh);
+      const char *Cb = combR + 16u * kTile * (NC * (z & 1));
+      double2 od;
       if (N == 1) {
-        o = t;
+        od = td;
       } else {
-        o = C[te];
+        od = ((const double2 *)Cb)[te];
 #pragma unroll
         for (int n = 1; n < N - 1; ++n) {
-          const double2 cc = C[kTile * n + te];
+          const double2 cc = cvt2<double2>(((const CH *)(Cb + 16u * kTile * n))[te]);
           const double2 zc = a.zc[n - 1];
-          o.x = o.x + (zc.x * cc.x - zc.y * cc.y);
-          o.y = o.y + (zc.x * cc.y + zc.y * cc.x);
+          od.x = od.x + (zc.x * cc.x - zc.y * cc.y);
+          od.y = od.y + (zc.x * cc.y + zc.y * cc.x);
         }
         const double2 zc = a.zc[N > 1 ? N - 2 : 0];
-        o.x = o.x + (zc.x * t.x - zc.y * t.y);
-        o.y = o.y + (zc.x * t.y + zc.y * t.x);
+        od.x = od.x + (zc.x * td.x - zc.y * td.y);
+        od.y = od.y + (zc.x * td.y + zc.y * td.x);
       }
       const double2 zc = a.zc[N - 1];
-      o.x = o.x + (zc.x * h.x - zc.y * h.y);
-      o.y = o.y + (zc.x * h.y + zc.y * h.x);
+      od.x = od.x + (zc.x * h.x - zc.y * h.y);
+      od.y = od.y + (zc.x * h.y + zc.y * h.x);
+      o = cvt2<TO>(od);
+    } else {
+      o = cvt2<TO>(hh);
     }
     // the store is the one compiler-visible vm op per plane (counted in kOpsAfterComb):
     // every lane stores, lanes without a point into the sink, so each wave issues it
-    double2 *dst = outY ? a.hout + orbOff + colOff + zs * z : a.sink + tid;
+    TO *dst = outY ? (TO *)a.hout + orbOff + colOff + zs * z : (TO *)a.sink + tid;
     *dst = o;
 #pragma unroll
     for (int i = 0; i < 8; ++i) q[i] = q[i + 1];
@@ -549,7 +624,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
 constexpr int kScatG = 8;
 
 struct ScatterArgs {
-  double2 *hout;
+  void *hout;             // double2, or float2 in the mixed-precision chain
   const double2 *c;       // (nb, nlma, nk), orbital fastest
   const double2 *zent;    // (cmax, nent, nk)
   const int *tp_g;        // (ntp) unpadded index of each touched point
@@ -561,6 +636,7 @@ struct ScatterArgs {
   int ntp, nb, nlma, nent, cmax, ngrp;
 };
 
+template <typename TH>
 __global__ __launch_bounds__(kBlock) void k_nl_scatter(ScatterArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   const int ik = blockIdx.y / a.ngrp;
@@ -591,18 +667,18 @@ __global__ __launch_bounds__(kBlock) void k_nl_scatter(ScatterArgs a) {
       }
     }
   }
-  double2 *p = a.hout + ((size_t)ik * a.nb + b0) * a.osz + a.tp_off[i];
+  TH *p = (TH *)a.hout + ((size_t)ik * a.nb + b0) * a.osz + a.tp_off[i];
   double2 old[kScatG];
 #pragma unroll
   for (int o = 0; o < kScatG; ++o)
-    if (o < nob) old[o] = p[(size_t)o * a.osz];
+    if (o < nob) old[o] = cvt2<double2>(p[(size_t)o * a.osz]);
 #pragma unroll
   for (int o = 0; o < kScatG; ++o) {
     if (o < nob) {
       double2 v = old[o];
       v.x += a.s.x * acc[o].x - a.s.y * acc[o].y;
       v.y += a.s.x * acc[o].y + a.s.y * acc[o].x;
-      p[(size_t)o * a.osz] = v;
+      p[(size_t)o * a.osz] = cvt2<TH>(v);
     }
   }
 }
@@ -1031,8 +1107,10 @@ struct salmon_hip_ctx {
   int2 *d_atom_lma = nullptr;
   double *d_rxyz = nullptr, *d_uv = nullptr, *d_rinv = nullptr;
   double2 *d_zekr = nullptr, *d_zent = nullptr, *d_c = nullptr, *d_curnl = nullptr;
-  // Taylor scratch: H^2 psi .. H^(N-2) psi (allocated on first use)
+  // Taylor scratch: H^2 psi .. H^(N-2) psi (allocated on first use); FP32 chain h_1 .. h_{N-1}
   std::vector<double2 *> tscr;
+  std::vector<float2 *> tscr32;
+  int prec = SALMON_HIP_PREC_FP64;
   // observables scratch
   double *d_curpart = nullptr, *d_contrib = nullptr, *d_denspart = nullptr;
   int dens_ngrp = 1;
@@ -1139,6 +1217,7 @@ int salmon_hip_destroy(salmon_hip_t c) {
   for (int d = 0; d < 3; ++d) dfree(c->d_tw[d]);
   dfree(c->d_gcoef); dfree(c->d_vh); dfree(c->d_fft[0]); dfree(c->d_fft[1]);
   for (double2 *&p : c->tscr) dfree(p);
+  for (float2 *&p : c->tscr32) dfree(p);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
@@ -1334,9 +1413,9 @@ int salmon_hip_set_vec_ac(salmon_hip_t c, const double *vec_Ac) {
   return 0;
 }
 
-static int launch_proj(salmon_hip_t c, const void *psi, int nw) {
+static int launch_proj(salmon_hip_t c, const void *psi, int nw, bool f32 = false) {
   ProjArgs a;
-  a.psi = (const double2 *)psi;
+  a.psi = psi;
   a.zekr = c->d_zekr;
   a.rxyz = c->d_rxyz;
   a.jidx = c->d_jidx;
@@ -1354,55 +1433,71 @@ static int launch_proj(salmon_hip_t c, const void *psi, int nw) {
   dim3 grid((unsigned)(a.ngrp * c->natom * c->nk));
   hipEvent_t e0, e1;
   prof_events(c, nw == 1 ? SALMON_HIP_KERNEL_PROJ : SALMON_HIP_KERNEL_PROJ_CURRENT, &e0, &e1);
-  if (nw == 1)
-    hipExtLaunchKernelGGL(k_proj<1>, grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+  if (nw == 1 && f32)
+    hipExtLaunchKernelGGL((k_proj<1, float2>), grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+  else if (nw == 1)
+    hipExtLaunchKernelGGL((k_proj<1, double2>), grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
   else
-    hipExtLaunchKernelGGL(k_proj<4>, grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+    hipExtLaunchKernelGGL((k_proj<4, double2>), grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
   LAUNCHCHK("k_proj");
   return 0;
 }
 
 }  // extern "C"
 
-template <int HC, int NC>
+template <typename TI, int HC, int NC>
 static size_t stencil_lds_bytes() {
-  return sizeof(double2) * (kRingT * kTile + kRingH * HC * kTile + (NC >= 0 ? 2 * NC * kTile : 0)) +
+  return sizeof(TI) * (kRingT * kTile + kRingH * HC * kTile) + sizeof(double2) * (NC >= 0 ? 2 * NC * kTile : 0) +
          sizeof(double) * kRingH * kTile;
 }
 
-template <int HC, int NC>
+template <typename TI, typename R, typename TO, int HC, int NC>
 static int launch_stencil_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
-  const size_t shmem = stencil_lds_bytes<HC, NC>();
+  const size_t shmem = stencil_lds_bytes<TI, HC, NC>();
   static bool attr_set = false;
   if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencil<HC, NC>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)shmem));
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencil<TI, R, TO, HC, NC>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     attr_set = true;
   }
-  hipExtLaunchKernelGGL(k_stencil<HC, NC>, grid, dim3(kBlock), shmem, c->stream, e0, e1, 0, a);
+  hipExtLaunchKernelGGL((k_stencil<TI, R, TO, HC, NC>), grid, dim3(kBlock), shmem, c->stream, e0, e1, 0, a);
   return 0;
 }
 
+// Stencil stages: plain FP64, and the three stages of the mixed-precision chain.
+enum StencilStage { kStagePlain = 0, kStageMixedFirst = 1, kStageMixedMiddle = 2, kStageMixedFinal = 3 };
+
 template <int HC>
-static int launch_stencil_hc(salmon_hip_ctx *c, int nc, dim3 grid, hipEvent_t e0, hipEvent_t e1,
+static int launch_stencil_hc(salmon_hip_ctx *c, int stage, int nc, dim3 grid, hipEvent_t e0, hipEvent_t e1,
                              const StencilArgs &a) {
+  if (stage == kStageMixedFirst) return launch_stencil_t<double2, float, float2, HC, -1>(c, grid, e0, e1, a);
+  if (stage == kStageMixedMiddle) return launch_stencil_t<float2, float, float2, HC, -1>(c, grid, e0, e1, a);
+  if (stage == kStageMixedFinal) {
+    switch (nc) {
+      case 1: return launch_stencil_t<float2, float, double2, HC, 1>(c, grid, e0, e1, a);
+      case 2: return launch_stencil_t<float2, float, double2, HC, 2>(c, grid, e0, e1, a);
+      case 3: return launch_stencil_t<float2, float, double2, HC, 3>(c, grid, e0, e1, a);
+      case 4: return launch_stencil_t<float2, float, double2, HC, 4>(c, grid, e0, e1, a);
+      default: return launch_stencil_t<float2, float, double2, HC, 5>(c, grid, e0, e1, a);
+    }
+  }
   switch (nc) {
-    case -1: return launch_stencil_t<HC, -1>(c, grid, e0, e1, a);
-    case 0: return launch_stencil_t<HC, 0>(c, grid, e0, e1, a);
-    case 1: return launch_stencil_t<HC, 1>(c, grid, e0, e1, a);
-    case 2: return launch_stencil_t<HC, 2>(c, grid, e0, e1, a);
-    case 3: return launch_stencil_t<HC, 3>(c, grid, e0, e1, a);
-    case 4: return launch_stencil_t<HC, 4>(c, grid, e0, e1, a);
-    default: return launch_stencil_t<HC, 5>(c, grid, e0, e1, a);
+    case -1: return launch_stencil_t<double2, double, double2, HC, -1>(c, grid, e0, e1, a);
+    case 0: return launch_stencil_t<double2, double, double2, HC, 0>(c, grid, e0, e1, a);
+    case 1: return launch_stencil_t<double2, double, double2, HC, 1>(c, grid, e0, e1, a);
+    case 2: return launch_stencil_t<double2, double, double2, HC, 2>(c, grid, e0, e1, a);
+    case 3: return launch_stencil_t<double2, double, double2, HC, 3>(c, grid, e0, e1, a);
+    case 4: return launch_stencil_t<double2, double, double2, HC, 4>(c, grid, e0, e1, a);
+    default: return launch_stencil_t<double2, double, double2, HC, 5>(c, grid, e0, e1, a);
   }
 }
 
 extern "C" {
 
-static int launch_scatter(salmon_hip_t c, void *hout, double2 s) {
+static int launch_scatter(salmon_hip_t c, void *hout, double2 s, bool f32 = false) {
   if (c->ntp == 0) return 0;
   ScatterArgs a;
-  a.hout = (double2 *)hout;
+  a.hout = hout;
   a.c = c->d_c;
   a.zent = c->d_zent;
   a.tp_g = c->d_tp_g;
@@ -1420,7 +1515,10 @@ static int launch_scatter(salmon_hip_t c, void *hout, double2 s) {
   hipEvent_t e0, e1;
   prof_events(c, SALMON_HIP_KERNEL_NL_SCATTER, &e0, &e1);
   dim3 grid((unsigned)((c->ntp + kBlock - 1) / kBlock), (unsigned)(c->nk * a.ngrp));
-  hipExtLaunchKernelGGL(k_nl_scatter, grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+  if (f32)
+    hipExtLaunchKernelGGL(k_nl_scatter<float2>, grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+  else
+    hipExtLaunchKernelGGL(k_nl_scatter<double2>, grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
   LAUNCHCHK("k_nl_scatter");
   return 0;
 }
@@ -1429,14 +1527,14 @@ static int launch_scatter(salmon_hip_t c, void *hout, double2 s) {
 // hout = psi + c1 h1 + ... + c_N H in (fin = true, comb = [psi, h1 .. h_{N-2}]).
 // The projections c of `in` must be current (launch_proj).
 static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, bool fin, void *const *comb, int ncomb,
-                       const double *zc) {
+                       const double *zc, int stage = kStagePlain) {
   StencilArgs a;
   memset(&a, 0, sizeof(a));
-  a.in = (const double2 *)in;
-  a.hout = (double2 *)hout;
+  a.in = in;
+  a.hout = hout;
   a.sink = c->d_sink;
   const int nc = fin ? ncomb : -1;
-  for (int n = 0; n < (fin ? ncomb : 0); ++n) a.comb[n] = (const double2 *)comb[n];
+  for (int n = 0; n < (fin ? ncomb : 0); ++n) a.comb[n] = comb[n];
   if (fin)
     for (int n = 0; n <= ncomb; ++n) a.zc[n] = make_double2(zc[2 * n], zc[2 * n + 1]);
   a.V = c->d_V;
@@ -1452,14 +1550,15 @@ static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, boo
   prof_events(c, kind, &e0, &e1);
   const int hc = (8 * c->nx + kTile - 1) / kTile;
   int rc = 0;
-  if (hc <= 1) rc = launch_stencil_hc<1>(c, nc, grid, e0, e1, a);
-  else if (hc <= 2) rc = launch_stencil_hc<2>(c, nc, grid, e0, e1, a);
-  else if (hc <= 4) rc = launch_stencil_hc<4>(c, nc, grid, e0, e1, a);
-  else rc = launch_stencil_hc<8>(c, nc, grid, e0, e1, a);
+  if (hc <= 1) rc = launch_stencil_hc<1>(c, stage, nc, grid, e0, e1, a);
+  else if (hc <= 2) rc = launch_stencil_hc<2>(c, stage, nc, grid, e0, e1, a);
+  else if (hc <= 4) rc = launch_stencil_hc<4>(c, stage, nc, grid, e0, e1, a);
+  else rc = launch_stencil_hc<8>(c, stage, nc, grid, e0, e1, a);
   if (rc) return rc;
   LAUNCHCHK("k_stencil");
+  const bool f32out = stage == kStageMixedFirst || stage == kStageMixedMiddle;
   if (c->natom > 0 && c->nlma > 0)
-    return launch_scatter(c, hout, fin ? a.zc[ncomb] : make_double2(1.0, 0.0));
+    return launch_scatter(c, hout, fin ? a.zc[ncomb] : make_double2(1.0, 0.0), f32out);
   return 0;
 }
 
@@ -1478,6 +1577,40 @@ int salmon_hip_hpsi(salmon_hip_t c, const void *tpsi, void *htpsi) {
   return launch_hpsi(c, SALMON_HIP_KERNEL_HPSI0, tpsi, htpsi, false, nullptr, 0, nullptr);
 }
 
+// Mixed-precision Taylor step: h_n = H^n psi in complex FP32 (stencil and
+// local-potential products in FP32; projections, the KB term and the Taylor sum
+// in FP64), out = psi + sum_n zc(n) h_n formed in FP64 by the last stencil pass.
+static int taylor_mixed(salmon_hip_t c, const double *zc, int n_hamil, void *tin, void *tout) {
+  if (n_hamil < 2) return fail(SALMON_HIP_EUNSUPPORTED, "mixed precision needs n_hamil >= 2");
+  while ((int)c->tscr32.size() < n_hamil - 1) {
+    float2 *p = nullptr;
+    if (int rc = dalloc(&p, c->osz * c->norb)) return rc;
+    c->tscr32.push_back(p);
+  }
+  void *h[kMaxComb + 1];
+  h[0] = tin;
+  for (int n = 1; n < n_hamil; ++n) h[n] = c->tscr32[n - 1];
+  for (int n = 1; n < n_hamil; ++n) {
+    if (c->natom > 0)
+      if (int rc = launch_proj(c, h[n - 1], 1, n > 1)) return rc;
+    const int kind = n == 1 ? SALMON_HIP_KERNEL_HPSI_TAYLOR1 : SALMON_HIP_KERNEL_HPSI_TAYLOR;
+    const int stage = n == 1 ? kStageMixedFirst : kStageMixedMiddle;
+    if (int rc = launch_hpsi(c, kind, h[n - 1], h[n], false, nullptr, 0, nullptr, stage)) return rc;
+  }
+  if (c->natom > 0)
+    if (int rc = launch_proj(c, h[n_hamil - 1], 1, true)) return rc;
+  return launch_hpsi(c, SALMON_HIP_KERNEL_HPSI_TAYLOR_LAST, h[n_hamil - 1], tout, true, h, n_hamil - 1, zc,
+                     kStageMixedFinal);
+}
+
+int salmon_hip_set_precision(salmon_hip_t c, int mode) {
+  if (int rc = check(c)) return rc;
+  if (mode != SALMON_HIP_PREC_FP64 && mode != SALMON_HIP_PREC_MIXED)
+    return fail(SALMON_HIP_EINVAL, "precision mode %d", mode);
+  c->prec = mode;
+  return 0;
+}
+
 int salmon_hip_taylor(salmon_hip_t c, const double *zc, int n_hamil, void *tspsi_in, void *tspsi_out,
                       void *sshtpsi) {
   if (int rc = check(c)) return rc;
@@ -1486,6 +1619,7 @@ int salmon_hip_taylor(salmon_hip_t c, const double *zc, int n_hamil, void *tspsi
   if (!tspsi_in || !tspsi_out || !sshtpsi || tspsi_in == tspsi_out || tspsi_in == sshtpsi || tspsi_out == sshtpsi)
     return fail(SALMON_HIP_EINVAL, "taylor buffers invalid or aliased");
   if (int rc = ready(c)) return rc;
+  if (c->prec == SALMON_HIP_PREC_MIXED) return taylor_mixed(c, zc, n_hamil, tspsi_in, tspsi_out);
   // h_1 = H psi -> sshtpsi, h_2 .. h_{N-1} -> internal scratch
   const int nscr = std::max(0, n_hamil - 2);
   while ((int)c->tscr.size() < nscr) {

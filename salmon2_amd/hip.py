@@ -29,6 +29,7 @@ EXPORTS = (
     "salmon_hip_set_vec_ac",
     "salmon_hip_hpsi",
     "salmon_hip_taylor",
+    "salmon_hip_set_precision",
     "salmon_hip_density",
     "salmon_hip_current",
     "salmon_hip_dipole",
@@ -81,6 +82,7 @@ def load():
     L.salmon_hip_set_vec_ac.argtypes = [C.c_void_p, _dp]
     L.salmon_hip_hpsi.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_taylor.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.salmon_hip_set_precision.argtypes = [C.c_void_p, C.c_int]
     L.salmon_hip_density.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_current.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p]
     L.salmon_hip_dipole.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -187,6 +189,12 @@ class Handle:
         z = np.ascontiguousarray(np.asarray(zc, dtype=np.complex128)).view(np.float64)
         _chk(load().salmon_hip_taylor(self.h, _p(z), len(z) // 2, C.c_void_p(_ptr(tin)), C.c_void_p(_ptr(tout)),
                                       C.c_void_p(_ptr(work))), "taylor")
+
+    PREC = {"fp64": 0, "mixed": 1}
+
+    def set_precision(self, mode):
+        """'fp64' (default) or 'mixed' (complex FP32 stencil chain, FP64 sums) for taylor."""
+        _chk(load().salmon_hip_set_precision(self.h, self.PREC[mode]), "set_precision")
 
     def density(self, psi, rho):
         _chk(load().salmon_hip_density(self.h, C.c_void_p(_ptr(psi)), C.c_void_p(_ptr(rho))), "density")

@@ -20,6 +20,9 @@ corrector Taylor step from the same input orbitals.  The library's taylor
 leaves its input untouched, so the orbital copy of the reference
 (psi_tmp, :370-372, 410-412) is not needed.
 
+``precision="mixed"`` runs the Taylor chain H^n psi in complex FP32 with the
+projections, the Taylor sum, density and current in FP64.
+
 Orbitals stay resident in HBM for the whole run; per-step observables are
 written to device arrays and read back once at the end.  torch is used only
 for device allocation and the stream.
@@ -33,12 +36,13 @@ from .hip import Handle
 
 
 class RTPropagator:
-    def __init__(self, sysm, device="cuda:0", psi=None, scf=False, pc=False):
+    def __init__(self, sysm, device="cuda:0", psi=None, scf=False, pc=False, precision="fp64"):
         self.s = sysm
         self.dev = torch.device(device)
         torch.cuda.set_device(self.dev)
         self.stream = torch.cuda.current_stream(self.dev)
         self.h = Handle(sysm, stream=self.stream)
+        self.h.set_precision(precision)
         src = sysm.psi if psi is None else psi
         self.psi_a = torch.from_numpy(np.ascontiguousarray(src)).to(self.dev)
         self.psi_b = torch.zeros_like(self.psi_a)
