@@ -74,6 +74,9 @@ def lib():
         L.orc_rt_run_scf.argtypes = [S, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
                                      _dp, _dp]
         L.orc_rt_run_scf.restype = C.c_int
+        L.orc_rt_run_full.argtypes = [S, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int,
+                                      _dp, _dp, _dp, _dp, _dp]
+        L.orc_rt_run_full.restype = C.c_int
         _lib = L
     return _lib
 
@@ -213,6 +216,32 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle rt_run_scf failed ({rc})")
         return p, cur, dip, rne, vloc
+
+
+    def rt_run_full(self, scf=False, pc=False, trans_longi="tr", nt=None, psi=None):
+        """General loop; returns final psi, J(t), dipole(t), rNe(t), Ac_tot (nt+2, 3), final V_local."""
+        s = self.s
+        nt = s.nt if nt is None else nt
+        ac = np.ascontiguousarray(s.pulse.table(s.dt, nt) if s.pulse else np.zeros((nt + 2, 3)))
+        p = np.array(s.psi if psi is None else psi, dtype=np.complex128, copy=True)
+        vpsl = np.ascontiguousarray(s.vlocal[0], dtype=np.float64)
+        vloc = np.zeros_like(vpsl)
+        al = np.ascontiguousarray(s.al, dtype=np.float64)
+        cur = np.zeros((nt, 3))
+        dip = np.zeros((nt, 3))
+        rne = np.zeros(nt)
+        if trans_longi not in ("tr", "lo"):
+            raise ValueError(trans_longi)
+        saved = self.o.vlocal
+        try:
+            rc = lib().orc_rt_run_full(self.ref, int(bool(scf)), int(bool(pc)), int(trans_longi == "lo"), _p(vpsl),
+                                       _p(vloc), _p(al), s.dt, s.n_hamil, nt, _p(ac), _p(p.view(np.float64)),
+                                       _p(cur), _p(dip), _p(rne))
+        finally:
+            self.o.vlocal = saved
+        if rc != 0:
+            raise RuntimeError(f"oracle rt_run_full failed ({rc})")
+        return p, cur, dip, rne, ac, (vloc if scf else None)
 
 
 def hartree(rho, al):

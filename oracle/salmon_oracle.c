@@ -33,6 +33,8 @@
  *   orc_xc_pz           v = d(rho e)/d rho by finite differences of rho*exc; the
  *                       published PZ81 values at rs = 1, 2, 5.
  *   orc_rt_run_scf      composition (time_evolution_step.f90:160-207, 354-423).
+ *   orc_rt_run_full     the above plus the 'lo' field feedback: the discrete plasma
+ *                       oscillation cos(w dt) = 1 - 2 pi n dt^2 of one constant orbital.
  */
 #include "salmon_oracle.h"
 
@@ -693,33 +695,40 @@ static void vlocal_from_rho(const orc_system *s, const double *vpsl, const doubl
 }
 
 /*
- * RT loop with the potential updated from the density every step (yn_fix_func = 'n'):
- * src/rt/time_evolution_step.f90:141-148 (optional predictor_corrector, :354-423),
- * :160-166 (density), :182-207 (Hartree, XC, update_vlocal), :243-253 (current);
- * the initial V_local is built from the initial density (src/rt/initialization_rt.f90).
- * vpsl is the fixed local pseudopotential; vloc receives V_local (s->vlocal is set to it).
+ * General RT loop: fixed V_local (scf = 0, s->vlocal as given) or V_local rebuilt from the
+ * density every step (scf = 1, yn_fix_func = 'n': src/rt/time_evolution_step.f90:160-166,
+ * 182-207, initial potential from the initial density, src/rt/initialization_rt.f90), with
+ * the optional predictor_corrector (:141-148, 354-423), and the field update of
+ * calc_emfields (src/rt/em_field.f90:24-70): trans_longi 'tr' (lo = 0, Ac_tot = Ac_ext) or
+ * 'lo' (lo = 1, Ac_ind(itt+1) = 2 Ac_ind(itt) - Ac_ind(itt-1) - 4 pi J(itt) dt^2).
+ * ac: (3, 0:nt+1); on entry Ac_ext, on exit Ac_tot.  vpsl / vloc / al only with scf = 1.
  */
-int orc_rt_run_scf(orc_system *s, const double *vpsl, double *vloc, const double al[3], int pc, double dt,
-                   int n_hamil, int nt, const double *ac_tot, double *psi, double *curr_t, double *dip_t,
-                   double *rne_t) {
-  if (s->nspin != 1) return -2;
+int orc_rt_run_full(orc_system *s, int scf, int pc, int lo, const double *vpsl, double *vloc, const double al[3],
+                    double dt, int n_hamil, int nt, double *ac, double *psi, double *curr_t, double *dip_t,
+                    double *rne_t) {
+  const double pi = 3.141592653589793238462643383279502884;
+  if (scf && s->nspin != 1) return -2;
+  pc = pc && scf;
   const size_t tot = C2(orb_size(s) * (size_t)s->nspin * s->no * s->nk);
   const size_t g = grid_size(s);
   double *a = psi;
   double *b = (double *)calloc(tot, sizeof(double));
   double *w = (double *)calloc(tot, sizeof(double));
   double *psi_tmp = pc ? (double *)malloc(sizeof(double) * tot) : NULL;
-  double *rho = (double *)malloc(sizeof(double) * g);
+  double *rho = (double *)malloc(sizeof(double) * g * s->nspin);
   double *v_tmp = (double *)malloc(sizeof(double) * g);
   double *zc = (double *)malloc(sizeof(double) * 2 * n_hamil);
-  if (!b || !w || !rho || !v_tmp || !zc || (pc && !psi_tmp)) return -1;
-  s->vlocal = vloc;
+  double *ac_ind = (double *)calloc(3 * (size_t)(nt + 2), sizeof(double));
+  if (!b || !w || !rho || !v_tmp || !zc || !ac_ind || (pc && !psi_tmp)) return -1;
   orc_taylor_coefs(dt, n_hamil, 0, zc);
-  orc_density(s, a, rho);
-  vlocal_from_rho(s, vpsl, al, rho, vloc);
+  if (scf) {
+    s->vlocal = vloc;
+    orc_density(s, a, rho);
+    vlocal_from_rho(s, vpsl, al, rho, vloc);
+  }
   for (int itt = 1; itt <= nt; ++itt) {
     double A[3];
-    for (int d = 0; d < 3; ++d) A[d] = 0.5 * (ac_tot[3 * itt + d] + ac_tot[3 * (itt - 1) + d]);
+    for (int d = 0; d < 3; ++d) A[d] = 0.5 * (ac[3 * itt + d] + ac[3 * (itt - 1) + d]);
     if (s->natom > 0) orc_update_kvector(s, A);
     if (pc) {
       memcpy(v_tmp, vloc, sizeof(double) * g);
@@ -732,11 +741,22 @@ int orc_rt_run_scf(orc_system *s, const double *vpsl, double *vloc, const double
     }
     orc_taylor(s, A, zc, n_hamil, a, b, w);
     orc_density(s, b, rho);
-    vlocal_from_rho(s, vpsl, al, rho, vloc);
-    double Ac[3] = {ac_tot[3 * itt], ac_tot[3 * itt + 1], ac_tot[3 * itt + 2]};
-    double cur[3];
+    if (scf) vlocal_from_rho(s, vpsl, al, rho, vloc);
+    double Ac[3] = {ac[3 * itt], ac[3 * itt + 1], ac[3 * itt + 2]};
+    double cur[6];
     orc_current(s, Ac, b, cur);
-    for (int d = 0; d < 3; ++d) curr_t[3 * (itt - 1) + d] = cur[d];
+    for (int d = 0; d < 3; ++d) {
+      double c = cur[d];
+      if (s->nspin == 2) c = c + cur[3 + d];
+      curr_t[3 * (itt - 1) + d] = c;
+    }
+    if (lo) {
+      for (int d = 0; d < 3; ++d) {
+        double *ai = ac_ind + 3 * (size_t)(itt + 1) + d;
+        *ai = 2.0 * ac_ind[3 * itt + d] - ac_ind[3 * (itt - 1) + d] - 4.0 * pi * curr_t[3 * (itt - 1) + d] * (dt * dt);
+        ac[3 * (itt + 1) + d] = ac[3 * (itt + 1) + d] + *ai;  /* Ac_tot = Ac_ext + Ac_ind */
+      }
+    }
     double dp[4];
     orc_dipole(s, rho, dp);
     for (int d = 0; d < 3; ++d) dip_t[3 * (itt - 1) + d] = dp[d];
@@ -751,6 +771,17 @@ int orc_rt_run_scf(orc_system *s, const double *vpsl, double *vloc, const double
   } else {
     free(b);
   }
-  free(w); free(psi_tmp); free(rho); free(v_tmp); free(zc);
+  free(w); free(psi_tmp); free(rho); free(v_tmp); free(zc); free(ac_ind);
   return 0;
+}
+
+int orc_rt_run_scf(orc_system *s, const double *vpsl, double *vloc, const double al[3], int pc, double dt,
+                   int n_hamil, int nt, const double *ac_tot, double *psi, double *curr_t, double *dip_t,
+                   double *rne_t) {
+  double *ac = (double *)malloc(sizeof(double) * 3 * (size_t)(nt + 2));
+  if (!ac) return -1;
+  memcpy(ac, ac_tot, sizeof(double) * 3 * (size_t)(nt + 2));
+  int rc = orc_rt_run_full(s, 1, pc, 0, vpsl, vloc, al, dt, n_hamil, nt, ac, psi, curr_t, dip_t, rne_t);
+  free(ac);
+  return rc;
 }

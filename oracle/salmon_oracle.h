@@ -82,6 +82,9 @@ void orc_xc_pz(long n, const double *rho, double *vxc, double *exc);
 /* RT loop with the Hartree + PZ-LDA potential updated every step (yn_fix_func = 'n'),
    optionally with the predictor-corrector; vpsl is the fixed local pseudopotential part,
    s->vlocal must point to a writable (nx,ny,nz) buffer that receives V_local. */
+int orc_rt_run_full(orc_system *s, int scf, int pc, int lo, const double *vpsl, double *vloc, const double al[3],
+                    double dt, int n_hamil, int nt, double *ac, double *psi, double *curr_t, double *dip_t,
+                    double *rne_t);
 int orc_rt_run_scf(orc_system *s, const double *vpsl, double *vloc, const double al[3], int pc, double dt,
                    int n_hamil, int nt, const double *ac_tot, double *psi, double *curr_t, double *dip_t,
                    double *rne_t);

@@ -23,11 +23,19 @@ leaves its input untouched, so the orbital copy of the reference
 ``precision="mixed"`` runs the Taylor chain H^n psi in complex FP32 with the
 projections, the Taylor sum, density and current in FP64.
 
+Fields follow calc_emfields (src/rt/em_field.f90:24-70): ``trans_longi="tr"``
+keeps Ac_tot = Ac_ext; ``"lo"`` adds the induced field Ac_ind(t+dt) =
+2 Ac_ind(t) - Ac_ind(t-dt) - 4 pi J(t) dt^2 (one 3-double read-back of J per
+step).  ``write_rt_data`` writes SYSname_rt.data as write_rt_data_3d does
+(src/io/write.f90:686-790, atomic units).
+
 Orbitals stay resident in HBM for the whole run; per-step observables are
 written to device arrays and read back once at the end.  torch is used only
 for device allocation and the stream.
 """
 from __future__ import annotations
+
+import math
 
 import numpy as np
 import torch
@@ -36,7 +44,7 @@ from .hip import Handle
 
 
 class RTPropagator:
-    def __init__(self, sysm, device="cuda:0", psi=None, scf=False, pc=False, precision="fp64"):
+    def __init__(self, sysm, device="cuda:0", psi=None, scf=False, pc=False, precision="fp64", trans_longi="tr"):
         self.s = sysm
         self.dev = torch.device(device)
         torch.cuda.set_device(self.dev)
@@ -50,6 +58,9 @@ class RTPropagator:
         self.rho = torch.zeros((sysm.nspin, sysm.nz, sysm.ny, sysm.nx), dtype=torch.float64, device=self.dev)
         self.zc = sysm.zc()
         self.itt = 0
+        if trans_longi not in ("tr", "lo"):
+            raise ValueError(f"trans_longi {trans_longi!r}: 'tr' or 'lo'")
+        self.trans_longi = trans_longi
         self.scf = bool(scf)
         self.pc = bool(pc) and self.scf  # with a fixed potential the corrector average is the identity
         if self.scf:
@@ -64,7 +75,9 @@ class RTPropagator:
     def setup_outputs(self, nt):
         s = self.s
         self.nt = nt
-        self.ac = s.pulse.table(s.dt, nt) if s.pulse is not None else np.zeros((nt + 2, 3))
+        self.ac_ext = s.pulse.table(s.dt, nt) if s.pulse is not None else np.zeros((nt + 2, 3))
+        self.ac_ind = np.zeros_like(self.ac_ext)
+        self.ac = self.ac_ext.copy()  # Ac_tot
         self.curr_t = torch.zeros((nt, s.nspin, 3), dtype=torch.float64, device=self.dev)
         self.dip_t = torch.zeros((nt, 4), dtype=torch.float64, device=self.dev)
 
@@ -84,6 +97,10 @@ class RTPropagator:
             self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
         self.h.current(self.ac[itt], self.psi_b, self.curr_t[itt - 1])
         self.h.dipole(self.rho, self.dip_t[itt - 1])
+        if self.trans_longi == "lo" and itt + 1 < len(self.ac):
+            j = self.curr_t[itt - 1].sum(dim=0).cpu().numpy()  # synchronises this step
+            self.ac_ind[itt + 1] = 2.0 * self.ac_ind[itt] - self.ac_ind[itt - 1] - 4.0 * math.pi * j * self.s.dt ** 2
+            self.ac[itt + 1] = self.ac_ext[itt + 1] + self.ac_ind[itt + 1]
         self.psi_a, self.psi_b = self.psi_b, self.psi_a
 
     def run(self, nt=None):
@@ -97,4 +114,45 @@ class RTPropagator:
         torch.cuda.synchronize(self.dev)
         curr = self.curr_t.sum(dim=1).cpu().numpy()
         dip = self.dip_t.cpu().numpy()
-        return dict(psi=self.psi_a, curr=curr, dip=dip[:, :3], rne=dip[:, 3])
+        dt = self.s.dt
+        n = len(curr)
+        e_ext = -(self.ac_ext[1 : n + 1] - self.ac_ext[0:n]) / dt
+        e_tot = -(self.ac[1 : n + 1] - self.ac[0:n]) / dt
+        return dict(psi=self.psi_a, curr=curr, dip=dip[:, :3], rne=dip[:, 3], ac_ext=self.ac_ext[1 : n + 1],
+                    ac_tot=self.ac[1 : n + 1], e_ext=e_ext, e_tot=e_tot)
+
+    def write_rt_data(self, path, res=None):
+        """SYSname_rt.data of the steps run so far (write_rt_data_3d, unpolarised, atomic units)."""
+        res = self.results() if res is None else res
+        write_rt_data(path, self.s.dt, res)
+
+
+def _fe(x: float, d: int = 15, e: int = 3) -> str:
+    """Fortran E(d+8).dEe edit of x: sign, 0.ddd..., E, signed e-digit exponent."""
+    if x == 0.0 or not math.isfinite(x):
+        mant, ex = 0.0, 0
+    else:
+        ex = math.floor(math.log10(abs(x))) + 1
+        mant = abs(x) / 10.0 ** ex
+        if round(mant, d) >= 1.0:
+            mant, ex = mant / 10.0, ex + 1
+    digits = f"{mant:.{d}f}"[1:]  # ".ddd"
+    s = ("-" if x < 0 else "") + "0" + digits + ("E-" if ex < 0 else "E+") + f"{abs(ex):0{e}d}"
+    return s.rjust(d + 5 + e)
+
+
+def write_rt_data(path, dt, res):
+    """Header and rows of SYSname_rt.data (src/io/write.f90:700-790): time, Ac_ext, E_ext,
+    Ac_tot, E_tot, Jm, columns 1-16, format (F16.8, 99(1X, E23.15E3))."""
+    lines = ["# Real time calculation: ", "# Ac_ext: External vector potential field",
+             "# E_ext: External electric field", "# Ac_tot: Total vector potential field",
+             "# E_tot: Total electric field", "# Jm: Matter current density (electrons)"]
+    names = ["Time", "Ac_ext_x", "Ac_ext_y", "Ac_ext_z", "E_ext_x", "E_ext_y", "E_ext_z", "Ac_tot_x", "Ac_tot_y",
+             "Ac_tot_z", "E_tot_x", "E_tot_y", "E_tot_z", "Jm_x", "Jm_y", "Jm_z"]
+    lines.append("#" + "".join(f" {i + 1}:{nm}[a.u.]" for i, nm in enumerate(names)))
+    for i in range(len(res["curr"])):
+        vals = list(res["ac_ext"][i]) + list(res["e_ext"][i]) + list(res["ac_tot"][i]) + list(res["e_tot"][i]) + \
+            list(res["curr"][i])
+        lines.append(f"{(i + 1) * dt:16.8f}" + "".join(" " + _fe(float(v)) for v in vals))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

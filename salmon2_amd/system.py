@@ -59,25 +59,49 @@ SI_A = 10.26  # bohr
 
 @dataclasses.dataclass
 class Pulse:
-    """'Asin2cos' vector-potential pulse A(t) = -epdir A0 sin^2(pi t/tw) cos(omega t + 2 pi phi)."""
+    """External vector potential A_ext(t) of calc_Ac_ext_t (src/rt/em_field.f90:86-306), A0 = f0/omega:
+
+    'Asin2cos'      A = -epdir A0 sin^2(pi t/tw) cos(omega t + 2 pi phi)            (t < tw)
+    'Acos2'..'Acos8' A = -A0 cos^n(pi tt/tw) Im(epdir exp(i(omega tt + 2 pi phi))),
+                    tt = t - tw/2 - t1_start                                       (|tt| < tw/2)
+    'impulse'       A = epdir e_impulse                                             (t >= 0)
+    """
 
     A0: float
     omega: float
     tw: float
     phi_cep: float = 0.0
     epdir: tuple = (1.0, 0.0, 0.0)
+    shape: str = "Asin2cos"
+    t1_start: float = 0.0
+    e_impulse: float = 1e-2
 
     def table(self, dt: float, nt: int) -> np.ndarray:
-        """Ac_ext(1:3, 0:nt+1) as an (nt+2, 3) array (em_field.f90:86-187)."""
+        """Ac_ext(1:3, 0:nt+1) as an (nt+2, 3) array (calc_Ac_ext_t(0, dt, 0, nt+1, ...))."""
         out = np.zeros((nt + 2, 3))
+        acos = {"Acos2": 2, "Acos3": 3, "Acos4": 4, "Acos6": 6, "Acos8": 8}
+        if self.shape not in ("Asin2cos", "impulse") and self.shape not in acos:
+            raise ValueError(f"pulse shape {self.shape!r} not supported")
         for i in range(nt + 2):
             t = 0.0 + i * dt
-            if t < self.tw:
-                s = math.sin(math.pi * t / self.tw)
-                env = s * s
-                c = math.cos(self.omega * t + self.phi_cep * 2.0 * math.pi)
-                for d in range(3):
-                    out[i, d] = -self.epdir[d] * self.A0 * env * c
+            if self.shape == "Asin2cos":
+                if t < self.tw:
+                    s = math.sin(math.pi * t / self.tw)
+                    env = s * s
+                    c = math.cos(self.omega * t + self.phi_cep * 2.0 * math.pi)
+                    for d in range(3):
+                        out[i, d] = -self.epdir[d] * self.A0 * env * c
+            elif self.shape == "impulse":
+                if t >= 0.0:
+                    for d in range(3):
+                        out[i, d] = self.epdir[d] * self.e_impulse
+            else:
+                tt = t - 0.5 * self.tw - self.t1_start
+                if abs(tt) < 0.5 * self.tw:
+                    env = math.cos(math.pi * tt / self.tw) ** acos[self.shape]
+                    sn = math.sin(self.omega * tt + self.phi_cep * 2.0 * math.pi)
+                    for d in range(3):
+                        out[i, d] = -self.A0 * env * (self.epdir[d] * sn)
         return out
 
 
