@@ -13,6 +13,9 @@ Forms timed on the same initial state (each for warmup + steps RT steps):
                steps, two Hartree + PZ-LDA potential updates per step; plain FP64,
                the baseline of the metric
   pc-mixed     the same step with the Taylor chain in complex FP32 (sums FP64)
+  learned-mixed  pc-mixed with the predictor Taylor step replaced by the learned
+               midpoint potential (salmon2_amd/learned.py), accepted per step by its
+               self-consistency residual, else the plain predictor-corrector step
 Each form's current, dipole and electron count are compared with pc-fp64's
 over the whole run.  `value` is the steps/s of the headline form.
 
@@ -38,6 +41,7 @@ FORMS = {
     "fixedv-fp64": dict(scf=False, pc=False, precision="fp64"),
     "pc-fp64": dict(scf=True, pc=True, precision="fp64"),
     "pc-mixed": dict(scf=True, pc=True, precision="mixed"),
+    "learned-mixed": dict(scf=True, pc=True, precision="mixed", learned=True),
 }
 BASE_FORM = "pc-fp64"
 HEADLINE_FORM = "pc-mixed"
@@ -77,13 +81,30 @@ def measured_traffic(workload: str, kernel: str):
     return None if e is None else e["read_bytes"] + e["write_bytes"]
 
 
-def run_form(s, form, steps, warmup, prof):
+def load_predictor(config, log):
+    """Committed weights of the predictor for this workload, else a small one trained now."""
+    import torch
+
+    from salmon2_amd import learned
+
+    path = learned.weights_path(config)
+    if os.path.exists(path):
+        return learned.Predictor.load(path, stream=torch.cuda.current_stream()), f"loaded {os.path.relpath(path, ROOT)}"
+    pred, hist = learned.build(config, ntraj=2, nt=60, epochs=3, log=None)
+    log(f"[bench] no committed predictor weights: trained 2 x 60-step trajectories, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    return pred, "trained in bench.py (2 trajectories x 60 steps), outside the timed region"
+
+
+def run_form(s, form, steps, warmup, prof, predictor=None):
     import numpy as np
     import torch
 
     from salmon2_amd.rt import RTPropagator
 
-    prop = RTPropagator(s, **FORMS[form])
+    kw = dict(FORMS[form])
+    if kw.pop("learned", False):
+        kw["learned"] = predictor
+    prop = RTPropagator(s, **kw)
     prop.setup_outputs(warmup + steps)
     for _ in range(warmup):
         prop.step()
@@ -104,6 +125,10 @@ def run_form(s, form, steps, warmup, prof):
         prop.h.prof_end()
     res = prop.results()
     res["psi"] = None
+    if prop.learned is not None:
+        res["accepted"] = prop.accepted
+        res["attempted"] = prop.attempted
+        res["resid_median"] = float(np.median(prop.resid_log)) if prop.resid_log else None
     assert np.all(np.isfinite(res["curr"])), f"non-finite current in {form}"
     prop.h.close()
     del prop
@@ -139,8 +164,11 @@ def main():
     if BASE_FORM not in forms:
         forms.insert(0, BASE_FORM)
     runs = {}
+    predictor, pred_note = (None, None)
+    if any(FORMS[f].get("learned") for f in forms):
+        predictor, pred_note = load_predictor(args.config, lambda m: print(m, file=sys.stderr, flush=True))
     for f in forms:
-        runs[f] = run_form(s, f, args.steps, args.warmup, prof=(f == BASE_FORM))
+        runs[f] = run_form(s, f, args.steps, args.warmup, prof=(f == BASE_FORM), predictor=predictor)
         print(f"[bench] {f}: {args.steps / runs[f]['elapsed']:.2f} steps/s", file=sys.stderr, flush=True)
 
     npts = s.nx * s.ny * s.nz
@@ -159,6 +187,10 @@ def main():
             "speedup_vs_" + BASE_FORM: sps / (args.steps / base["elapsed"]),
             "grid_point_orbital_updates_per_s": sps * npts * norb,
         }
+        if "accepted" in r:
+            form_out[f]["predictor"] = {"accepted_steps": r["accepted"], "attempted_steps": r["attempted"],
+                                        "median_residual": r["resid_median"], "tol": predictor.tol,
+                                        "weights": pred_note}
         if f != BASE_FORM and FORMS[f]["scf"]:
             form_out[f]["rel_err_vs_" + BASE_FORM] = {
                 "current": rel(r["curr"], base["curr"]),

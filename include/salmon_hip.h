@@ -63,7 +63,8 @@ int salmon_hip_create(int nx, int ny, int nz, int ly, int nspin, int no, int nk,
 /* Release every device resource of the handle. */
 int salmon_hip_destroy(salmon_hip_t h);
 
-/* Enqueue subsequent work on `stream` (a hipStream_t; NULL = the handle's own stream). */
+/* Enqueue subsequent work on `stream` (a hipStream_t; NULL = HIP's null stream, ordered with
+   every blocking stream).  A new handle starts on a stream of its own (salmon_hip_get_stream). */
 int salmon_hip_set_stream(salmon_hip_t h, void *stream);
 
 /* The handle's current stream (a hipStream_t). */
@@ -137,6 +138,19 @@ int salmon_hip_taylor(salmon_hip_t h, const double *zc, int n_hamil, void *tspsi
 int salmon_hip_set_precision(salmon_hip_t h, int mode);
 
 /*
+ * Project autotuner (no counterpart in the reference): times the stencil pass of the
+ * terms without the Taylor sum under each candidate launch shape — whole xy-plane
+ * tiles (when nx*ny <= 1024) with 1 or 2 z-chunks per orbital, 256-point y-tiles with
+ * 1, 2 or 4 z-chunks — `reps` launches each on handle-allocated scratch, and keeps the
+ * fastest for later hpsi/taylor calls.  *choice = 10*plane + z_chunks, *best_ms its
+ * time per launch (either may be null).  Needs V_local and vec_Ac set; synchronises.
+ */
+int salmon_hip_autotune(salmon_hip_t h, int reps, int *choice, double *best_ms);
+
+/* Set the launch shape directly: whole_plane -1 (rule) / 0 / 1, z_chunks 0 (rule) or a count. */
+int salmon_hip_set_launch_shape(salmon_hip_t h, int whole_plane, int z_chunks);
+
+/*
  * rho(ix,iy,iz,ispin) = sum_{ik,io} rocc*wtk*|psi|^2 into a DEVICE array
  * (calc_density, src/common/density_matrix.f90:26-236).
  */
@@ -197,6 +211,42 @@ int salmon_hip_update_vlocal(salmon_hip_t h, const double *vpsl_dev, const doubl
  */
 int salmon_hip_update_potential(salmon_hip_t h, const double *rho_dev, const double *vpsl_dev,
                                 const double *vmix_dev, double *vout_dev);
+
+/*
+ * Batched RT step over many independent systems (no counterpart in the reference,
+ * which runs one system per process).  Each system keeps its own handle (grid,
+ * orbitals, projectors, V_local); the batch runs the plain fixed-potential step
+ * (update_kvector, taylor, calc_density, calc_current, calc_dip of
+ * src/rt/time_evolution_step.f90) for all of them together: the stencil passes of
+ * all systems of one halo class run as ONE persistent kernel over a combined tile
+ * list, A(t) is read from per-system DEVICE tables (3, 0:nt+1) at a device step
+ * counter, and the per-step current curr(3*nspin) and dipole/electron count (4) are
+ * written into per-system DEVICE series at row itt-1, so that a captured step can be
+ * replayed.  FP64 only.  All handles are moved onto `stream` (NULL: the first
+ * handle's own stream; the null stream cannot be captured).  Calls on a batch and
+ * its handles must not overlap.
+ *   batch_set_system: buffers of system i (psi_a holds the initial orbitals; the two
+ *     orbital buffers alternate by step), Ac_tot table, series of nt rows.
+ *   batch_prepare: Taylor coefficients; builds the tile lists and descriptor tables,
+ *     sets the step counter to 1.
+ *   batch_autotune: times the packed stencil for each candidate (z-chunks per orbital:
+ *     the systems' rule, 1 or 2; persistent blocks: 256, 512, 1024, 2048 or one per
+ *     tile) and keeps the fastest; *choice = 100*z + blocks/256 (99 = one per tile).
+ *   batch_step: enqueue one step; batch_capture: capture two consecutive steps (both
+ *     buffer parities) into a hipGraph; batch_replay: launch it npairs times.
+ *   batch_set_time: set the device step counter.
+ */
+typedef struct salmon_hip_batch *salmon_hip_batch_t;
+int salmon_hip_batch_create(const salmon_hip_t *handles, int n, void *stream, salmon_hip_batch_t *out);
+int salmon_hip_batch_destroy(salmon_hip_batch_t b);
+int salmon_hip_batch_set_system(salmon_hip_batch_t b, int i, void *psi_a, void *psi_b, void *work, double *rho_dev,
+                                const double *ac_dev, int nt, double *curr_t_dev, double *dip_t_dev);
+int salmon_hip_batch_prepare(salmon_hip_batch_t b, const double *zc, int n_hamil);
+int salmon_hip_batch_autotune(salmon_hip_batch_t b, int reps, int *choice, double *best_ms);
+int salmon_hip_batch_set_time(salmon_hip_batch_t b, int itt);
+int salmon_hip_batch_step(salmon_hip_batch_t b);
+int salmon_hip_batch_capture(salmon_hip_batch_t b);
+int salmon_hip_batch_replay(salmon_hip_batch_t b, int npairs);
 
 /*
  * Kernel timing (no counterpart in the reference, whose timers are host

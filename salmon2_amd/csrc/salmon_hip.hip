@@ -61,7 +61,7 @@ int fail(int code, const char *fmt, ...) {
   } while (0)
 
 constexpr int kBlock = 256;
-constexpr int kProjJC = 16;      // sphere points staged per LDS chunk in k_proj
+constexpr int kProjJC = 32;      // sphere points staged per LDS chunk in k_proj
 constexpr int kProjOrb = 64;     // orbitals per k_proj block (4 waves x 16)
 constexpr int kMaxLmaPerAtom = 16;
 constexpr int kDensGroup = 32;   // orbitals per density partial
@@ -103,15 +103,34 @@ struct KvecArgs {
   double coef_nab[12];
   double coef_lap0;
   double ax, ay, az;
+  // with actab: A = (actab(:, itt) + actab(:, itt-1)) / 2 at the device step counter *itt
+  const double *actab;
+  const int *itt;
   int nlma, nps, nk, nent, cmax;
 };
+
+// the midpoint field of the current step from the device table, else the by-value field
+__device__ __forceinline__ void kvec_field(const KvecArgs &a, double &ax, double &ay, double &az) {
+  if (a.actab) {
+    const int t = *a.itt;
+    ax = 0.5 * (a.actab[3 * t + 0] + a.actab[3 * (t - 1) + 0]);
+    ay = 0.5 * (a.actab[3 * t + 1] + a.actab[3 * (t - 1) + 1]);
+    az = 0.5 * (a.actab[3 * t + 2] + a.actab[3 * (t - 1) + 2]);
+  } else {
+    ax = a.ax;
+    ay = a.ay;
+    az = a.az;
+  }
+}
 
 __global__ __launch_bounds__(kBlock) void k_update_kvector(KvecArgs a) {
   const int ilma = blockIdx.x;
   const int ik = blockIdx.y;
-  const double kx = a.veck[3 * ik + 0] + a.ax;
-  const double ky = a.veck[3 * ik + 1] + a.ay;
-  const double kz = a.veck[3 * ik + 2] + a.az;
+  double ax, ay, az;
+  kvec_field(a, ax, ay, az);
+  const double kx = a.veck[3 * ik + 0] + ax;
+  const double ky = a.veck[3 * ik + 1] + ay;
+  const double kz = a.veck[3 * ik + 2] + az;
   if (ilma == 0 && threadIdx.x == 0) {
     double *kp = a.kpar + 16 * ik;
     double ss = kx * kx;
@@ -148,9 +167,11 @@ __global__ __launch_bounds__(kBlock) void k_update_kvector(KvecArgs a) {
 __global__ void k_set_kpar(KvecArgs a) {
   const int ik = threadIdx.x;
   if (ik >= a.nk) return;
-  const double kx = a.veck[3 * ik + 0] + a.ax;
-  const double ky = a.veck[3 * ik + 1] + a.ay;
-  const double kz = a.veck[3 * ik + 2] + a.az;
+  double ax, ay, az;
+  kvec_field(a, ax, ay, az);
+  const double kx = a.veck[3 * ik + 0] + ax;
+  const double ky = a.veck[3 * ik + 1] + ay;
+  const double kz = a.veck[3 * ik + 2] + az;
   double *kp = a.kpar + 16 * ik;
   double ss = kx * kx;
   ss = ss + ky * ky;
@@ -195,6 +216,8 @@ template <int NW, typename TP>
 __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
   constexpr int MAXT = (NW * kMaxLmaPerAtom + 7) / 8;
   __shared__ double2 sp[kProjJC][kProjOrb];
+  __shared__ double2 zsh[kProjJC][kMaxLmaPerAtom];
+  __shared__ double rsh[kProjJC][3];
   const int grp = blockIdx.x % a.ngrp;
   const int t = blockIdx.x / a.ngrp;
   const int ia = t % a.natom;
@@ -216,20 +239,57 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
   const int comp = kq & 1;       // 0: re, 1: im of psi
   const int oc = lane & 15;      // orbital column (B) / output row (A)
 
-  for (int j0 = 0; j0 < m; j0 += kProjJC) {
-    // stage psi at kProjJC sphere points for the block's 64 orbitals
+  // psi at kProjJC sphere points for the block's 64 orbitals: chunk j0 + kProjJC is loaded into
+  // registers while the MFMAs consume chunk j0 from LDS
+  constexpr int kPer = (kProjJC * kProjOrb) / kBlock;
+  constexpr int kZPer = (kProjJC * kMaxLmaPerAtom + kBlock - 1) / kBlock;
+  TP pre[kPer];
+  double2 zpre[kZPer];
+  double rpre = 0.0;
+  const TP *psi = (const TP *)a.psi;
+  const int *jx = a.jidx + (size_t)a.nps * ia;
+  const double2 *zk = a.zekr + (size_t)a.nps * ((size_t)al.x + (size_t)a.nlma * ik);
+  auto load_chunk = [&](int j0) {
 #pragma unroll
-    for (int r = 0; r < (kProjJC * kProjOrb) / kBlock; ++r) {
+    for (int r = 0; r < kZPer; ++r) {
+      const int e = threadIdx.x + kBlock * r;  // (l, jj) = (e / kProjJC, e % kProjJC)
+      const int jj = e % kProjJC, l = e / kProjJC;
+      const int j = j0 + jj;
+      zpre[r] = (j < m && l < al.y) ? zk[(size_t)j + (size_t)a.nps * l] : make_double2(0.0, 0.0);
+    }
+    if (NW > 1 && threadIdx.x < 3 * kProjJC) {
+      const int j = j0 + threadIdx.x / 3;
+      rpre = j < m ? a.rxyz[3 * ((size_t)j0 + (size_t)a.nps * ia) + threadIdx.x] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
       const int e = threadIdx.x + kBlock * r;
       const int jj = e % kProjJC;
       const int o = e / kProjJC;
       const int j = j0 + jj;
       const int ob = grp * kProjOrb + o;
-      double2 v = make_double2(0.0, 0.0);
-      if (j < m && ob < a.nb) v = cvt2<double2>(((const TP *)a.psi)[(kbase + ob) * a.osz + a.jidx[(size_t)a.nps * ia + j]]);
-      sp[jj][o] = v;
+      TP v;
+      v.x = 0;
+      v.y = 0;
+      if (j < m && ob < a.nb) v = psi[(kbase + ob) * a.osz + jx[j]];
+      pre[r] = v;
     }
+  };
+  load_chunk(0);
+  for (int j0 = 0; j0 < m; j0 += kProjJC) {
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const int e = threadIdx.x + kBlock * r;
+      sp[e % kProjJC][e / kProjJC] = cvt2<double2>(pre[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < kZPer; ++r) {
+      const int e = threadIdx.x + kBlock * r;
+      if (e < kProjJC * kMaxLmaPerAtom) zsh[e % kProjJC][e / kProjJC] = zpre[r];
+    }
+    if (NW > 1 && threadIdx.x < 3 * kProjJC) rsh[threadIdx.x / 3][threadIdx.x % 3] = rpre;
     __syncthreads();
+    if (j0 + kProjJC < m) load_chunk(j0 + kProjJC);
 #pragma unroll 2
     for (int ks = 0; ks < kProjJC / 2; ++ks) {
       const int jp = 2 * ks + (kq >> 1);
@@ -239,10 +299,9 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
       double zr = 0.0, zi = 0.0, rw[3] = {0.0, 0.0, 0.0};
       const bool jv = j < m;
       if (jv && NW > 1) {
-        const double *rr = a.rxyz + 3 * ((size_t)j + (size_t)a.nps * ia);
-        rw[0] = rr[0];
-        rw[1] = rr[1];
-        rw[2] = rr[2];
+        rw[0] = rsh[jp][0];
+        rw[1] = rsh[jp][1];
+        rw[2] = rsh[jp][2];
       }
 #pragma unroll
       for (int tt = 0; tt < MAXT; ++tt) {
@@ -253,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
           if (jv && lw < nlw) {
             const int w = lw % NW;
             const int l = lw / NW;
-            const double2 z = a.zekr[(size_t)j + (size_t)a.nps * ((size_t)(al.x + l) + (size_t)a.nlma * ik)];
+            const double2 z = zsh[jp][l];
             const double r = (w == 0) ? 1.0 : rw[w - 1];
             zr = z.x * r;
             zi = z.y * r;
@@ -350,7 +409,7 @@ __device__ __forceinline__ void dma_chunk(const E *base, const size_t *off, unsi
 struct StencilArgs {
   const void *in;
   void *hout;
-  double2 *sink;          // kTile elements absorbing the stores of lanes without a point
+  double2 *sink;          // one element per thread absorbing the stores of lanes without a point
   const void *comb[kMaxComb];  // comb[0] double2 (psi); comb[n >= 1] the chain type
   double2 zc[kMaxComb + 1];
   const double *V;        // (nx, ny, nz, nspin)
@@ -366,8 +425,10 @@ struct StencilArgs {
 // Mixed form: (double2, float, float2) for the first term, (float2, float, float2)
 // for the middle ones and (float2, float, double2) for the last, whose Taylor sum
 // is formed in FP64 from psi (FP64) and h_1 .. h_N (FP32).
-template <typename TI, typename R, typename TO, int HC, int NC>
-__global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
+// One tile (orbital block, y-tile, z-chunk) = `bid` of the system described by `a`.
+template <typename TI, typename R, typename TO, int HC, int NC, int BS>
+__device__ __forceinline__ void stencil_tile(const StencilArgs &a, int bid, double2 *lds) {
+  static_assert(HC == 0 || BS == kTile, "y-halo tiles are 4 waves wide");
   using CH = typename Cx<R>::T;
   constexpr bool FINAL = NC >= 0;
   constexpr int D = kDmaDepth;
@@ -377,15 +438,11 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
   // DMA instructions per wave per plane, in issue order: comb, tile (WI), halo (HC*WI), V (2 x dword);
   // then 1 store
   constexpr int kOpsAfterComb = WI + HC * WI + 2 + 1;
-  extern __shared__ double2 lds[];
-  TI *tileR = (TI *)lds;                                  // kRingT x kTile
-  TI *haloR = tileR + kRingT * kTile;                     // kRingH x HC*kTile
-  double *vR = (double *)(haloR + kRingH * HC * kTile);   // kRingH x kTile doubles
-  char *combR = (char *)(vR + kRingH * kTile);            // 2 x NC x kTile x 16 bytes
+  TI *tileR = (TI *)lds;                                  // kRingT x BS
+  TI *haloR = tileR + kRingT * BS;                     // kRingH x HC*BS
+  double *vR = (double *)(haloR + kRingH * HC * BS);   // kRingH x BS doubles
+  char *combR = (char *)(vR + kRingH * BS);            // 2 x NC x BS x 16 bytes
 
-  int bid = blockIdx.x;
-  const int nfull = (a.nblk / 8) * 8;
-  if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;  // XCD-aware tile order
   const int ytile = bid % a.nyt;
   const int zt = (bid / a.nyt) % a.nzt;
   const int b = bid / (a.nyt * a.nzt);
@@ -439,7 +496,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
   }
 
   // halo points of this lane: chunk c = wave + 4k, points 64c + (lane | 32kk + lane/2)
-  size_t hoff[HC][2];
+  size_t hoff[HC > 0 ? HC : 1][2];
 #pragma unroll
   for (int k = 0; k < HC; ++k)
 #pragma unroll
@@ -458,29 +515,29 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
   const unsigned combAC = __builtin_amdgcn_readfirstlane(lds_addr(combR) + (unsigned)sizeof(CH) * 64u * wave);
 
   auto dma_tile = [&](int plane, int slot) {
-    dma_chunk<TI>(in + zs * wrapi(plane, nz), toffI, tileA + EB * kTile * slot, lane);
+    dma_chunk<TI>(in + zs * wrapi(plane, nz), toffI, tileA + EB * BS * slot, lane);
   };
   auto dma_halo = [&](int plane, int slot) {
 #pragma unroll
     for (int k = 0; k < HC; ++k)
-      dma_chunk<TI>(in + zs * wrapi(plane, nz), hoff[k], haloA + EB * (HC * kTile * slot + 4 * 64 * k), lane);
+      dma_chunk<TI>(in + zs * wrapi(plane, nz), hoff[k], haloA + EB * (HC * BS * slot + 4 * 64 * k), lane);
   };
   auto dma_v = [&](int plane, int slot) {
     const float *vb = (const float *)Vb + 2 * gplane * wrapi(plane, nz);
-    dma4(vb + voff[0], vA + 8u * kTile * slot);
-    dma4(vb + voff[1], vA + 8u * kTile * slot + 256u);
+    dma4(vb + voff[0], vA + 8u * BS * slot);
+    dma4(vb + voff[1], vA + 8u * BS * slot + 256u);
   };
   auto dma_comb = [&](int plane, int slot) {
     const size_t pl = orbOff + zs * wrapi(plane, nz);
 #pragma unroll
     for (int n = 0; n < (NC > 0 ? NC : 1); ++n) {
       if (n >= NC) break;
-      // every comb array owns kTile x 16 bytes per slot; psi is double2, the chain arrays CH
+      // every comb array owns BS x 16 bytes per slot; psi is double2, the chain arrays CH
       if (n == 0) {
         size_t o0[1] = {colOff};
-        dma_chunk<double2>((const double2 *)a.comb[0] + pl, o0, combA + 16u * kTile * (NC * slot), lane);
+        dma_chunk<double2>((const double2 *)a.comb[0] + pl, o0, combA + 16u * BS * (NC * slot), lane);
       } else {
-        dma_chunk<CH>((const CH *)a.comb[n] + pl, toffC, combAC + 16u * kTile * (NC * slot + n), lane);
+        dma_chunk<CH>((const CH *)a.comb[n] + pl, toffC, combAC + 16u * BS * (NC * slot + n), lane);
       }
     }
   };
@@ -492,7 +549,10 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
   for (int i = 0; i < 8; ++i) {
     const int n = i < 4 ? i + 1 : -(i - 3);
     const int r = ty + n;
-    if (r >= 0 && r < TY) {
+    if (HC == 0) {  // the tile is the whole xy-plane (TY == ny): y wraps inside it
+      yoff[i] = (r < 0 ? r + TY : (r >= TY ? r - TY : r)) * nx + tx;
+      yhal[i] = false;
+    } else if (r >= 0 && r < TY) {
       yoff[i] = r * nx + tx;
       yhal[i] = false;
     } else {
@@ -540,9 +600,9 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
     dma_halo(z + D, sHn);
     dma_v(z + D, sHn);
 
-    const TI *T = tileR + kTile * sT;
-    const TI *H = haloR + HC * kTile * sH;
-    q[8] = cvt2<CH>(tileR[kTile * ((sT + 4) % kRingT) + te]);
+    const TI *T = tileR + BS * sT;
+    const TI *H = haloR + HC * BS * sH;
+    q[8] = cvt2<CH>(tileR[BS * ((sT + 4) % kRingT) + te]);
     CH v, w;
     {
       const CH p1 = cvt2<CH>(T[xo[0]]), p2 = cvt2<CH>(T[xo[1]]), p3 = cvt2<CH>(T[xo[2]]), p4 = cvt2<CH>(T[xo[3]]);
@@ -568,7 +628,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
     w.y += nab[8] * (q[5].y - q[3].y) + nab[9] * (q[6].y - q[2].y) + nab[10] * (q[7].y - q[1].y) + nab[11] * (q[8].y - q[0].y);
 
     const CH t = q[4];
-    const R vcur = (R)vR[kTile * sH + te];
+    const R vcur = (R)vR[BS * sH + te];
     CH hh;
     hh.x = (vcur * t.x + klap0 * t.x) - (R)0.5 * v.x + w.y;
     hh.y = (vcur * t.y + klap0 * t.y) - (R)0.5 * v.y - w.x;
@@ -576,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
     if constexpr (FINAL) {
       constexpr int N = NC + 1;
       const double2 td = cvt2<double2>(t), h = cvt2<double2>(hh);
-      const char *Cb = combR + 16u * kTile * (NC * (z & 1));
+      const char *Cb = combR + 16u * BS * (NC * (z & 1));
       double2 od;
       if (N == 1) {
         od = td;
@@ -584,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
         od = ((const double2 *)Cb)[te];
 #pragma unroll
         for (int n = 1; n < N - 1; ++n) {
-          const double2 cc = cvt2<double2>(((const CH *)(Cb + 16u * kTile * n))[te]);
+          const double2 cc = cvt2<double2>(((const CH *)(Cb + 16u * BS * n))[te]);
           const double2 zc = a.zc[n - 1];
           od.x = od.x + (zc.x * cc.x - zc.y * cc.y);
           od.y = od.y + (zc.x * cc.y + zc.y * cc.x);
@@ -610,6 +670,27 @@ __global__ __launch_bounds__(kBlock) void k_stencil(StencilArgs a) {
     sH = (sH + 1 == kRingH) ? 0 : sH + 1;
   }
   wait_vm<0>();  // drain the look-ahead DMAs before the LDS is released
+}
+
+template <typename TI, typename R, typename TO, int HC, int NC, int BS>
+__global__ __launch_bounds__(BS) void k_stencil(StencilArgs a) {
+  extern __shared__ double2 lds[];
+  int bid = blockIdx.x;
+  const int nfull = (a.nblk / 8) * 8;
+  if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;  // XCD-aware tile order
+  stencil_tile<TI, R, TO, HC, NC, BS>(a, bid, lds);
+}
+
+// Persistent form over many systems (the batched step): block b takes tiles b, b + grid, ...
+// of `tiles` = (system index, tile index); all systems share the template shape.
+template <typename TI, typename R, typename TO, int HC, int NC, int BS>
+__global__ __launch_bounds__(BS) void k_stencil_packed(const StencilArgs *sys, const int2 *tiles, int ntiles) {
+  extern __shared__ double2 lds[];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int2 tr = tiles[t];
+    stencil_tile<TI, R, TO, HC, NC, BS>(sys[tr.x], tr.y, lds);
+    __syncthreads();  // every wave is done with the LDS rings before the next tile's DMAs
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -645,6 +726,12 @@ __global__ __launch_bounds__(kBlock) void k_nl_scatter(ScatterArgs a) {
   const int g = a.tp_g[i];
   const int e0 = a.pt_off[g], e1 = a.pt_off[g + 1];
   const int nob = min(kScatG, a.nb - b0);
+  // the read half of the read-modify-write is issued first and overlaps the KB sum
+  TH *p = (TH *)a.hout + ((size_t)ik * a.nb + b0) * a.osz + a.tp_off[i];
+  double2 old[kScatG];
+#pragma unroll
+  for (int o = 0; o < kScatG; ++o)
+    if (o < nob) old[o] = cvt2<double2>(p[(size_t)o * a.osz]);
   const double2 *zb = a.zent + (size_t)ik * a.nent * a.cmax;
   const double2 *cb = a.c + (size_t)ik * a.nlma * a.nb + b0;
   double2 acc[kScatG];
@@ -667,11 +754,6 @@ __global__ __launch_bounds__(kBlock) void k_nl_scatter(ScatterArgs a) {
       }
     }
   }
-  TH *p = (TH *)a.hout + ((size_t)ik * a.nb + b0) * a.osz + a.tp_off[i];
-  double2 old[kScatG];
-#pragma unroll
-  for (int o = 0; o < kScatG; ++o)
-    if (o < nob) old[o] = cvt2<double2>(p[(size_t)o * a.osz]);
 #pragma unroll
   for (int o = 0; o < kScatG; ++o) {
     if (o < nob) {
@@ -838,6 +920,8 @@ struct CurFinArgs {
   double *contrib;         // (3, no*nk) scratch
   double *curr;            // (3, nspin)
   double ax, ay, az;
+  const double *actab;     // with actab: vec_Ac = actab(:, *itt) (Ac_tot at the device step counter)
+  const int *itt;
   double inv_ngrid;
   int nslab, nspin, no, nk, nlma;
 };
@@ -875,7 +959,14 @@ __global__ __launch_bounds__(kBlock) void k_current_orb(CurFinArgs a, int ispin)
       s[1] = s[1] + o[2];
       s[2] = s[2] + o[3];
     }
-    const double kA[3] = {a.veck[3 * ik] + a.ax, a.veck[3 * ik + 1] + a.ay, a.veck[3 * ik + 2] + a.az};
+    double A[3] = {a.ax, a.ay, a.az};
+    if (a.actab) {
+      const int t = *a.itt;
+      A[0] = a.actab[3 * t];
+      A[1] = a.actab[3 * t + 1];
+      A[2] = a.actab[3 * t + 2];
+    }
+    const double kA[3] = {a.veck[3 * ik] + A[0], a.veck[3 * ik + 1] + A[1], a.veck[3 * ik + 2] + A[2]};
     const double occ = a.rocc[io + (size_t)a.no * (ik + (size_t)a.nk * ispin)];
     for (int d = 0; d < 3; ++d) {
       const double j1 = kA[d] * rt;
@@ -1095,6 +1186,11 @@ struct salmon_hip_ctx {
   size_t osz = 0, ngrid = 0;
   double coef_lap[12] = {}, coef_nab[12] = {}, coef_lap0 = 0, hgs[3] = {}, hvol = 0;
   hipStream_t own = nullptr, stream = nullptr;
+  // projections run on `side`, forked from and joined back into `stream` by events, so
+  // that they overlap the stencil pass reading the same input
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool join_pending = false;
   bool have_V = false, have_k = false;
   double *d_veck = nullptr, *d_rocc = nullptr, *d_wtk = nullptr, *d_V = nullptr, *d_kpar = nullptr;
   // nonlocal
@@ -1121,6 +1217,9 @@ struct salmon_hip_ctx {
   double2 *d_fft[2] = {nullptr, nullptr};
   // stencil launch shape
   int TY = 1, TZ = 1, nyt = 1, nzt = 1;
+  // launch shape of the terms without the Taylor sum (salmon_hip_autotune): tile_plane -1 = rule
+  // (whole plane when it fits 256 threads), 0 = y tiles, 1 = whole plane; mid_nzt 0 = rule
+  int tile_plane = -1, mid_nzt = 0;
   // kernel timing (salmon_hip_prof_*): start/stop event pairs per launch
   bool prof = false;
   std::vector<hipEvent_t> ev;
@@ -1143,6 +1242,9 @@ static int check(salmon_hip_t h) {
   if (!h) return fail(SALMON_HIP_EINVAL, "null handle");
   return 0;
 }
+
+// Error text from the other translation units of the library (the predictor).
+void salmon_hip_internal_error(const char *msg) { g_err = msg; }
 
 extern "C" {
 
@@ -1173,8 +1275,11 @@ int salmon_hip_create(int nx, int ny, int nz, int ly, int nspin, int no, int nk,
   c->hvol = hvol;
   int rc = 0;
   auto bail = [&](int code) { salmon_hip_destroy(c); return code; };
-  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess)
-    return bail(fail(SALMON_HIP_EHIP, "hipStreamCreate failed"));
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(SALMON_HIP_EHIP, "stream / event creation failed"));
   c->stream = c->own;
   if ((rc = dalloc(&c->d_veck, 3 * (size_t)nk))) return bail(rc);
   if ((rc = dalloc(&c->d_rocc, (size_t)no * nk * nspin))) return bail(rc);
@@ -1199,7 +1304,7 @@ int salmon_hip_create(int nx, int ny, int nz, int ly, int nspin, int no, int nk,
   if (c->dens_ngrp > 1 && (rc = dalloc(&c->d_denspart, c->ngrid * nspin * c->dens_ngrp))) return bail(rc);
   if ((rc = dalloc(&c->d_curpart, 4 * (size_t)c->nyt * c->norb))) return bail(rc);
   if ((rc = dalloc(&c->d_contrib, 3 * (size_t)no * nk))) return bail(rc);
-  if ((rc = dalloc(&c->d_sink, kTile))) return bail(rc);
+  if ((rc = dalloc(&c->d_sink, 1024))) return bail(rc);  // one element per thread of the largest tile
   *out = c;
   return 0;
 }
@@ -1219,6 +1324,10 @@ int salmon_hip_destroy(salmon_hip_t c) {
   for (double2 *&p : c->tscr) dfree(p);
   for (float2 *&p : c->tscr32) dfree(p);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  if (c->side) (void)hipStreamSynchronize(c->side);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
   return 0;
@@ -1226,7 +1335,7 @@ int salmon_hip_destroy(salmon_hip_t c) {
 
 int salmon_hip_set_stream(salmon_hip_t c, void *stream) {
   if (int rc = check(c)) return rc;
-  c->stream = stream ? (hipStream_t)stream : c->own;
+  c->stream = (hipStream_t)stream;  // NULL: HIP's null stream (ordered with every blocking stream)
   return 0;
 }
 
@@ -1361,9 +1470,7 @@ int salmon_hip_set_vlocal_device(salmon_hip_t c, const double *vlocal_dev) {
   return 0;
 }
 
-int salmon_hip_update_kvector(salmon_hip_t c, const double *vec_Ac) {
-  if (int rc = check(c)) return rc;
-  if (!vec_Ac) return fail(SALMON_HIP_EINVAL, "vec_Ac is null");
+static int update_kvector_impl(salmon_hip_t c, const double *vec_Ac, const double *actab, const int *itt) {
   KvecArgs a;
   a.veck = c->d_veck;
   a.rxyz = c->d_rxyz;
@@ -1379,9 +1486,11 @@ int salmon_hip_update_kvector(salmon_hip_t c, const double *vec_Ac) {
   a.kpar = c->d_kpar;
   memcpy(a.coef_nab, c->coef_nab, sizeof(a.coef_nab));
   a.coef_lap0 = c->coef_lap0;
-  a.ax = vec_Ac[0];
-  a.ay = vec_Ac[1];
-  a.az = vec_Ac[2];
+  a.ax = vec_Ac ? vec_Ac[0] : 0.0;
+  a.ay = vec_Ac ? vec_Ac[1] : 0.0;
+  a.az = vec_Ac ? vec_Ac[2] : 0.0;
+  a.actab = actab;
+  a.itt = itt;
   a.nlma = c->nlma;
   a.nps = c->nps;
   a.nk = c->nk;
@@ -1390,6 +1499,12 @@ int salmon_hip_update_kvector(salmon_hip_t c, const double *vec_Ac) {
   LAUNCHCHK("k_update_kvector");
   c->have_k = true;
   return 0;
+}
+
+int salmon_hip_update_kvector(salmon_hip_t c, const double *vec_Ac) {
+  if (int rc = check(c)) return rc;
+  if (!vec_Ac) return fail(SALMON_HIP_EINVAL, "vec_Ac is null");
+  return update_kvector_impl(c, vec_Ac, nullptr, nullptr);
 }
 
 int salmon_hip_set_vec_ac(salmon_hip_t c, const double *vec_Ac) {
@@ -1433,39 +1548,63 @@ static int launch_proj(salmon_hip_t c, const void *psi, int nw, bool f32 = false
   dim3 grid((unsigned)(a.ngrp * c->natom * c->nk));
   hipEvent_t e0, e1;
   prof_events(c, nw == 1 ? SALMON_HIP_KERNEL_PROJ : SALMON_HIP_KERNEL_PROJ_CURRENT, &e0, &e1);
+  // fork: everything enqueued on `stream` so far (the producer of psi, the last reader of
+  // the projection buffer) precedes the projection
+  HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+  HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
   if (nw == 1 && f32)
-    hipExtLaunchKernelGGL((k_proj<1, float2>), grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+    hipExtLaunchKernelGGL((k_proj<1, float2>), grid, dim3(kBlock), 0, c->side, e0, e1, 0, a);
   else if (nw == 1)
-    hipExtLaunchKernelGGL((k_proj<1, double2>), grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+    hipExtLaunchKernelGGL((k_proj<1, double2>), grid, dim3(kBlock), 0, c->side, e0, e1, 0, a);
   else
-    hipExtLaunchKernelGGL((k_proj<4, double2>), grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+    hipExtLaunchKernelGGL((k_proj<4, double2>), grid, dim3(kBlock), 0, c->side, e0, e1, 0, a);
   LAUNCHCHK("k_proj");
+  HIPCHK(hipEventRecord(c->ev_join, c->side));
+  c->join_pending = true;
+  return 0;
+}
+
+// join: later work on `stream` waits for the last projection
+static int join_proj(salmon_hip_t c) {
+  if (!c->join_pending) return 0;
+  HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  c->join_pending = false;
   return 0;
 }
 
 }  // extern "C"
 
-template <typename TI, int HC, int NC>
+template <typename TI, int HC, int NC, int BS>
 static size_t stencil_lds_bytes() {
-  return sizeof(TI) * (kRingT * kTile + kRingH * HC * kTile) + sizeof(double2) * (NC >= 0 ? 2 * NC * kTile : 0) +
-         sizeof(double) * kRingH * kTile;
+  return sizeof(TI) * (kRingT * BS + kRingH * HC * BS) + sizeof(double2) * (NC >= 0 ? 2 * NC * BS : 0) +
+         sizeof(double) * kRingH * BS;
 }
 
-template <typename TI, typename R, typename TO, int HC, int NC>
+template <typename TI, typename R, typename TO, int HC, int NC, int BS = kTile>
 static int launch_stencil_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
-  const size_t shmem = stencil_lds_bytes<TI, HC, NC>();
+  const size_t shmem = stencil_lds_bytes<TI, HC, NC, BS>();
   static bool attr_set = false;
   if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencil<TI, R, TO, HC, NC>,
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencil<TI, R, TO, HC, NC, BS>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     attr_set = true;
   }
-  hipExtLaunchKernelGGL((k_stencil<TI, R, TO, HC, NC>), grid, dim3(kBlock), shmem, c->stream, e0, e1, 0, a);
+  hipExtLaunchKernelGGL((k_stencil<TI, R, TO, HC, NC, BS>), grid, dim3(BS), shmem, c->stream, e0, e1, 0, a);
   return 0;
 }
 
 // Stencil stages: plain FP64, and the three stages of the mixed-precision chain.
 enum StencilStage { kStagePlain = 0, kStageMixedFirst = 1, kStageMixedMiddle = 2, kStageMixedFinal = 3 };
+
+// Whole-plane tiles (no y halo) for the terms without the Taylor sum, BS >= nx * ny.
+template <int BS>
+static int launch_stencil_plane(salmon_hip_ctx *c, int stage, dim3 grid, hipEvent_t e0, hipEvent_t e1,
+                                const StencilArgs &a) {
+  if (stage == kStageMixedFirst) return launch_stencil_t<double2, float, float2, 0, -1, BS>(c, grid, e0, e1, a);
+  if (stage == kStageMixedMiddle) return launch_stencil_t<float2, float, float2, 0, -1, BS>(c, grid, e0, e1, a);
+  return launch_stencil_t<double2, double, double2, 0, -1, BS>(c, grid, e0, e1, a);
+}
+
 
 template <int HC>
 static int launch_stencil_hc(salmon_hip_ctx *c, int stage, int nc, dim3 grid, hipEvent_t e0, hipEvent_t e1,
@@ -1526,8 +1665,10 @@ static int launch_scatter(salmon_hip_t c, void *hout, double2 s, bool f32 = fals
 // One H application: hout = H in (fin = false), or the last Taylor term
 // hout = psi + c1 h1 + ... + c_N H in (fin = true, comb = [psi, h1 .. h_{N-2}]).
 // The projections c of `in` must be current (launch_proj).
-static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, bool fin, void *const *comb, int ncomb,
-                       const double *zc, int stage = kStagePlain) {
+// Stencil descriptor of one H application; *bs_out = whole-plane block size (0 = y tiles).
+// tiled_only forces the y-tile shape (the batched step).
+static StencilArgs stencil_args(salmon_hip_t c, const void *in, void *hout, bool fin, void *const *comb, int ncomb,
+                                const double *zc, int *bs_out, bool tiled_only) {
   StencilArgs a;
   memset(&a, 0, sizeof(a));
   a.in = in;
@@ -1544,19 +1685,49 @@ static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, boo
   a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.ly = c->ly;
   a.nspin = c->nspin; a.nb = c->nb; a.nk = c->nk;
   a.TY = c->TY; a.TZ = c->TZ; a.nyt = c->nyt; a.nzt = c->nzt;
-  a.nblk = c->norb * c->nyt * c->nzt;
+  // terms without the Taylor sum use whole xy-plane tiles when a plane fits one block
+  const int plane = c->nx * c->ny;
+  // rule (tile_plane = -1): whole planes only up to 256 points (one 256-thread block); the
+  // 1024-thread FP64 block spills and holds one block per CU (slower at 32x32, measured)
+  const bool use_plane = !tiled_only && nc < 0 && plane <= 1024 &&
+                         (c->tile_plane == 1 || (c->tile_plane == -1 && plane <= 256));
+  const int bs = !use_plane ? 0 : (plane <= 256 ? 256 : (plane <= 512 ? 512 : 1024));
+  if (bs) {
+    a.TY = c->ny;
+    a.nyt = 1;
+    a.nzt = std::max(1, std::min((512 + c->norb - 1) / c->norb, c->nz / 8));
+  }
+  if (nc < 0 && c->mid_nzt > 0) a.nzt = std::min(c->mid_nzt, c->nz);
+  if (bs || (nc < 0 && c->mid_nzt > 0)) {
+    a.TZ = (c->nz + a.nzt - 1) / a.nzt;
+    a.nzt = (c->nz + a.TZ - 1) / a.TZ;
+  }
+  a.nblk = c->norb * a.nyt * a.nzt;
+  *bs_out = bs;
+  return a;
+}
+
+static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, bool fin, void *const *comb, int ncomb,
+                       const double *zc, int stage = kStagePlain) {
+  int bs = 0;
+  const int nc = fin ? ncomb : -1;
+  StencilArgs a = stencil_args(c, in, hout, fin, comb, ncomb, zc, &bs, false);
   dim3 grid((unsigned)a.nblk);
   hipEvent_t e0, e1;
   prof_events(c, kind, &e0, &e1);
   const int hc = (8 * c->nx + kTile - 1) / kTile;
   int rc = 0;
-  if (hc <= 1) rc = launch_stencil_hc<1>(c, stage, nc, grid, e0, e1, a);
+  if (bs == 256) rc = launch_stencil_plane<256>(c, stage, grid, e0, e1, a);
+  else if (bs == 512) rc = launch_stencil_plane<512>(c, stage, grid, e0, e1, a);
+  else if (bs == 1024) rc = launch_stencil_plane<1024>(c, stage, grid, e0, e1, a);
+  else if (hc <= 1) rc = launch_stencil_hc<1>(c, stage, nc, grid, e0, e1, a);
   else if (hc <= 2) rc = launch_stencil_hc<2>(c, stage, nc, grid, e0, e1, a);
   else if (hc <= 4) rc = launch_stencil_hc<4>(c, stage, nc, grid, e0, e1, a);
   else rc = launch_stencil_hc<8>(c, stage, nc, grid, e0, e1, a);
   if (rc) return rc;
   LAUNCHCHK("k_stencil");
   const bool f32out = stage == kStageMixedFirst || stage == kStageMixedMiddle;
+  if (int rc = join_proj(c)) return rc;
   if (c->natom > 0 && c->nlma > 0)
     return launch_scatter(c, hout, fin ? a.zc[ncomb] : make_double2(1.0, 0.0), f32out);
   return 0;
@@ -1642,6 +1813,76 @@ int salmon_hip_taylor(salmon_hip_t c, const double *zc, int n_hamil, void *tspsi
   return launch_hpsi(c, SALMON_HIP_KERNEL_HPSI_TAYLOR_LAST, h[n_hamil - 1], tspsi_out, true, h, n_hamil - 1, zc);
 }
 
+int salmon_hip_autotune(salmon_hip_t c, int reps, int *choice, double *best_ms) {
+  if (int rc = check(c)) return rc;
+  if (int rc = ready(c)) return rc;
+  if (reps < 1) reps = 3;
+  // two orbital buffers of scratch (zero orbitals: the timing does not depend on the values)
+  double2 *bin = nullptr, *bout = nullptr;
+  int rc = dalloc(&bin, c->osz * c->norb);
+  if (!rc) rc = dalloc(&bout, c->osz * c->norb);
+  if (rc) { dfree(bin); return rc; }
+  (void)hipMemsetAsync(bin, 0, sizeof(double2) * c->osz * c->norb, c->stream);
+  struct Cand { int plane, nzt; };
+  std::vector<Cand> cands;
+  if (c->nx * c->ny <= 1024)
+    for (int n : {1, 2}) cands.push_back({1, n});
+  for (int n : {1, 2, 4})
+    if (n <= std::max(1, c->nz / 8)) cands.push_back({0, n});
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int save_plane = c->tile_plane, save_nzt = c->mid_nzt;
+  const bool save_prof = c->prof;
+  c->prof = false;
+  double best = 1e30;
+  int besti = 0;
+  for (size_t i = 0; i < cands.size() && !rc; ++i) {
+    c->tile_plane = cands[i].plane;
+    c->mid_nzt = cands[i].nzt;
+    // local stencil only (the KB scatter does not depend on the tile shape)
+    const int nat = c->natom;
+    c->natom = 0;
+    rc = launch_hpsi(c, SALMON_HIP_KERNEL_HPSI0, bin, bout, false, nullptr, 0, nullptr);
+    if (!rc && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(SALMON_HIP_EHIP, "event record");
+    for (int r = 0; r < reps && !rc; ++r) rc = launch_hpsi(c, SALMON_HIP_KERNEL_HPSI0, bin, bout, false, nullptr, 0, nullptr);
+    c->natom = nat;
+    if (rc) break;
+    float ms = 0.f;
+    if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = fail(SALMON_HIP_EHIP, "autotune timing failed");
+      break;
+    }
+    const double t = ms / reps;
+    if (t < best) { best = t; besti = (int)i; }
+  }
+  c->prof = save_prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(c->stream);
+  dfree(bin);
+  dfree(bout);
+  if (rc) {
+    c->tile_plane = save_plane;
+    c->mid_nzt = save_nzt;
+    return rc;
+  }
+  c->tile_plane = cands[besti].plane;
+  c->mid_nzt = cands[besti].nzt;
+  if (choice) *choice = 10 * cands[besti].plane + cands[besti].nzt;
+  if (best_ms) *best_ms = best;
+  return 0;
+}
+
+int salmon_hip_set_launch_shape(salmon_hip_t c, int whole_plane, int z_chunks) {
+  if (int rc = check(c)) return rc;
+  if (whole_plane < -1 || whole_plane > 1 || z_chunks < 0) return fail(SALMON_HIP_EINVAL, "launch shape");
+  c->tile_plane = whole_plane;
+  c->mid_nzt = z_chunks;
+  return 0;
+}
+
 int salmon_hip_density(salmon_hip_t c, const void *psi, double *rho_dev) {
   if (int rc = check(c)) return rc;
   if (!psi || !rho_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
@@ -1666,10 +1907,8 @@ int salmon_hip_density(salmon_hip_t c, const void *psi, double *rho_dev) {
   return 0;
 }
 
-int salmon_hip_current(salmon_hip_t c, const double *vec_Ac, const void *psi, double *curr_dev) {
-  if (int rc = check(c)) return rc;
-  if (!vec_Ac || !psi || !curr_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
-  if (c->natom > 0 && !c->have_k) return fail(SALMON_HIP_ESTATE, "update_kvector not called");
+static int current_impl(salmon_hip_t c, const double *vec_Ac, const double *actab, const int *itt, const void *psi,
+                        double *curr_dev) {
   CurArgs a;
   a.psi = (const double2 *)psi;
   a.part = c->d_curpart;
@@ -1679,10 +1918,12 @@ int salmon_hip_current(salmon_hip_t c, const double *vec_Ac, const void *psi, do
   a.TY = c->TY;
   a.nyt = c->nyt;
   const size_t shm = sizeof(double2) * 2 * (size_t)(c->TY + 4) * c->nx;
-  hipLaunchKernelGGL(k_current_local, dim3((unsigned)(c->norb * c->nyt)), dim3(kBlock), shm, c->stream, a);
-  LAUNCHCHK("k_current_local");
+  // the projector moments run on the side stream, overlapping the local part
   if (c->natom > 0)
     if (int rc = launch_proj(c, psi, 4)) return rc;
+  hipLaunchKernelGGL(k_current_local, dim3((unsigned)(c->norb * c->nyt)), dim3(kBlock), shm, c->stream, a);
+  LAUNCHCHK("k_current_local");
+  if (int rc = join_proj(c)) return rc;
   CurFinArgs f;
   f.part = c->d_curpart;
   f.nl = c->natom > 0 ? c->d_curnl : nullptr;
@@ -1692,9 +1933,11 @@ int salmon_hip_current(salmon_hip_t c, const double *vec_Ac, const void *psi, do
   f.wtk = c->d_wtk;
   f.contrib = c->d_contrib;
   f.curr = curr_dev;
-  f.ax = vec_Ac[0];
-  f.ay = vec_Ac[1];
-  f.az = vec_Ac[2];
+  f.ax = vec_Ac ? vec_Ac[0] : 0.0;
+  f.ay = vec_Ac ? vec_Ac[1] : 0.0;
+  f.az = vec_Ac ? vec_Ac[2] : 0.0;
+  f.actab = actab;
+  f.itt = itt;
   f.inv_ngrid = 1.0 / (double)c->ngrid;
   f.nslab = c->nyt;
   f.nspin = c->nspin;
@@ -1709,6 +1952,13 @@ int salmon_hip_current(salmon_hip_t c, const double *vec_Ac, const void *psi, do
     LAUNCHCHK("k_current_sum");
   }
   return 0;
+}
+
+int salmon_hip_current(salmon_hip_t c, const double *vec_Ac, const void *psi, double *curr_dev) {
+  if (int rc = check(c)) return rc;
+  if (!vec_Ac || !psi || !curr_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+  if (c->natom > 0 && !c->have_k) return fail(SALMON_HIP_ESTATE, "update_kvector not called");
+  return current_impl(c, vec_Ac, nullptr, nullptr, psi, curr_dev);
 }
 
 int salmon_hip_dipole(salmon_hip_t c, const double *rho_dev, double *out_dev) {
@@ -1842,6 +2092,397 @@ int salmon_hip_update_potential(salmon_hip_t c, const double *rho_dev, const dou
   return launch_vlocal(c, vpsl_dev, c->d_vh, nullptr, rho_dev, vmix_dev, vout_dev);
 }
 
+// ---------------------------------------------------------------------------
+// Batched step over many independent systems (no counterpart in the reference).
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+// record the per-step observables at the device step counter, advance it
+__global__ void k_record(const double *const *csrc, double *const *cdst, const int *cw, const double *const *dsrc,
+                         double *const *ddst, const int *nt, int nsys, int *itt) {
+  const int i = threadIdx.x;
+  const int t = *itt;
+  if (i < nsys && t >= 1 && t <= nt[i]) {
+    for (int k = 0; k < cw[i]; ++k) cdst[i][(size_t)(t - 1) * cw[i] + k] = csrc[i][k];
+    for (int k = 0; k < 4; ++k) ddst[i][(size_t)(t - 1) * 4 + k] = dsrc[i][k];
+  }
+  __syncthreads();
+  if (i == 0) *itt = t + 1;
+}
+
+__global__ void k_set_itt(int *itt, int v) { *itt = v; }
+
+template <int HC, int NC>
+int launch_packed_t(hipStream_t st, const StencilArgs *sys, const int2 *tiles, int ntiles, int grid) {
+  const size_t shmem = stencil_lds_bytes<double2, HC, NC, kTile>();
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencil_packed<double2, double, double2, HC, NC, kTile>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_stencil_packed<double2, double, double2, HC, NC, kTile>), dim3(grid), dim3(kTile), shmem, st,
+                     sys, tiles, ntiles);
+  LAUNCHCHK("k_stencil_packed");
+  return 0;
+}
+
+template <int HC>
+int launch_packed_hc(hipStream_t st, int nc, const StencilArgs *sys, const int2 *tiles, int ntiles, int grid) {
+  switch (nc) {
+    case -1: return launch_packed_t<HC, -1>(st, sys, tiles, ntiles, grid);
+    case 0: return launch_packed_t<HC, 0>(st, sys, tiles, ntiles, grid);
+    case 1: return launch_packed_t<HC, 1>(st, sys, tiles, ntiles, grid);
+    case 2: return launch_packed_t<HC, 2>(st, sys, tiles, ntiles, grid);
+    case 3: return launch_packed_t<HC, 3>(st, sys, tiles, ntiles, grid);
+    case 4: return launch_packed_t<HC, 4>(st, sys, tiles, ntiles, grid);
+    default: return launch_packed_t<HC, 5>(st, sys, tiles, ntiles, grid);
+  }
+}
+
+int launch_packed(hipStream_t st, int hc, int nc, const StencilArgs *sys, const int2 *tiles, int ntiles, int grid) {
+  if (hc <= 1) return launch_packed_hc<1>(st, nc, sys, tiles, ntiles, grid);
+  if (hc <= 2) return launch_packed_hc<2>(st, nc, sys, tiles, ntiles, grid);
+  if (hc <= 4) return launch_packed_hc<4>(st, nc, sys, tiles, ntiles, grid);
+  return launch_packed_hc<8>(st, nc, sys, tiles, ntiles, grid);
+}
+
+}  // namespace
+
+struct salmon_hip_batch {
+  std::vector<salmon_hip_t> h;
+  hipStream_t stream = nullptr;
+  struct Sys {
+    void *psi[2] = {nullptr, nullptr};
+    void *work = nullptr;
+    double *rho = nullptr, *curr_t = nullptr, *dip_t = nullptr;
+    const double *actab = nullptr;
+    int nt = 0;
+    double *curr_s = nullptr, *dip_s = nullptr;  // this step's observables (scratch)
+  };
+  std::vector<Sys> sys;
+  struct Group {
+    int hc = 1;
+    std::vector<int> mem;               // systems of this halo class
+    int2 *d_tiles[2] = {nullptr, nullptr};  // middle terms, last term (different z-chunking)
+    int ntiles[2] = {0, 0}, grid[2] = {0, 0};
+    StencilArgs *d_args = nullptr;      // [parity][term][member]
+  };
+  std::vector<Group> groups;
+  int n_hamil = 0;
+  std::vector<double> zc;
+  int z_chunks = 0;                     // 0: each system's own rule
+  int grid_mult = 4;                    // persistent blocks = grid_mult x 256 (capped at the tile count)
+  int *d_itt = nullptr;
+  // k_record tables
+  double **d_csrc = nullptr, **d_cdst = nullptr, **d_dsrc = nullptr, **d_ddst = nullptr;
+  int *d_cw = nullptr, *d_nt = nullptr;
+  int parity = 0;
+  bool prepared = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+template <typename T>
+int balloc(salmon_hip_batch *b, T **p, size_t n) {
+  if (int rc = dalloc(p, std::max<size_t>(n, 1))) return rc;
+  b->allocs.push_back((void *)*p);
+  return 0;
+}
+
+// Taylor buffers of system i at parity p: h[0] = input, h[1] = work, h[2..] = handle scratch, out
+void batch_buffers(salmon_hip_batch *b, int i, int p, void **hh, void **out) {
+  salmon_hip_t c = b->h[i];
+  hh[0] = b->sys[i].psi[p];
+  hh[1] = b->sys[i].work;
+  for (int n = 2; n < b->n_hamil; ++n) hh[n] = c->tscr[n - 2];
+  *out = b->sys[i].psi[1 - p];
+}
+
+// (re)build tile lists and descriptor tables for the current z_chunks / grid_mult
+int batch_tables(salmon_hip_batch *b) {
+  for (auto &g : b->groups) {
+    std::vector<int2> tiles[2];
+    std::vector<StencilArgs> args;
+    for (int p = 0; p < 2; ++p)
+      for (int n = 1; n <= b->n_hamil; ++n)
+        for (size_t m = 0; m < g.mem.size(); ++m) {
+          const int i = g.mem[m];
+          salmon_hip_t c = b->h[i];
+          void *hh[kMaxComb + 1], *out;
+          batch_buffers(b, i, p, hh, &out);
+          const bool fin = n == b->n_hamil;
+          int bs = 0;
+          const int save = c->mid_nzt;
+          c->mid_nzt = b->z_chunks;
+          StencilArgs a = stencil_args(c, hh[n - 1], fin ? out : hh[n], fin, hh, n - 1, b->zc.data(), &bs, true);
+          c->mid_nzt = save;
+          args.push_back(a);
+          if (p == 0 && (n == 1 || fin))
+            for (int t = 0; t < a.nblk; ++t) tiles[fin ? 1 : 0].push_back(make_int2((int)m, t));
+        }
+    for (int k = 0; k < 2; ++k) {
+      dfree(g.d_tiles[k]);
+      if (int rc = dalloc(&g.d_tiles[k], tiles[k].size())) return rc;
+      HIPCHK(hipMemcpy(g.d_tiles[k], tiles[k].data(), sizeof(int2) * tiles[k].size(), hipMemcpyHostToDevice));
+      g.ntiles[k] = (int)tiles[k].size();
+      g.grid[k] = std::min(g.ntiles[k], b->grid_mult * 256);
+    }
+    dfree(g.d_args);
+    if (int rc = dalloc(&g.d_args, args.size())) return rc;
+    HIPCHK(hipMemcpy(g.d_args, args.data(), sizeof(StencilArgs) * args.size(), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int batch_stencil(salmon_hip_batch *b, int p, int n) {
+  const int nc = n == b->n_hamil ? b->n_hamil - 1 : -1;
+  for (auto &g : b->groups) {
+    const StencilArgs *args = g.d_args + ((size_t)p * b->n_hamil + (n - 1)) * g.mem.size();
+    const int k = nc >= 0 ? 1 : 0;
+    if (int rc = launch_packed(b->stream, g.hc, nc, args, g.d_tiles[k], g.ntiles[k], g.grid[k])) return rc;
+  }
+  return 0;
+}
+
+int batch_step(salmon_hip_batch *b, int p) {
+  const int N = b->n_hamil;
+  const size_t ns = b->h.size();
+  for (size_t i = 0; i < ns; ++i)
+    if (int rc = update_kvector_impl(b->h[i], nullptr, b->sys[i].actab, b->d_itt)) return rc;
+  for (int n = 1; n <= N; ++n) {
+    for (size_t i = 0; i < ns; ++i) {
+      salmon_hip_t c = b->h[i];
+      void *hh[kMaxComb + 1], *out;
+      batch_buffers(b, (int)i, p, hh, &out);
+      if (c->natom > 0)
+        if (int rc = launch_proj(c, hh[n - 1], 1)) return rc;
+    }
+    if (int rc = batch_stencil(b, p, n)) return rc;
+    for (size_t i = 0; i < ns; ++i) {
+      salmon_hip_t c = b->h[i];
+      void *hh[kMaxComb + 1], *out;
+      batch_buffers(b, (int)i, p, hh, &out);
+      if (int rc = join_proj(c)) return rc;
+      if (c->natom > 0 && c->nlma > 0) {
+        const double2 sc = n == N ? make_double2(b->zc[2 * (N - 1)], b->zc[2 * (N - 1) + 1]) : make_double2(1.0, 0.0);
+        if (int rc = launch_scatter(c, n == N ? out : hh[n], sc)) return rc;
+      }
+    }
+  }
+  for (size_t i = 0; i < ns; ++i) {
+    salmon_hip_t c = b->h[i];
+    void *out = b->sys[i].psi[1 - p];
+    if (int rc = salmon_hip_density(c, out, b->sys[i].rho)) return rc;
+    if (int rc = current_impl(c, nullptr, b->sys[i].actab, b->d_itt, out, b->sys[i].curr_s)) return rc;
+    if (int rc = salmon_hip_dipole(c, b->sys[i].rho, b->sys[i].dip_s)) return rc;
+  }
+  hipLaunchKernelGGL(k_record, dim3(1), dim3(std::max<size_t>(64, (ns + 63) / 64 * 64)), 0, b->stream, b->d_csrc,
+                     b->d_cdst, b->d_cw, b->d_dsrc, b->d_ddst, b->d_nt, (int)ns, b->d_itt);
+  LAUNCHCHK("k_record");
+  return 0;
+}
+
+int bcheck(salmon_hip_batch_t b) {
+  if (!b) return fail(SALMON_HIP_EINVAL, "null batch");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int salmon_hip_batch_create(const salmon_hip_t *handles, int n, void *stream, salmon_hip_batch_t *out) {
+  if (!out || !handles || n < 1 || n > 1024) return fail(SALMON_HIP_EINVAL, "batch of %d systems", n);
+  *out = nullptr;
+  salmon_hip_batch *b = new salmon_hip_batch();
+  for (int i = 0; i < n; ++i)
+    if (!handles[i]) { delete b; return fail(SALMON_HIP_EINVAL, "handle %d is null", i); }
+  // the null stream cannot be captured into a graph: NULL selects the first handle's own stream
+  b->stream = stream ? (hipStream_t)stream : handles[0]->own;
+  for (int i = 0; i < n; ++i) {
+    b->h.push_back(handles[i]);
+    handles[i]->stream = b->stream;
+  }
+  b->sys.resize(n);
+  int rc = balloc(b, &b->d_itt, 1);
+  if (!rc) rc = balloc(b, &b->d_csrc, n);
+  if (!rc) rc = balloc(b, &b->d_cdst, n);
+  if (!rc) rc = balloc(b, &b->d_dsrc, n);
+  if (!rc) rc = balloc(b, &b->d_ddst, n);
+  if (!rc) rc = balloc(b, &b->d_cw, n);
+  if (!rc) rc = balloc(b, &b->d_nt, n);
+  for (int i = 0; i < n && !rc; ++i) {
+    rc = balloc(b, &b->sys[i].curr_s, 3 * (size_t)handles[i]->nspin);
+    if (!rc) rc = balloc(b, &b->sys[i].dip_s, 4);
+  }
+  if (rc) { salmon_hip_batch_destroy(b); return rc; }
+  *out = b;
+  return 0;
+}
+
+int salmon_hip_batch_destroy(salmon_hip_batch_t b) {
+  if (!b) return 0;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto &g : b->groups) { dfree(g.d_tiles[0]); dfree(g.d_tiles[1]); dfree(g.d_args); }
+  for (void *p : b->allocs) (void)hipFree(p);
+  delete b;
+  return 0;
+}
+
+int salmon_hip_batch_set_system(salmon_hip_batch_t b, int i, void *psi_a, void *psi_b, void *work, double *rho_dev,
+                                const double *ac_dev, int nt, double *curr_t_dev, double *dip_t_dev) {
+  if (int rc = bcheck(b)) return rc;
+  if (i < 0 || i >= (int)b->h.size()) return fail(SALMON_HIP_EINVAL, "system %d of %zu", i, b->h.size());
+  if (!psi_a || !psi_b || !work || !rho_dev || !ac_dev || !curr_t_dev || !dip_t_dev || nt < 1)
+    return fail(SALMON_HIP_EINVAL, "system %d: null buffer or nt < 1", i);
+  auto &y = b->sys[i];
+  y.psi[0] = psi_a; y.psi[1] = psi_b; y.work = work; y.rho = rho_dev; y.actab = ac_dev; y.nt = nt;
+  y.curr_t = curr_t_dev; y.dip_t = dip_t_dev;
+  b->prepared = false;
+  return 0;
+}
+
+int salmon_hip_batch_prepare(salmon_hip_batch_t b, const double *zc, int n_hamil) {
+  if (int rc = bcheck(b)) return rc;
+  if (!zc || n_hamil < 2 || n_hamil > kMaxComb + 1)
+    return fail(SALMON_HIP_EINVAL, "zc null or n_hamil=%d outside 2..%d", n_hamil, kMaxComb + 1);
+  const int ns = (int)b->h.size();
+  for (int i = 0; i < ns; ++i) {
+    salmon_hip_t c = b->h[i];
+    if (!b->sys[i].psi[0]) return fail(SALMON_HIP_ESTATE, "system %d: set_system not called", i);
+    if (!c->have_V) return fail(SALMON_HIP_ESTATE, "system %d: V_local not set", i);
+    if (c->prec != SALMON_HIP_PREC_FP64) return fail(SALMON_HIP_EUNSUPPORTED, "batched step is FP64");
+    while ((int)c->tscr.size() < n_hamil - 2) {
+      double2 *q = nullptr;
+      if (int rc = dalloc(&q, c->osz * c->norb)) return rc;
+      c->tscr.push_back(q);
+    }
+    c->have_k = true;  // the phase table is rebuilt from the device field table every step
+  }
+  b->n_hamil = n_hamil;
+  b->zc.assign(zc, zc + 2 * n_hamil);
+  for (auto &g : b->groups) { dfree(g.d_tiles[0]); dfree(g.d_tiles[1]); dfree(g.d_args); }
+  b->groups.clear();
+  for (int i = 0; i < ns; ++i) {
+    const int hc = (8 * b->h[i]->nx + kTile - 1) / kTile;
+    const int cls = hc <= 1 ? 1 : hc <= 2 ? 2 : hc <= 4 ? 4 : 8;
+    salmon_hip_batch::Group *g = nullptr;
+    for (auto &q : b->groups)
+      if (q.hc == cls) g = &q;
+    if (!g) {
+      b->groups.push_back(salmon_hip_batch::Group());
+      g = &b->groups.back();
+      g->hc = cls;
+    }
+    g->mem.push_back(i);
+  }
+  if (int rc = batch_tables(b)) return rc;
+  std::vector<const double *> cs(ns), ds(ns);
+  std::vector<double *> cd(ns), dd(ns);
+  std::vector<int> cw(ns), nt(ns);
+  for (int i = 0; i < ns; ++i) {
+    cs[i] = b->sys[i].curr_s; ds[i] = b->sys[i].dip_s; cd[i] = b->sys[i].curr_t; dd[i] = b->sys[i].dip_t;
+    cw[i] = 3 * b->h[i]->nspin; nt[i] = b->sys[i].nt;
+  }
+  HIPCHK(hipMemcpy(b->d_csrc, cs.data(), sizeof(double *) * ns, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_dsrc, ds.data(), sizeof(double *) * ns, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_cdst, cd.data(), sizeof(double *) * ns, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_ddst, dd.data(), sizeof(double *) * ns, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_cw, cw.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_nt, nt.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+  b->prepared = true;
+  b->parity = 0;
+  return salmon_hip_batch_set_time(b, 1);
+}
+
+int salmon_hip_batch_set_time(salmon_hip_batch_t b, int itt) {
+  if (int rc = bcheck(b)) return rc;
+  hipLaunchKernelGGL(k_set_itt, dim3(1), dim3(1), 0, b->stream, b->d_itt, itt);
+  LAUNCHCHK("k_set_itt");
+  b->parity = (itt - 1) & 1;
+  return 0;
+}
+
+int salmon_hip_batch_step(salmon_hip_batch_t b) {
+  if (int rc = bcheck(b)) return rc;
+  if (!b->prepared) return fail(SALMON_HIP_ESTATE, "batch_prepare not called");
+  if (int rc = batch_step(b, b->parity)) return rc;
+  b->parity ^= 1;
+  return 0;
+}
+
+int salmon_hip_batch_capture(salmon_hip_batch_t b) {
+  if (int rc = bcheck(b)) return rc;
+  if (!b->prepared) return fail(SALMON_HIP_ESTATE, "batch_prepare not called");
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = batch_step(b, b->parity);
+  if (!rc) rc = batch_step(b, b->parity ^ 1);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return fail(SALMON_HIP_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  b->graph = g;
+  HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+int salmon_hip_batch_replay(salmon_hip_batch_t b, int npairs) {
+  if (int rc = bcheck(b)) return rc;
+  if (!b->exec) return fail(SALMON_HIP_ESTATE, "batch_capture not called");
+  for (int k = 0; k < npairs; ++k) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+  return 0;
+}
+
+int salmon_hip_batch_autotune(salmon_hip_batch_t b, int reps, int *choice, double *best_ms) {
+  if (int rc = bcheck(b)) return rc;
+  if (!b->prepared) return fail(SALMON_HIP_ESTATE, "batch_prepare not called");
+  if (reps < 1) reps = 3;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double best = 1e30;
+  int bz = 0, bg = 4;
+  int rc = 0;
+  for (int z : {0, 1, 2}) {
+    for (int gm : {1, 2, 4, 8, 1 << 20}) {
+      b->z_chunks = z;
+      b->grid_mult = gm;
+      if ((rc = batch_tables(b))) break;
+      // middle-term packed stencil of every group (parity 0, term 2: reads h1, writes h2)
+      if ((rc = batch_stencil(b, 0, 2))) break;
+      if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(SALMON_HIP_EHIP, "event"); break; }
+      for (int r = 0; r < reps && !rc; ++r) rc = batch_stencil(b, 0, 2);
+      float ms = 0.f;
+      if (rc || hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        if (!rc) rc = fail(SALMON_HIP_EHIP, "autotune timing");
+        break;
+      }
+      if (ms / reps < best) { best = ms / reps; bz = z; bg = gm; }
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  b->z_chunks = bz;
+  b->grid_mult = bg;
+  if (int rc2 = batch_tables(b)) return rc2;
+  if (choice) *choice = 100 * bz + (bg > 64 ? 99 : bg);
+  if (best_ms) *best_ms = best;
+  return 0;
+}
+
 int salmon_hip_prof_start(salmon_hip_t c, int max_launches) {
   if (int rc = check(c)) return rc;
   if (max_launches < 1) return fail(SALMON_HIP_EINVAL, "max_launches=%d", max_launches);
@@ -1885,6 +2526,7 @@ int salmon_hip_prof_end(salmon_hip_t c) {
 
 int salmon_hip_synchronize(salmon_hip_t c) {
   if (int rc = check(c)) return rc;
+  HIPCHK(hipStreamSynchronize(c->side));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -30,6 +30,17 @@ EXPORTS = (
     "salmon_hip_hpsi",
     "salmon_hip_taylor",
     "salmon_hip_set_precision",
+    "salmon_hip_autotune",
+    "salmon_hip_set_launch_shape",
+    "salmon_hip_batch_create",
+    "salmon_hip_batch_destroy",
+    "salmon_hip_batch_set_system",
+    "salmon_hip_batch_prepare",
+    "salmon_hip_batch_autotune",
+    "salmon_hip_batch_set_time",
+    "salmon_hip_batch_step",
+    "salmon_hip_batch_capture",
+    "salmon_hip_batch_replay",
     "salmon_hip_density",
     "salmon_hip_current",
     "salmon_hip_dipole",
@@ -83,6 +94,18 @@ def load():
     L.salmon_hip_hpsi.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_taylor.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_set_precision.argtypes = [C.c_void_p, C.c_int]
+    L.salmon_hip_autotune.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
+    L.salmon_hip_set_launch_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    vp = C.c_void_p
+    L.salmon_hip_batch_create.argtypes = [C.POINTER(vp), C.c_int, vp, C.POINTER(vp)]
+    L.salmon_hip_batch_destroy.argtypes = [vp]
+    L.salmon_hip_batch_set_system.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, vp, vp]
+    L.salmon_hip_batch_prepare.argtypes = [vp, _dp, C.c_int]
+    L.salmon_hip_batch_autotune.argtypes = [vp, C.c_int, _ip, _dp]
+    L.salmon_hip_batch_set_time.argtypes = [vp, C.c_int]
+    L.salmon_hip_batch_step.argtypes = [vp]
+    L.salmon_hip_batch_capture.argtypes = [vp]
+    L.salmon_hip_batch_replay.argtypes = [vp, C.c_int]
     L.salmon_hip_density.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_current.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p]
     L.salmon_hip_dipole.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -155,7 +178,8 @@ class Handle:
             pass
 
     def set_stream(self, stream):
-        """stream: torch.cuda.Stream, raw hipStream_t int, or None for the handle's own."""
+        """stream: torch.cuda.Stream or raw hipStream_t int; torch's default stream (and 0 / None)
+        is HIP's null stream, so library work stays ordered with torch work on it."""
         raw = getattr(stream, "cuda_stream", stream)
         _chk(load().salmon_hip_set_stream(self.h, C.c_void_p(raw or None)), "set_stream")
 
@@ -191,6 +215,16 @@ class Handle:
                                       C.c_void_p(_ptr(work))), "taylor")
 
     PREC = {"fp64": 0, "mixed": 1}
+
+    def autotune(self, reps=3):
+        """Pick the fastest stencil launch shape; returns (whole_plane, z_chunks, ms per launch)."""
+        ch = C.c_int()
+        ms = C.c_double()
+        _chk(load().salmon_hip_autotune(self.h, int(reps), C.byref(ch), C.byref(ms)), "autotune")
+        return ch.value // 10, ch.value % 10, ms.value
+
+    def _set_shape(self, whole_plane, z_chunks):
+        _chk(load().salmon_hip_set_launch_shape(self.h, int(whole_plane), int(z_chunks)), "set_launch_shape")
 
     def set_precision(self, mode):
         """'fp64' (default) or 'mixed' (complex FP32 stencil chain, FP64 sums) for taylor."""
@@ -245,3 +279,54 @@ class Handle:
 
     def synchronize(self):
         _chk(load().salmon_hip_synchronize(self.h), "synchronize")
+
+
+class Batch:
+    """Batched step over several Handles (salmon_hip_batch_*)."""
+
+    def __init__(self, handles, stream=None):
+        L = load()
+        self.handles = list(handles)
+        arr = (C.c_void_p * len(self.handles))(*[h.h.value for h in self.handles])
+        raw = getattr(stream, "cuda_stream", stream)
+        b = C.c_void_p()
+        _chk(L.salmon_hip_batch_create(arr, len(self.handles), C.c_void_p(raw or None), C.byref(b)), "batch_create")
+        self.b = b
+
+    def close(self):
+        if getattr(self, "b", None) is not None and self.b.value:
+            load().salmon_hip_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_system(self, i, psi_a, psi_b, work, rho, ac, nt, curr_t, dip_t):
+        _chk(load().salmon_hip_batch_set_system(self.b, i, *[C.c_void_p(_ptr(x)) for x in (psi_a, psi_b, work, rho, ac)],
+                                                int(nt), C.c_void_p(_ptr(curr_t)), C.c_void_p(_ptr(dip_t))),
+             "batch_set_system")
+
+    def prepare(self, zc):
+        z = np.ascontiguousarray(np.asarray(zc, dtype=np.complex128)).view(np.float64)
+        _chk(load().salmon_hip_batch_prepare(self.b, _p(z), len(z) // 2), "batch_prepare")
+
+    def autotune(self, reps=3):
+        ch = C.c_int()
+        ms = C.c_double()
+        _chk(load().salmon_hip_batch_autotune(self.b, int(reps), C.byref(ch), C.byref(ms)), "batch_autotune")
+        return ch.value, ms.value
+
+    def set_time(self, itt):
+        _chk(load().salmon_hip_batch_set_time(self.b, int(itt)), "batch_set_time")
+
+    def step(self):
+        _chk(load().salmon_hip_batch_step(self.b), "batch_step")
+
+    def capture(self):
+        _chk(load().salmon_hip_batch_capture(self.b), "batch_capture")
+
+    def replay(self, npairs):
+        _chk(load().salmon_hip_batch_replay(self.b, int(npairs)), "batch_replay")

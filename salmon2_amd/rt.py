@@ -20,6 +20,13 @@ corrector Taylor step from the same input orbitals.  The library's taylor
 leaves its input untouched, so the orbital copy of the reference
 (psi_tmp, :370-372, 410-412) is not needed.
 
+``learned=Predictor`` (salmon2_amd.learned) replaces the predictor Taylor
+step by the network's midpoint potential: one Taylor step with it, accepted
+when the self-consistency residual max|V_pred - (V(t) + V(t+dt))/2| is within
+the predictor's tolerance; otherwise ``fallback="plain"`` redoes the step
+with the plain predictor-corrector and ``"reuse"`` takes the attempt as the
+predictor (one corrector Taylor step).
+
 ``precision="mixed"`` runs the Taylor chain H^n psi in complex FP32 with the
 projections, the Taylor sum, density and current in FP64.
 
@@ -44,7 +51,8 @@ from .hip import Handle
 
 
 class RTPropagator:
-    def __init__(self, sysm, device="cuda:0", psi=None, scf=False, pc=False, precision="fp64", trans_longi="tr"):
+    def __init__(self, sysm, device="cuda:0", psi=None, scf=False, pc=False, precision="fp64", trans_longi="tr",
+                 learned=None, fallback="plain", record=False):
         self.s = sysm
         self.dev = torch.device(device)
         torch.cuda.set_device(self.dev)
@@ -71,6 +79,25 @@ class RTPropagator:
             self.vcur = torch.zeros_like(self.vpsl)
             self.h.density(self.psi_a, self.rho)
             self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
+            self.vmid = torch.zeros_like(self.vpsl)  # the predictor-corrector midpoint potential of the last step
+        # learned predictor (salmon2_amd.learned.Predictor): needs the self-consistent predictor-corrector step
+        self.learned = learned
+        if learned is not None:
+            if not self.pc:
+                raise ValueError("the learned predictor replaces the predictor of the predictor-corrector (scf=pc=True)")
+            if fallback not in ("plain", "reuse"):
+                raise ValueError(f"fallback {fallback!r}: 'plain' or 'reuse'")
+            self.fallback = fallback
+            self.hist = [torch.zeros_like(self.vpsl) for _ in range(4)]  # V(t), V(t-dt), V(t-2dt), V(t-3dt)
+            self.hist[0].copy_(self.vcur)
+            self.nhist = 1
+            self.vpred = torch.zeros_like(self.vpsl)
+            self.vnew = torch.zeros_like(self.vpsl)
+            self.resid = torch.zeros(1, dtype=torch.float64, device=self.dev)
+            self.accepted = 0
+            self.attempted = 0
+            self.resid_log = []
+        self.record = [] if record else None
 
     def setup_outputs(self, nt):
         s = self.s
@@ -87,14 +114,23 @@ class RTPropagator:
         itt = self.itt
         A = 0.5 * (self.ac[itt] + self.ac[itt - 1])
         self.h.update_kvector(A)
-        if self.pc:
+        if self.learned is not None and self.nhist >= 4 and self._learned_step():
+            pass
+        else:
+            if self.pc:
+                self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
+                self.h.density(self.psi_b, self.rho)
+                self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)
             self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
             self.h.density(self.psi_b, self.rho)
-            self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur)
-        self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
-        self.h.density(self.psi_b, self.rho)
-        if self.scf:
-            self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
+            if self.scf:
+                self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
+        if self.learned is not None:
+            self.hist = [self.hist[3]] + self.hist[:3]
+            self.hist[0].copy_(self.vcur)
+            self.nhist = min(self.nhist + 1, 4)
+        if self.record is not None:
+            self.record.append((self.vcur.clone(), self.vmid.clone()))
         self.h.current(self.ac[itt], self.psi_b, self.curr_t[itt - 1])
         self.h.dipole(self.rho, self.dip_t[itt - 1])
         if self.trans_longi == "lo" and itt + 1 < len(self.ac):
@@ -102,6 +138,37 @@ class RTPropagator:
             self.ac_ind[itt + 1] = 2.0 * self.ac_ind[itt] - self.ac_ind[itt - 1] - 4.0 * math.pi * j * self.s.dt ** 2
             self.ac[itt + 1] = self.ac_ext[itt + 1] + self.ac_ind[itt + 1]
         self.psi_a, self.psi_b = self.psi_b, self.psi_a
+
+    def _learned_step(self):
+        """Predictor step by the network; True when the step is complete (accepted or fallen back)."""
+        P = self.learned
+        P.predict(self.hist, self.vpsl, self.vpred)
+        self.h.set_vlocal_device(self.vpred)
+        self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
+        self.h.density(self.psi_b, self.rho)
+        self.h.update_potential(self.rho, self.vpsl, vout=self.vnew)
+        P.net.residual(self.vpred, self.vcur, self.vnew, self.resid)
+        r = float(self.resid.item())  # synchronises the step
+        self.attempted += 1
+        self.resid_log.append(r)
+        if r <= P.tol:
+            self.accepted += 1
+            self.vmid.copy_(self.vpred)
+            self.vcur, self.vnew = self.vnew, self.vcur
+            return True
+        if self.fallback == "reuse":
+            # the attempt serves as the predictor: V = (V(t) + V[rho'])/2, then the corrector
+            self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)
+        else:
+            # the plain predictor: a Taylor step with V(t)
+            self.h.set_vlocal_device(self.vcur)
+            self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
+            self.h.density(self.psi_b, self.rho)
+            self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)
+        self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
+        self.h.density(self.psi_b, self.rho)
+        self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
+        return True
 
     def run(self, nt=None):
         nt = self.s.nt if nt is None else nt

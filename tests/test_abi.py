@@ -1,4 +1,4 @@
-"""The C-ABI library loads without a GPU and exports every function include/salmon_hip.h declares."""
+"""The C-ABI library loads without a GPU and exports every function include/*.h declares."""
 import ctypes
 import os
 import re
@@ -8,19 +8,31 @@ from salmon2_amd import hip
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared():
-    txt = open(os.path.join(ROOT, "include", "salmon_hip.h")).read()
+def declared(header="salmon_hip.h", prefix="salmon_hip_"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(salmon_hip_[a-z_0-9]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z_0-9]+)\s*\(", txt)))
+
+
+def all_declared():
+    return declared() + declared("salmon_unet.h", "salmon_unet_")
+
+
+def test_headers_are_all_covered():
+    assert sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")) == [
+        "salmon_hip.h", "salmon_unet.h"]
 
 
 def test_header_declares_the_binding_list():
+    from salmon2_amd import unet
+
     assert set(declared()) == set(hip.EXPORTS)
+    assert set(declared("salmon_unet.h", "salmon_unet_")) == set(unet.EXPORTS)
 
 
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(hip.lib_path())
-    missing = [n for n in declared() if not hasattr(lib, n)]
+    missing = [n for n in all_declared() if not hasattr(lib, n)]
     assert not missing, missing
 
 

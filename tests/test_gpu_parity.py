@@ -165,3 +165,38 @@ def test_rt_c1_full_run(torch_dev, oracle_lib):
     assert e_psi <= 1e-10
     # the induced current is a real signal, not round-off
     assert np.abs(cur_ref).max() > 1e-6
+
+
+@pytest.mark.parametrize("name", ["si64_32cube", "ragged_20x26x28", "c1"])
+def test_launch_shapes_keep_parity(torch_dev, oracle_lib, name):
+    """Every stencil launch shape the autotuner may pick gives the same hpsi and Taylor step."""
+    torch = torch_dev
+    s = CASES[name]()
+    print(f"[gpu] launch shapes case={name}", flush=True)
+    from salmon2_amd.hip import Handle
+
+    A = np.array([0.01, 0.02, -0.015])
+    h = Handle(s, stream=torch.cuda.current_stream())
+    h.update_kvector(A)
+    plane, nzt, ms = h.autotune(reps=2)
+    print(f"[gpu] autotune {name}: whole_plane={plane} z_chunks={nzt} {ms:.4f} ms", flush=True)
+    O = oracle_lib.Oracle(s)
+    O.update_kvector(A)
+    ref_h = O.hpsi(A, s.psi)
+    ref_t, _, _ = O.taylor(A, s.psi)
+    t = torch.from_numpy(s.psi).cuda()
+    for shape in [(1, 1), (1, 2), (0, 1), (0, 2), (0, 4)]:
+        if shape[0] == 1 and s.nx * s.ny > 1024:
+            continue
+        h._set_shape(*shape)
+        out = torch.zeros_like(t)
+        w = torch.zeros_like(t)
+        h.hpsi(t, out)
+        torch.cuda.synchronize()
+        e1 = rel(out.cpu().numpy()[..., : s.ny, :], ref_h[..., : s.ny, :])
+        h.taylor(s.zc(), t, out, w)
+        torch.cuda.synchronize()
+        e2 = rel(out.cpu().numpy()[..., : s.ny, :], ref_t[..., : s.ny, :])
+        _report(name, f"shape {shape} hpsi/taylor", max(e1, e2))
+        assert e1 <= HPSI_TOL and e2 <= HPSI_TOL
+    h.close()
