@@ -16,6 +16,13 @@ Forms timed on the same initial state (each for warmup + steps RT steps):
   learned-mixed  pc-mixed with the predictor Taylor step replaced by the learned
                midpoint potential (salmon2_amd/learned.py), accepted per step by its
                self-consistency residual, else the plain predictor-corrector step
+  learned-mixed-reuse  the same, but a rejected attempt serves as the predictor
+               (one corrector step instead of the plain predictor-corrector)
+  learned-mixed-open   every predicted step accepted (no gate): the speed the learned
+               predictor offers and the error it costs on this workload
+The batched form (salmon2_amd/batch.py) runs separately on 24 seeded uneven systems
+(system.batch_systems): graph-replayed batched steps against the same systems
+advanced one after another; reported under "batch".
 Each form's current, dipole and electron count are compared with pc-fp64's
 over the whole run.  `value` is the steps/s of the headline form.
 
@@ -42,9 +49,11 @@ FORMS = {
     "pc-fp64": dict(scf=True, pc=True, precision="fp64"),
     "pc-mixed": dict(scf=True, pc=True, precision="mixed"),
     "learned-mixed": dict(scf=True, pc=True, precision="mixed", learned=True),
+    "learned-mixed-reuse": dict(scf=True, pc=True, precision="mixed", learned=True, fallback="reuse"),
+    "learned-mixed-open": dict(scf=True, pc=True, precision="mixed", learned=True, tol=float("inf")),
 }
 BASE_FORM = "pc-fp64"
-HEADLINE_FORM = "pc-mixed"
+HEADLINE_FORM = "learned-mixed"
 
 
 def cpu_baseline(sysm, steps: int):
@@ -102,8 +111,10 @@ def run_form(s, form, steps, warmup, prof, predictor=None):
     from salmon2_amd.rt import RTPropagator
 
     kw = dict(FORMS[form])
+    tol = kw.pop("tol", None)
     if kw.pop("learned", False):
         kw["learned"] = predictor
+        predictor.tol_run = predictor.tol if tol is None else tol
     prop = RTPropagator(s, **kw)
     prop.setup_outputs(warmup + steps)
     for _ in range(warmup):
@@ -136,6 +147,64 @@ def run_form(s, form, steps, warmup, prof, predictor=None):
     return dict(elapsed=t1 - t0, kern=kern, **res)
 
 
+def run_batch(steps, warmup, log):
+    """Graph-replayed batched steps of the seeded uneven systems, and the same systems one by one."""
+    import numpy as np
+    import torch
+
+    from salmon2_amd import system
+    from salmon2_amd.batch import BatchRunner
+    from salmon2_amd.rt import RTPropagator
+
+    systems = system.batch_systems(nt=warmup + steps)
+    upd = sum(s.nx * s.ny * s.nz * s.norb for s in systems)
+    br = BatchRunner(systems, warmup + steps)
+    choice, tune_ms = br.batch.autotune(reps=2)
+    br.reset()
+    br.run(warmup, graph=False)
+    br.batch.capture()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    br.batch.replay(steps // 2)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nb = 2 * (steps // 2)
+    res = br.results(warmup + nb)
+    bad = [i for i, r in enumerate(res) if not np.all(np.isfinite(r["curr"]))]
+    assert not bad, f"non-finite current in batched systems {bad}"
+    del br
+    torch.cuda.empty_cache()
+    # the same systems advanced one after another (single-system path, eager)
+    seq_steps = max(2, steps // 10)
+    tseq = 0.0
+    for s in systems:
+        p = RTPropagator(s)
+        p.setup_outputs(seq_steps + 1)
+        p.step()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(seq_steps):
+            p.step()
+        torch.cuda.synchronize()
+        tseq += time.perf_counter() - a
+        p.h.close()
+        del p
+    torch.cuda.empty_cache()
+    log(f"[bench] batch: {nb / (t1 - t0):.2f} batched steps/s, one-by-one {seq_steps / tseq:.2f}")
+    return {
+        "systems": len(systems),
+        "grid_point_orbital_updates_per_step": upd,
+        "steps": nb,
+        "batched_steps_per_s": nb / (t1 - t0),
+        "grid_point_orbital_updates_per_s": upd * nb / (t1 - t0),
+        "one_by_one_steps_per_s": seq_steps / tseq,
+        "speedup_vs_one_by_one": (nb / (t1 - t0)) / (seq_steps / tseq),
+        "autotune_choice": choice,
+        "form": "fixed-potential FP64 step; one persistent stencil kernel per halo class and Taylor term, "
+                "two steps per hipGraph replay",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +213,7 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--forms", default=",".join(FORMS))
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--batch-steps", type=int, default=200, help="batched-form steps (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the run, copy 1 GiB device-to-device (HBM-traffic counter calibration)")
@@ -189,8 +259,8 @@ def main():
         }
         if "accepted" in r:
             form_out[f]["predictor"] = {"accepted_steps": r["accepted"], "attempted_steps": r["attempted"],
-                                        "median_residual": r["resid_median"], "tol": predictor.tol,
-                                        "weights": pred_note}
+                                        "median_residual": r["resid_median"],
+                                        "tol": FORMS[f].get("tol", predictor.tol), "weights": pred_note}
         if f != BASE_FORM and FORMS[f]["scf"]:
             form_out[f]["rel_err_vs_" + BASE_FORM] = {
                 "current": rel(r["curr"], base["curr"]),
@@ -232,7 +302,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.config, "k_stencil<1, -1>") if s.nx <= 32 else None,
+            "traffic": measured_traffic(args.config, "k_stencil<d2,double,d2,1,-1,256>"),
             "alg_bytes_per_launch": alg_bytes,
             "avg_launch_ms": kern["hpsi_taylor"]["avg_ms"],
         },
@@ -240,6 +310,8 @@ def main():
     }
     runs = None
     torch.cuda.empty_cache()
+    if args.batch_steps > 0:
+        out["batch"] = run_batch(args.batch_steps, 4, lambda m: print(m, file=sys.stderr, flush=True))
     if args.calib_copy:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)

@@ -150,6 +150,11 @@ int salmon_hip_autotune(salmon_hip_t h, int reps, int *choice, double *best_ms);
 /* Set the launch shape directly: whole_plane -1 (rule) / 0 / 1, z_chunks 0 (rule) or a count. */
 int salmon_hip_set_launch_shape(salmon_hip_t h, int whole_plane, int z_chunks);
 
+/* Run the KB projections on a second stream of the handle, concurrently with the stencil
+   pass that reads the same orbitals (joined by events before the scatter).  Off by default:
+   at c2 the two passes share HBM and the step gains about 1%. */
+int salmon_hip_set_overlap(salmon_hip_t h, int on);
+
 /*
  * rho(ix,iy,iz,ispin) = sum_{ik,io} rocc*wtk*|psi|^2 into a DEVICE array
  * (calc_density, src/common/density_matrix.f90:26-236).

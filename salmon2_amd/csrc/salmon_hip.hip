@@ -684,11 +684,17 @@ __global__ __launch_bounds__(BS) void k_stencil(StencilArgs a) {
 // Persistent form over many systems (the batched step): block b takes tiles b, b + grid, ...
 // of `tiles` = (system index, tile index); all systems share the template shape.
 template <typename TI, typename R, typename TO, int HC, int NC, int BS>
-__global__ __launch_bounds__(BS) void k_stencil_packed(const StencilArgs *sys, const int2 *tiles, int ntiles) {
+__global__ __launch_bounds__(BS) void k_stencil_packed(const StencilArgs *__restrict__ sys,
+                                                       const int2 *__restrict__ tiles, int ntiles) {
   extern __shared__ double2 lds[];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int2 tr = tiles[t];
-    stencil_tile<TI, R, TO, HC, NC, BS>(sys[tr.x], tr.y, lds);
+    // the tile entry is the same for the whole block: uniform indices let the descriptor come in
+    // through scalar loads, once per tile
+    const int si = __builtin_amdgcn_readfirstlane(tr.x);
+    const int bid = __builtin_amdgcn_readfirstlane(tr.y);
+    const StencilArgs a = sys[si];
+    stencil_tile<TI, R, TO, HC, NC, BS>(a, bid, lds);
     __syncthreads();  // every wave is done with the LDS rings before the next tile's DMAs
   }
 }
@@ -1191,6 +1197,7 @@ struct salmon_hip_ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool join_pending = false;
+  bool overlap = false;  // salmon_hip_set_overlap
   bool have_V = false, have_k = false;
   double *d_veck = nullptr, *d_rocc = nullptr, *d_wtk = nullptr, *d_V = nullptr, *d_kpar = nullptr;
   // nonlocal
@@ -1548,19 +1555,25 @@ static int launch_proj(salmon_hip_t c, const void *psi, int nw, bool f32 = false
   dim3 grid((unsigned)(a.ngrp * c->natom * c->nk));
   hipEvent_t e0, e1;
   prof_events(c, nw == 1 ? SALMON_HIP_KERNEL_PROJ : SALMON_HIP_KERNEL_PROJ_CURRENT, &e0, &e1);
-  // fork: everything enqueued on `stream` so far (the producer of psi, the last reader of
-  // the projection buffer) precedes the projection
-  HIPCHK(hipEventRecord(c->ev_fork, c->stream));
-  HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  // with overlap: fork onto the side stream after everything enqueued on `stream` so far (the
+  // producer of psi, the last reader of the projection buffer); joined before the scatter
+  hipStream_t st = c->stream;
+  if (c->overlap) {
+    HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    st = c->side;
+  }
   if (nw == 1 && f32)
-    hipExtLaunchKernelGGL((k_proj<1, float2>), grid, dim3(kBlock), 0, c->side, e0, e1, 0, a);
+    hipExtLaunchKernelGGL((k_proj<1, float2>), grid, dim3(kBlock), 0, st, e0, e1, 0, a);
   else if (nw == 1)
-    hipExtLaunchKernelGGL((k_proj<1, double2>), grid, dim3(kBlock), 0, c->side, e0, e1, 0, a);
+    hipExtLaunchKernelGGL((k_proj<1, double2>), grid, dim3(kBlock), 0, st, e0, e1, 0, a);
   else
-    hipExtLaunchKernelGGL((k_proj<4, double2>), grid, dim3(kBlock), 0, c->side, e0, e1, 0, a);
+    hipExtLaunchKernelGGL((k_proj<4, double2>), grid, dim3(kBlock), 0, st, e0, e1, 0, a);
   LAUNCHCHK("k_proj");
-  HIPCHK(hipEventRecord(c->ev_join, c->side));
-  c->join_pending = true;
+  if (c->overlap) {
+    HIPCHK(hipEventRecord(c->ev_join, c->side));
+    c->join_pending = true;
+  }
   return 0;
 }
 
@@ -1872,6 +1885,13 @@ int salmon_hip_autotune(salmon_hip_t c, int reps, int *choice, double *best_ms) 
   c->mid_nzt = cands[besti].nzt;
   if (choice) *choice = 10 * cands[besti].plane + cands[besti].nzt;
   if (best_ms) *best_ms = best;
+  return 0;
+}
+
+int salmon_hip_set_overlap(salmon_hip_t c, int on) {
+  if (int rc = check(c)) return rc;
+  if (int rc = join_proj(c)) return rc;
+  c->overlap = on != 0;
   return 0;
 }
 

@@ -32,6 +32,7 @@ EXPORTS = (
     "salmon_hip_set_precision",
     "salmon_hip_autotune",
     "salmon_hip_set_launch_shape",
+    "salmon_hip_set_overlap",
     "salmon_hip_batch_create",
     "salmon_hip_batch_destroy",
     "salmon_hip_batch_set_system",
@@ -96,6 +97,7 @@ def load():
     L.salmon_hip_set_precision.argtypes = [C.c_void_p, C.c_int]
     L.salmon_hip_autotune.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
     L.salmon_hip_set_launch_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.salmon_hip_set_overlap.argtypes = [C.c_void_p, C.c_int]
     vp = C.c_void_p
     L.salmon_hip_batch_create.argtypes = [C.POINTER(vp), C.c_int, vp, C.POINTER(vp)]
     L.salmon_hip_batch_destroy.argtypes = [vp]
@@ -222,6 +224,9 @@ class Handle:
         ms = C.c_double()
         _chk(load().salmon_hip_autotune(self.h, int(reps), C.byref(ch), C.byref(ms)), "autotune")
         return ch.value // 10, ch.value % 10, ms.value
+
+    def set_overlap(self, on=True):
+        _chk(load().salmon_hip_set_overlap(self.h, int(bool(on))), "set_overlap")
 
     def _set_shape(self, whole_plane, z_chunks):
         _chk(load().salmon_hip_set_launch_shape(self.h, int(whole_plane), int(z_chunks)), "set_launch_shape")

@@ -151,7 +151,7 @@ class RTPropagator:
         r = float(self.resid.item())  # synchronises the step
         self.attempted += 1
         self.resid_log.append(r)
-        if r <= P.tol:
+        if r <= getattr(P, "tol_run", P.tol):
             self.accepted += 1
             self.vmid.copy_(self.vpred)
             self.vcur, self.vnew = self.vnew, self.vcur

@@ -453,3 +453,30 @@ def config(name: str, **over) -> System:
     kw.update(over)
     kw.setdefault("name", name)
     return make_si_system(**kw)
+
+
+def batch_systems(n=24, seed=20261205, nt=100):
+    """Seeded uneven systems of the batched workload: Si supercells of 8..128 atoms, grid extents
+    16..40 (step 4) chosen per axis so that the spacing stays in [0.55, 1.1] bohr (the 4th-order
+    Taylor step with dt = 0.08 is stable for |dt E| < 2 sqrt 2, E_max ~ 9.75/h^2 for random
+    orbitals), 16..256 orbitals, 1..27 k-points (orbitals x k-points capped at 512 to bound the
+    host-side orbital generation and memory)."""
+    a = 10.26
+    rng = np.random.default_rng(seed)
+    cells = [(1, 1, 1), (1, 1, 2), (2, 2, 1), (2, 2, 2), (2, 2, 4)]
+    kgrids = [(1, 1, 1), (2, 1, 1), (2, 2, 1), (2, 2, 2), (3, 3, 3)]
+    out = []
+    for i in range(n):
+        cell = cells[rng.integers(len(cells))]
+        grid = []
+        for m in cell:
+            cand = [g for g in range(16, 44, 4) if 0.55 <= a * m / g <= 1.1]
+            grid.append(int(rng.choice(cand)))
+        kg = kgrids[rng.integers(len(kgrids))]
+        nk = int(np.prod(kg))
+        no = int(rng.choice([16, 32, 64, 128, 256]))
+        while no * nk > 512 and no > 16:
+            no //= 2
+        out.append(make_si_system(ncell=cell, n=tuple(grid), no=no, nkgrid=kg, seed=seed + 1 + i, nt=nt,
+                                  name=f"batch{i}"))
+    return out

@@ -185,10 +185,11 @@ def test_launch_shapes_keep_parity(torch_dev, oracle_lib, name):
     ref_h = O.hpsi(A, s.psi)
     ref_t, _, _ = O.taylor(A, s.psi)
     t = torch.from_numpy(s.psi).cuda()
-    for shape in [(1, 1), (1, 2), (0, 1), (0, 2), (0, 4)]:
+    for shape in [(1, 1), (1, 2), (0, 1), (0, 2), (0, 4), (-1, 0, "overlap")]:
         if shape[0] == 1 and s.nx * s.ny > 1024:
             continue
-        h._set_shape(*shape)
+        h._set_shape(shape[0], shape[1])
+        h.set_overlap(len(shape) > 2)
         out = torch.zeros_like(t)
         w = torch.zeros_like(t)
         h.hpsi(t, out)
