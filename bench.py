@@ -24,7 +24,8 @@ The batched form (salmon2_amd/batch.py) runs separately on 24 seeded uneven syst
 (system.batch_systems): graph-replayed batched steps against the same systems
 advanced one after another; reported under "batch".
 Each form's current, dipole and electron count are compared with pc-fp64's
-over the whole run.  `value` is the steps/s of the headline form.
+over the whole run.  `value` is the steps/s of the fastest learned+mixed form whose
+current stays within 1e-6 (relative, max over the run) of pc-fp64's.
 
 Prints ONE JSON line.  The roofline entry is for the dominant kernel, the
 stencil of the middle Taylor terms (k_stencil) in pc-fp64, timed live with
@@ -53,7 +54,9 @@ FORMS = {
     "learned-mixed-open": dict(scf=True, pc=True, precision="mixed", learned=True, tol=float("inf")),
 }
 BASE_FORM = "pc-fp64"
-HEADLINE_FORM = "learned-mixed"
+# `value` is the fastest learned+mixed form whose current stays within CUR_BAR of pc-fp64's
+HEADLINE_FORMS = ("learned-mixed", "learned-mixed-reuse", "learned-mixed-open")
+CUR_BAR = 1e-6
 
 
 def cpu_baseline(sysm, steps: int):
@@ -267,7 +270,8 @@ def main():
                 "dipole": rel(r["dip"], base["dip"]),
                 "electron_count": float(np.abs(r["rne"] - base["rne"]).max() / base["rne"][0]),
             }
-    head = HEADLINE_FORM if HEADLINE_FORM in runs else BASE_FORM
+    ok = [f for f in HEADLINE_FORMS if f in form_out and form_out[f]["rel_err_vs_" + BASE_FORM]["current"] <= CUR_BAR]
+    head = max(ok, key=lambda f: form_out[f]["steps_per_s"]) if ok else BASE_FORM
     kern = base["kern"]
     # algorithmic bytes of one middle-term launch: read H^(n-1) psi (16 B) and write H^n psi (16 B)
     # per point-orbital, plus V_local once (8 B per point)
@@ -291,6 +295,7 @@ def main():
             "workload": f"{args.config}: Si {s.natom} atoms, {s.nx}x{s.ny}x{s.nz} grid, {s.no} orbitals x {s.nk} k, "
                         f"{s.nlma} KB projectors, Taylor order {s.n_hamil}, dt {s.dt}",
             "form": head,
+            "form_rule": f"fastest of {list(HEADLINE_FORMS)} with current within {CUR_BAR:g} of {BASE_FORM}",
             "baseline_form": BASE_FORM,
             "grid_point_orbital_updates_per_s": form_out[head]["grid_point_orbital_updates_per_s"],
         },

@@ -370,6 +370,9 @@ constexpr int kMaxComb = 5;   // Taylor order up to 6
 // NC >= 0: hout = (((psi + c1 h1) + ... + c_{N-1} in) + c_N H_loc in), N = NC + 1
 // ---------------------------------------------------------------------------
 constexpr int kDmaDepth = 2;
+// The per-plane wait lets the DMAs issued in the previous iteration stay in flight: the data a
+// plane needs must have been issued at least two iterations earlier.
+static_assert(kDmaDepth >= 2, "the ring wait assumes a look-ahead of at least two planes");
 constexpr int kRingT = 5 + kDmaDepth;
 constexpr int kRingH = kDmaDepth + 1;
 constexpr int kTile = 256;  // tile slot size (elements), = 4 waves x 64 lanes
@@ -452,8 +455,13 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs &a, int bid, doub
   const int nx = a.nx, ny = a.ny, nz = a.nz, ly = a.ly;
   const int TY = a.TY;
   const int y0 = ytile * TY;
-  const int z0 = zt * a.TZ;
-  const int z1 = min(z0 + a.TZ, nz);
+  // With one z chunk the column is a full period and may start anywhere: y-tile t starts 4t planes
+  // earlier, so that its upper y-halo rows of plane p are fetched in the same iteration as the
+  // next tile's own rows of p (4 planes ahead of their use) and come from the L2.  Planes are
+  // counted unwrapped in [z0, z1) and wrapped on every access.
+  const int zph = (HC > 0 && a.nzt == 1) ? (nz - (4 * ytile) % nz) % nz : 0;
+  const int z0 = zt * a.TZ + zph;
+  const int z1 = min(zt * a.TZ + a.TZ, nz) + zph;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int ntile = TY * nx;
@@ -595,10 +603,12 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs &a, int bid, doub
     __syncthreads();
     const int sTn = (sT + 4 + D) % kRingT;  // slot of plane z+4+D (= slot of z-1)
     const int sHn = (sH + D) % kRingH;      // slot of plane z+D   (= slot of z-1)
-    if (FINAL) dma_comb(z + 1, (z + 1) & 1);
-    dma_tile(z + 4 + D, sTn);
-    dma_halo(z + D, sHn);
-    dma_v(z + D, sHn);
+    // past the chunk's last needed plane the look-ahead re-reads that plane (an L2 hit) into the
+    // slot, which is never read: every wave still issues the same memory operations per plane
+    if (FINAL) dma_comb(min(z + 1, z1 - 1), (z + 1) & 1);
+    dma_tile(min(z + 4 + D, z1 + 3), sTn);
+    dma_halo(min(z + D, z1 - 1), sHn);
+    dma_v(min(z + D, z1 - 1), sHn);
 
     const TI *T = tileR + BS * sT;
     const TI *H = haloR + HC * BS * sH;
@@ -662,7 +672,7 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs &a, int bid, doub
     }
     // the store is the one compiler-visible vm op per plane (counted in kOpsAfterComb):
     // every lane stores, lanes without a point into the sink, so each wave issues it
-    TO *dst = outY ? (TO *)a.hout + orbOff + colOff + zs * z : (TO *)a.sink + tid;
+    TO *dst = outY ? (TO *)a.hout + orbOff + colOff + zs * wrapi(z, nz) : (TO *)a.sink + tid;
     *dst = o;
 #pragma unroll
     for (int i = 0; i < 8; ++i) q[i] = q[i + 1];
@@ -672,6 +682,173 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs &a, int bid, doub
   wait_vm<0>();  // drain the look-ahead DMAs before the LDS is released
 }
 
+// ---------------------------------------------------------------------------
+// Whole-plane form of the terms without the Taylor sum, for planes of up to PY * BS
+// points: lane `tid` computes the plane elements tid + k * BS (k < PY), so a 32 x 32
+// plane takes 512 threads with two points each.  There is no y halo (y wraps inside
+// the plane held in LDS), so every orbital plane is read once, plus the 4 + 4 wrap
+// planes of the z period.  LDS: tile ring (5 + D) x PY*BS elements, V ring (D + 1) x
+// PY*BS doubles, D = look-ahead in planes (D = 2: 136 KB for 32 x 32 in FP64, one block of
+// 8 waves per CU, two waves per SIMD, which leaves 256 registers per lane for the two z
+// queues).  The wait at the top of a plane lets the DMAs of the last D - 1 planes stay in
+// flight.  The DMA issue discipline is the one of stencil_tile; lanes without a point
+// store into the sink.
+// ---------------------------------------------------------------------------
+template <typename TI, typename R, typename TO, int PY, int BS, int D>
+__device__ __forceinline__ void stencil_plane(const StencilArgs &a, int bid, double2 *lds) {
+  static_assert(PY * BS <= 1024, "the sink holds 1024 elements");
+  static_assert(D >= 2, "the ring wait assumes a look-ahead of at least two planes");
+  using CH = typename Cx<R>::T;
+  constexpr int kRingT = 5 + D, kRingH = D + 1;
+  constexpr int WI = DmaW<TI>::W;
+  constexpr unsigned EB = sizeof(TI);
+  constexpr int SL = PY * BS;  // elements per ring slot
+  // memory instructions per wave per plane, in issue order: tile DMAs (PY x WI), V DMAs (PY x 2), PY stores
+  constexpr int kOps = PY * WI + 2 * PY + PY;
+  TI *tileR = (TI *)lds;                       // kRingT x SL
+  double *vR = (double *)(tileR + kRingT * SL);  // kRingH x SL
+
+  const int zt = bid % a.nzt;
+  const int b = bid / a.nzt;
+  const int ik = b / a.nb;
+  const int ispin = b % a.nspin;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, ly = a.ly;
+  const int z0 = zt * a.TZ;
+  const int z1 = min(z0 + a.TZ, nz);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int ntile = nx * ny;
+  const size_t zs = (size_t)nx * ly;
+  const size_t orbOff = (size_t)b * a.osz;
+  const TI *in = (const TI *)a.in + orbOff;
+  const size_t gplane = (size_t)nx * ny;
+  // in-plane offset x + nx y of element e (elements past the plane repeat element 0)
+  auto pofs = [&](int e) {
+    const int ec = e < ntile ? e : 0;
+    return (size_t)(ec % nx) + (size_t)nx * (ec / nx);
+  };
+
+  int te[PY], xo[PY][8], yo[PY][8];
+  bool act[PY];
+  size_t col[PY], toff[PY][2], voff[PY][2];
+#pragma unroll
+  for (int k = 0; k < PY; ++k) {
+    const int e = tid + k * BS;
+    act[k] = e < ntile;
+    te[k] = act[k] ? e : 0;
+    const int tx = te[k] % nx, ty = te[k] / nx;
+    col[k] = (size_t)tx + (size_t)nx * ty;
+#pragma unroll
+    for (int n = 1; n <= 4; ++n) {
+      xo[k][n - 1] = ty * nx + wrapi(tx + n, nx);
+      xo[k][3 + n] = ty * nx + wrapi(tx - n, nx);
+      yo[k][n - 1] = wrapi(ty + n, ny) * nx + tx;
+      yo[k][3 + n] = wrapi(ty - n, ny) * nx + tx;
+    }
+    // DMA sources: W = 1 the lane's own point; W = 2 (and V) points 32 kk + lane/2 of the wave's chunk
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const size_t p = pofs(64 * wave + k * BS + 32 * kk + (lane >> 1));
+      toff[k][kk] = WI == 1 ? col[k] : p;
+      voff[k][kk] = 2 * p + (lane & 1);
+    }
+  }
+  const double *Vb = a.V + (size_t)ispin * gplane * nz;
+  const unsigned tileA = __builtin_amdgcn_readfirstlane(lds_addr(tileR) + EB * 64u * wave);
+  const unsigned vA = __builtin_amdgcn_readfirstlane(lds_addr(vR) + 8u * 64u * wave);
+  auto dma_tile = [&](int plane, int slot) {
+    const TI *src = in + zs * wrapi(plane, nz);
+#pragma unroll
+    for (int k = 0; k < PY; ++k) dma_chunk<TI>(src, toff[k], tileA + EB * (unsigned)(SL * slot + k * BS), lane);
+  };
+  auto dma_v = [&](int plane, int slot) {
+    const float *vb = (const float *)Vb + 2 * gplane * wrapi(plane, nz);
+#pragma unroll
+    for (int k = 0; k < PY; ++k) {
+      const unsigned m0 = vA + 8u * (unsigned)(SL * slot + k * BS);
+      dma4(vb + voff[k][0], m0);
+      dma4(vb + voff[k][1], m0 + 256u);
+    }
+  };
+  const double *kp = a.kpar + 16 * ik;
+  const R klap0 = (R)kp[0];
+  R nab[12], lap[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    nab[i] = (R)kp[1 + i];
+    lap[i] = (R)a.lap[i];
+  }
+
+  // prologue: queues of planes z0-4 .. z0+3 (plain loads), rings for planes z0 .. z0+3+D
+  CH q[PY][9];
+#pragma unroll
+  for (int k = 0; k < PY; ++k)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) q[k][i] = cvt2<CH>(in[col[k] + zs * wrapi(z0 - 4 + i, nz)]);
+  __builtin_amdgcn_s_waitcnt(0);
+  for (int p = 0; p < 4 + D; ++p) dma_tile(z0 + p, (z0 + p) % kRingT);
+  for (int p = 0; p < D; ++p) dma_v(z0 + p, (z0 + p) % kRingH);
+  wait_vm<0>();
+
+  int sT = z0 % kRingT, sH = z0 % kRingH;
+  for (int z = z0; z < z1; ++z) {
+    wait_vm<(D - 1) * kOps>();
+    __syncthreads();
+    // past the chunk's last needed plane the look-ahead re-reads that plane into a slot never read
+    dma_tile(min(z + 4 + D, z1 + 3), (sT + 4 + D) % kRingT);
+    dma_v(min(z + D, z1 - 1), (sH + D) % kRingH);
+    const TI *T = tileR + SL * sT;
+    const TI *T4 = tileR + SL * ((sT + 4) % kRingT);
+    const double *Vs = vR + SL * sH;
+#pragma unroll
+    for (int k = 0; k < PY; ++k) {
+      q[k][8] = cvt2<CH>(T4[te[k]]);
+      CH v, w;
+      {
+        const CH p1 = cvt2<CH>(T[xo[k][0]]), p2 = cvt2<CH>(T[xo[k][1]]), p3 = cvt2<CH>(T[xo[k][2]]),
+                 p4 = cvt2<CH>(T[xo[k][3]]);
+        const CH m1 = cvt2<CH>(T[xo[k][4]]), m2 = cvt2<CH>(T[xo[k][5]]), m3 = cvt2<CH>(T[xo[k][6]]),
+                 m4 = cvt2<CH>(T[xo[k][7]]);
+        v.x = lap[0] * (p1.x + m1.x) + lap[1] * (p2.x + m2.x) + lap[2] * (p3.x + m3.x) + lap[3] * (p4.x + m4.x);
+        v.y = lap[0] * (p1.y + m1.y) + lap[1] * (p2.y + m2.y) + lap[2] * (p3.y + m3.y) + lap[3] * (p4.y + m4.y);
+        w.x = nab[0] * (p1.x - m1.x) + nab[1] * (p2.x - m2.x) + nab[2] * (p3.x - m3.x) + nab[3] * (p4.x - m4.x);
+        w.y = nab[0] * (p1.y - m1.y) + nab[1] * (p2.y - m2.y) + nab[2] * (p3.y - m3.y) + nab[3] * (p4.y - m4.y);
+      }
+      {
+        const CH p1 = cvt2<CH>(T[yo[k][0]]), p2 = cvt2<CH>(T[yo[k][1]]), p3 = cvt2<CH>(T[yo[k][2]]),
+                 p4 = cvt2<CH>(T[yo[k][3]]);
+        const CH m1 = cvt2<CH>(T[yo[k][4]]), m2 = cvt2<CH>(T[yo[k][5]]), m3 = cvt2<CH>(T[yo[k][6]]),
+                 m4 = cvt2<CH>(T[yo[k][7]]);
+        v.x += lap[4] * (p1.x + m1.x) + lap[5] * (p2.x + m2.x) + lap[6] * (p3.x + m3.x) + lap[7] * (p4.x + m4.x);
+        v.y += lap[4] * (p1.y + m1.y) + lap[5] * (p2.y + m2.y) + lap[6] * (p3.y + m3.y) + lap[7] * (p4.y + m4.y);
+        w.x += nab[4] * (p1.x - m1.x) + nab[5] * (p2.x - m2.x) + nab[6] * (p3.x - m3.x) + nab[7] * (p4.x - m4.x);
+        w.y += nab[4] * (p1.y - m1.y) + nab[5] * (p2.y - m2.y) + nab[6] * (p3.y - m3.y) + nab[7] * (p4.y - m4.y);
+      }
+      const CH *qq = q[k];
+      v.x += lap[8] * (qq[5].x + qq[3].x) + lap[9] * (qq[6].x + qq[2].x) + lap[10] * (qq[7].x + qq[1].x) +
+             lap[11] * (qq[8].x + qq[0].x);
+      v.y += lap[8] * (qq[5].y + qq[3].y) + lap[9] * (qq[6].y + qq[2].y) + lap[10] * (qq[7].y + qq[1].y) +
+             lap[11] * (qq[8].y + qq[0].y);
+      w.x += nab[8] * (qq[5].x - qq[3].x) + nab[9] * (qq[6].x - qq[2].x) + nab[10] * (qq[7].x - qq[1].x) +
+             nab[11] * (qq[8].x - qq[0].x);
+      w.y += nab[8] * (qq[5].y - qq[3].y) + nab[9] * (qq[6].y - qq[2].y) + nab[10] * (qq[7].y - qq[1].y) +
+             nab[11] * (qq[8].y - qq[0].y);
+      const CH t = qq[4];
+      const R vcur = (R)Vs[te[k]];
+      CH hh;
+      hh.x = (vcur * t.x + klap0 * t.x) - (R)0.5 * v.x + w.y;
+      hh.y = (vcur * t.y + klap0 * t.y) - (R)0.5 * v.y - w.x;
+      TO *dst = act[k] ? (TO *)a.hout + orbOff + col[k] + zs * z : (TO *)a.sink + tid + k * BS;
+      *dst = cvt2<TO>(hh);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) q[k][i] = q[k][i + 1];
+    }
+    sT = (sT + 1 == kRingT) ? 0 : sT + 1;
+    sH = (sH + 1 == kRingH) ? 0 : sH + 1;
+  }
+  wait_vm<0>();
+}
+
 template <typename TI, typename R, typename TO, int HC, int NC, int BS>
 __global__ __launch_bounds__(BS) void k_stencil(StencilArgs a) {
   extern __shared__ double2 lds[];
@@ -679,6 +856,15 @@ __global__ __launch_bounds__(BS) void k_stencil(StencilArgs a) {
   const int nfull = (a.nblk / 8) * 8;
   if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;  // XCD-aware tile order
   stencil_tile<TI, R, TO, HC, NC, BS>(a, bid, lds);
+}
+
+template <typename TI, typename R, typename TO, int PY, int BS, int D>
+__global__ __launch_bounds__(BS) void k_stencil_plane(StencilArgs a) {
+  extern __shared__ double2 lds[];
+  int bid = blockIdx.x;
+  const int nfull = (a.nblk / 8) * 8;
+  if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;  // XCD-aware tile order
+  stencil_plane<TI, R, TO, PY, BS, D>(a, bid, lds);
 }
 
 // Persistent form over many systems (the batched step): block b takes tiles b, b + grid, ...
@@ -1609,7 +1795,7 @@ static int launch_stencil_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEven
 // Stencil stages: plain FP64, and the three stages of the mixed-precision chain.
 enum StencilStage { kStagePlain = 0, kStageMixedFirst = 1, kStageMixedMiddle = 2, kStageMixedFinal = 3 };
 
-// Whole-plane tiles (no y halo) for the terms without the Taylor sum, BS >= nx * ny.
+// Whole-plane tiles (no y halo) for the terms without the Taylor sum, nx * ny <= BS = 256.
 template <int BS>
 static int launch_stencil_plane(salmon_hip_ctx *c, int stage, dim3 grid, hipEvent_t e0, hipEvent_t e1,
                                 const StencilArgs &a) {
@@ -1618,6 +1804,31 @@ static int launch_stencil_plane(salmon_hip_ctx *c, int stage, dim3 grid, hipEven
   return launch_stencil_t<double2, double, double2, 0, -1, BS>(c, grid, e0, e1, a);
 }
 
+
+// Whole planes of 256 < nx*ny <= 1024 points: 512 threads with PY points each (stencil_plane).
+constexpr int kPlaneDepth = 2;  // look-ahead of the whole-plane kernel (planes; 3 measured no faster)
+
+template <typename TI, typename R, typename TO, int PY>
+static int launch_plane_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
+  constexpr int BS = 512, D = kPlaneDepth;
+  const size_t shmem = sizeof(TI) * (5 + D) * PY * BS + sizeof(double) * (D + 1) * PY * BS;
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencil_plane<TI, R, TO, PY, BS, D>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    attr_set = true;
+  }
+  hipExtLaunchKernelGGL((k_stencil_plane<TI, R, TO, PY, BS, D>), grid, dim3(BS), shmem, c->stream, e0, e1, 0, a);
+  return 0;
+}
+
+template <int PY>
+static int launch_stencil_wp(salmon_hip_ctx *c, int stage, dim3 grid, hipEvent_t e0, hipEvent_t e1,
+                             const StencilArgs &a) {
+  if (stage == kStageMixedFirst) return launch_plane_t<double2, float, float2, PY>(c, grid, e0, e1, a);
+  if (stage == kStageMixedMiddle) return launch_plane_t<float2, float, float2, PY>(c, grid, e0, e1, a);
+  return launch_plane_t<double2, double, double2, PY>(c, grid, e0, e1, a);
+}
 
 template <int HC>
 static int launch_stencil_hc(salmon_hip_ctx *c, int stage, int nc, dim3 grid, hipEvent_t e0, hipEvent_t e1,
@@ -1731,8 +1942,8 @@ static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, boo
   const int hc = (8 * c->nx + kTile - 1) / kTile;
   int rc = 0;
   if (bs == 256) rc = launch_stencil_plane<256>(c, stage, grid, e0, e1, a);
-  else if (bs == 512) rc = launch_stencil_plane<512>(c, stage, grid, e0, e1, a);
-  else if (bs == 1024) rc = launch_stencil_plane<1024>(c, stage, grid, e0, e1, a);
+  else if (bs == 512) rc = launch_stencil_wp<1>(c, stage, grid, e0, e1, a);
+  else if (bs == 1024) rc = launch_stencil_wp<2>(c, stage, grid, e0, e1, a);
   else if (hc <= 1) rc = launch_stencil_hc<1>(c, stage, nc, grid, e0, e1, a);
   else if (hc <= 2) rc = launch_stencil_hc<2>(c, stage, nc, grid, e0, e1, a);
   else if (hc <= 4) rc = launch_stencil_hc<4>(c, stage, nc, grid, e0, e1, a);
