@@ -249,7 +249,14 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
   const TP *psi = (const TP *)a.psi;
   const int *jx = a.jidx + (size_t)a.nps * ia;
   const double2 *zk = a.zekr + (size_t)a.nps * ((size_t)al.x + (size_t)a.nlma * ik);
+  // every thread loads the same sphere point of each chunk (kBlock is a multiple of kProjJC);
+  // its grid index is fetched one chunk ahead, so the psi gathers wait on one load, not two
+  static_assert(kBlock % kProjJC == 0, "one sphere point per thread and chunk");
+  const int jt = threadIdx.x % kProjJC;
+  int jnext = jt < m ? jx[jt] : 0;
   auto load_chunk = [&](int j0) {
+    const int jv = jnext;
+    jnext = j0 + kProjJC + jt < m ? jx[j0 + kProjJC + jt] : 0;
 #pragma unroll
     for (int r = 0; r < kZPer; ++r) {
       const int e = threadIdx.x + kBlock * r;  // (l, jj) = (e / kProjJC, e % kProjJC)
@@ -271,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
       TP v;
       v.x = 0;
       v.y = 0;
-      if (j < m && ob < a.nb) v = psi[(kbase + ob) * a.osz + jx[j]];
+      if (j < m && ob < a.nb) v = psi[(kbase + ob) * a.osz + jv];
       pre[r] = v;
     }
   };
@@ -427,7 +434,8 @@ struct StencilArgs {
 // products; TO: element type of `hout`.  Plain FP64: (double2, double, double2).
 // Mixed form: (double2, float, float2) for the first term, (float2, float, float2)
 // for the middle ones and (float2, float, double2) for the last, whose Taylor sum
-// is formed in FP64 from psi (FP64) and h_1 .. h_N (FP32).
+// is formed in FP64 from psi (FP64) and h_1 .. h_N (FP32).  (FP64 arithmetic in the
+// first term, rounded once at the store, measured 10% slower at 32^3.)
 // One tile (orbital block, y-tile, z-chunk) = `bid` of the system described by `a`.
 template <typename TI, typename R, typename TO, int HC, int NC, int BS>
 __device__ __forceinline__ void stencil_tile(const StencilArgs &a, int bid, double2 *lds) {
@@ -891,63 +899,92 @@ __global__ __launch_bounds__(BS) void k_stencil_packed(const StencilArgs *__rest
 // the point -> (atom, j) inverse map, entries in ascending ilma:
 //   hout(p) += s * sum_e sum_l c(ilma0_e + l) zekr(j_e, ilma0_e + l)
 // with s = 1 (plain / intermediate terms) or s = c_N (last Taylor term).
-// One thread per touched point and kScatG orbitals of one k-point: the entry
-// tables are read once and reused, the orbitals' read-modify-writes overlap.
+// One thread per touched point and G orbitals of one k-point.  The kernel is
+// bound by the latency of its dependent loads, so the per-point record
+// (orbital-block offset, first entry, entry count, first entry's ilma0 | cnt)
+// comes in one 16-byte load, and with CM > 0 the CM projectors of an entry are
+// loaded together (clamped to the entry's own, unused ones weighted by zero are
+// skipped in the sum, so the order of the additions is the sequential one).
 // ---------------------------------------------------------------------------
-constexpr int kScatG = 8;
+// Orbitals per thread and projectors per entry loaded together (entries with cmax <= CM), per
+// element type of the chain: FP64 <8, 4>, FP32 <4, 0> (the fastest of G in {4, 8}, CM in {0, 4}
+// measured at 32^3 with 64 s+p atoms).
+constexpr int kScatG = 8, kScatCM = 4;
+constexpr int kScatG32 = 4, kScatCM32 = 0;
 
 struct ScatterArgs {
   void *hout;             // double2, or float2 in the mixed-precision chain
   const double2 *c;       // (nb, nlma, nk), orbital fastest
   const double2 *zent;    // (cmax, nent, nk)
-  const int *tp_g;        // (ntp) unpadded index of each touched point
-  const int *tp_off;      // (ntp) orbital-block offset of each touched point
-  const int *pt_off;      // (ngrid + 1)
+  const int4 *tp_info;    // (ntp) orbital-block offset, first entry, entry count, pt_lma of the first entry
   const int *pt_lma;      // (nent) ilma0 | cnt << 24
   double2 s;
   size_t osz;
   int ntp, nb, nlma, nent, cmax, ngrp;
 };
 
-template <typename TH>
+template <typename TH, int G, int CM>
 __global__ __launch_bounds__(kBlock) void k_nl_scatter(ScatterArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   const int ik = blockIdx.y / a.ngrp;
-  const int b0 = (blockIdx.y % a.ngrp) * kScatG;
+  const int b0 = (blockIdx.y % a.ngrp) * G;
   if (i >= a.ntp) return;
-  const int g = a.tp_g[i];
-  const int e0 = a.pt_off[g], e1 = a.pt_off[g + 1];
-  const int nob = min(kScatG, a.nb - b0);
+  const int4 inf = a.tp_info[i];
+  const int nob = min(G, a.nb - b0);
   // the read half of the read-modify-write is issued first and overlaps the KB sum
-  TH *p = (TH *)a.hout + ((size_t)ik * a.nb + b0) * a.osz + a.tp_off[i];
-  double2 old[kScatG];
+  TH *p = (TH *)a.hout + ((size_t)ik * a.nb + b0) * a.osz + inf.x;
+  double2 old[G];
 #pragma unroll
-  for (int o = 0; o < kScatG; ++o)
+  for (int o = 0; o < G; ++o)
     if (o < nob) old[o] = cvt2<double2>(p[(size_t)o * a.osz]);
   const double2 *zb = a.zent + (size_t)ik * a.nent * a.cmax;
   const double2 *cb = a.c + (size_t)ik * a.nlma * a.nb + b0;
-  double2 acc[kScatG];
+  double2 acc[G];
 #pragma unroll
-  for (int o = 0; o < kScatG; ++o) acc[o] = make_double2(0.0, 0.0);
-  for (int e = e0; e < e1; ++e) {
-    const int lc = a.pt_lma[e];
+  for (int o = 0; o < G; ++o) acc[o] = make_double2(0.0, 0.0);
+  int lc = inf.w;
+  for (int q = 0; q < inf.z; ++q) {
+    const int e = inf.y + q;
+    if (q > 0) lc = a.pt_lma[e];
     const int ilma0 = lc & 0xffffff;
     const int cnt = lc >> 24;
-    for (int l = 0; l < cnt; ++l) {
-      const double2 zv = zb[(size_t)e * a.cmax + l];
-      const double2 *cl = cb + (size_t)(ilma0 + l) * a.nb;
+    const double2 *ze = zb + (size_t)e * a.cmax;
+    if (CM > 0) {
+      double2 zv[CM > 0 ? CM : 1], cv[CM > 0 ? CM : 1][G];
 #pragma unroll
-      for (int o = 0; o < kScatG; ++o) {
-        if (o < nob) {
-          const double2 cv = cl[o];
-          acc[o].x += cv.x * zv.x - cv.y * zv.y;
-          acc[o].y += cv.x * zv.y + cv.y * zv.x;
+      for (int l = 0; l < CM; ++l) {
+        const int lx = min(l, cnt - 1);
+        zv[l] = ze[lx];
+#pragma unroll
+        for (int o = 0; o < G; ++o) cv[l][o] = cb[(size_t)(ilma0 + lx) * a.nb + min(o, nob - 1)];
+      }
+#pragma unroll
+      for (int l = 0; l < CM; ++l) {
+        if (l < cnt) {
+#pragma unroll
+          for (int o = 0; o < G; ++o) {
+            acc[o].x += cv[l][o].x * zv[l].x - cv[l][o].y * zv[l].y;
+            acc[o].y += cv[l][o].x * zv[l].y + cv[l][o].y * zv[l].x;
+          }
+        }
+      }
+    } else {
+      for (int l = 0; l < cnt; ++l) {
+        const double2 zvl = ze[l];
+        const double2 *cl = cb + (size_t)(ilma0 + l) * a.nb;
+#pragma unroll
+        for (int o = 0; o < G; ++o) {
+          if (o < nob) {
+            const double2 c = cl[o];
+            acc[o].x += c.x * zvl.x - c.y * zvl.y;
+            acc[o].y += c.x * zvl.y + c.y * zvl.x;
+          }
         }
       }
     }
   }
 #pragma unroll
-  for (int o = 0; o < kScatG; ++o) {
+  for (int o = 0; o < G; ++o) {
     if (o < nob) {
       double2 v = old[o];
       v.x += a.s.x * acc[o].x - a.s.y * acc[o].y;
@@ -1390,7 +1427,8 @@ struct salmon_hip_ctx {
   int natom = 0, nps = 0, nlma = 0, nent = 0;
   int cmax = 0;
   int *d_mps = nullptr, *d_jidx = nullptr, *d_lma_atom = nullptr, *d_pt_off = nullptr;
-  int *d_pt_lma = nullptr, *d_entof = nullptr, *d_tp_g = nullptr, *d_tp_off = nullptr;
+  int *d_pt_lma = nullptr, *d_entof = nullptr;
+  int4 *d_tp_info = nullptr;  // per touched point: orbital-block offset, first entry, entry count, its pt_lma
   int ntp = 0;
   double2 *d_sink = nullptr;
   int2 *d_atom_lma = nullptr;
@@ -1510,7 +1548,7 @@ int salmon_hip_destroy(salmon_hip_t c) {
   dfree(c->d_mps); dfree(c->d_jidx); dfree(c->d_lma_atom); dfree(c->d_pt_off);
   dfree(c->d_atom_lma); dfree(c->d_pt_lma); dfree(c->d_entof); dfree(c->d_rxyz); dfree(c->d_uv);
   dfree(c->d_rinv); dfree(c->d_zekr); dfree(c->d_zent); dfree(c->d_c); dfree(c->d_curnl);
-  dfree(c->d_tp_g); dfree(c->d_tp_off); dfree(c->d_sink);
+  dfree(c->d_tp_info); dfree(c->d_sink);
   dfree(c->d_curpart); dfree(c->d_contrib); dfree(c->d_denspart);
   for (int d = 0; d < 3; ++d) dfree(c->d_tw[d]);
   dfree(c->d_gcoef); dfree(c->d_vh); dfree(c->d_fft[0]); dfree(c->d_fft[1]);
@@ -1542,7 +1580,7 @@ int salmon_hip_set_nonlocal(salmon_hip_t c, int natom, int nps, const int *mps, 
   dfree(c->d_mps); dfree(c->d_jidx); dfree(c->d_lma_atom); dfree(c->d_pt_off);
   dfree(c->d_atom_lma); dfree(c->d_pt_lma); dfree(c->d_entof); dfree(c->d_rxyz); dfree(c->d_uv);
   dfree(c->d_rinv); dfree(c->d_zekr); dfree(c->d_zent); dfree(c->d_c); dfree(c->d_curnl);
-  dfree(c->d_tp_g); dfree(c->d_tp_off);
+  dfree(c->d_tp_info);
   c->natom = c->nps = c->nlma = c->nent = c->cmax = c->ntp = 0;
   c->have_k = false;
   if (natom == 0 || nlma == 0) return 0;
@@ -1599,23 +1637,18 @@ int salmon_hip_set_nonlocal(salmon_hip_t c, int natom, int nps, const int *mps, 
       entof[(size_t)nps * ia + j] = e;
     }
   }
-  std::vector<int> tp_g, tp_off;
+  std::vector<int4> tp_info;
   for (int z = 0; z < c->nz; ++z)
     for (int y = 0; y < c->ny; ++y)
       for (int x = 0; x < c->nx; ++x) {
         const size_t g = (size_t)x + (size_t)c->nx * ((size_t)y + (size_t)c->ny * z);
-        if (cnt[g + 1] > cnt[g]) {
-          tp_g.push_back((int)g);
-          tp_off.push_back(x + c->nx * (y + c->ly * z));
-        }
+        if (cnt[g + 1] > cnt[g])
+          tp_info.push_back(make_int4(x + c->nx * (y + c->ly * z), cnt[g], cnt[g + 1] - cnt[g], pt_lma[cnt[g]]));
       }
   int rc = 0;
-  if (!tp_g.empty() &&
-      ((rc = dalloc(&c->d_tp_g, tp_g.size())) || (rc = dalloc(&c->d_tp_off, tp_off.size()))))
-    return rc;
-  if (!tp_g.empty()) {
-    HIPCHK(hipMemcpy(c->d_tp_g, tp_g.data(), sizeof(int) * tp_g.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_tp_off, tp_off.data(), sizeof(int) * tp_off.size(), hipMemcpyHostToDevice));
+  if (!tp_info.empty()) {
+    if ((rc = dalloc(&c->d_tp_info, tp_info.size()))) return rc;
+    HIPCHK(hipMemcpy(c->d_tp_info, tp_info.data(), sizeof(int4) * tp_info.size(), hipMemcpyHostToDevice));
   }
   if ((rc = dalloc(&c->d_mps, natom)) || (rc = dalloc(&c->d_jidx, (size_t)natom * nps)) ||
       (rc = dalloc(&c->d_lma_atom, nlma)) || (rc = dalloc(&c->d_pt_off, c->ngrid + 1)) ||
@@ -1641,7 +1674,7 @@ int salmon_hip_set_nonlocal(salmon_hip_t c, int natom, int nps, const int *mps, 
   c->nlma = nlma;
   c->nent = nent;
   c->cmax = cmax;
-  c->ntp = (int)tp_g.size();
+  c->ntp = (int)tp_info.size();
   return 0;
 }
 
@@ -1863,9 +1896,7 @@ static int launch_scatter(salmon_hip_t c, void *hout, double2 s, bool f32 = fals
   a.hout = hout;
   a.c = c->d_c;
   a.zent = c->d_zent;
-  a.tp_g = c->d_tp_g;
-  a.tp_off = c->d_tp_off;
-  a.pt_off = c->d_pt_off;
+  a.tp_info = c->d_tp_info;
   a.pt_lma = c->d_pt_lma;
   a.s = s;
   a.osz = c->osz;
@@ -1874,14 +1905,22 @@ static int launch_scatter(salmon_hip_t c, void *hout, double2 s, bool f32 = fals
   a.nlma = c->nlma;
   a.nent = c->nent;
   a.cmax = c->cmax;
-  a.ngrp = (c->nb + kScatG - 1) / kScatG;
+  const int G = f32 ? kScatG32 : kScatG;
+  a.ngrp = (c->nb + G - 1) / G;
   hipEvent_t e0, e1;
   prof_events(c, SALMON_HIP_KERNEL_NL_SCATTER, &e0, &e1);
   dim3 grid((unsigned)((c->ntp + kBlock - 1) / kBlock), (unsigned)(c->nk * a.ngrp));
-  if (f32)
-    hipExtLaunchKernelGGL(k_nl_scatter<float2>, grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
-  else
-    hipExtLaunchKernelGGL(k_nl_scatter<double2>, grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+  if (f32) {
+    if (c->cmax <= kScatCM32)
+      hipExtLaunchKernelGGL((k_nl_scatter<float2, kScatG32, kScatCM32>), grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+    else
+      hipExtLaunchKernelGGL((k_nl_scatter<float2, kScatG32, 0>), grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+  } else {
+    if (c->cmax <= kScatCM)
+      hipExtLaunchKernelGGL((k_nl_scatter<double2, kScatG, kScatCM>), grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+    else
+      hipExtLaunchKernelGGL((k_nl_scatter<double2, kScatG, 0>), grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
+  }
   LAUNCHCHK("k_nl_scatter");
   return 0;
 }
