@@ -2454,9 +2454,17 @@ struct salmon_hip_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<void *> allocs;
+  // the per-system kernels of a phase (projections, scatters, observables) run on kLanes side
+  // streams forked from and joined back to `stream`, so small systems overlap (captured as
+  // parallel branches of the graph)
+  std::vector<hipStream_t> lanes;
+  std::vector<hipEvent_t> lane_ev;
+  hipEvent_t fork_ev = nullptr;
 };
 
 namespace {
+
+constexpr int kLanes = 4;  // side streams of a batch
 
 template <typename T>
 int balloc(salmon_hip_batch *b, T **p, size_t n) {
@@ -2520,38 +2528,66 @@ int batch_stencil(salmon_hip_batch *b, int p, int n) {
   return 0;
 }
 
+// fork: system i's kernels go to lane i % kLanes after all work enqueued on the batch stream
+int batch_fork(salmon_hip_batch *b) {
+  HIPCHK(hipEventRecord(b->fork_ev, b->stream));
+  for (size_t k = 0; k < b->lanes.size(); ++k) HIPCHK(hipStreamWaitEvent(b->lanes[k], b->fork_ev, 0));
+  for (size_t i = 0; i < b->h.size(); ++i) b->h[i]->stream = b->lanes[i % b->lanes.size()];
+  return 0;
+}
+
+// join: later work on the batch stream waits for every lane
+int batch_join(salmon_hip_batch *b) {
+  for (size_t i = 0; i < b->h.size(); ++i) b->h[i]->stream = b->stream;
+  for (size_t k = 0; k < b->lanes.size(); ++k) {
+    HIPCHK(hipEventRecord(b->lane_ev[k], b->lanes[k]));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->lane_ev[k], 0));
+  }
+  return 0;
+}
+
 int batch_step(salmon_hip_batch *b, int p) {
   const int N = b->n_hamil;
   const size_t ns = b->h.size();
-  for (size_t i = 0; i < ns; ++i)
-    if (int rc = update_kvector_impl(b->h[i], nullptr, b->sys[i].actab, b->d_itt)) return rc;
+  int rc = batch_fork(b);
+  for (size_t i = 0; i < ns && !rc; ++i) rc = update_kvector_impl(b->h[i], nullptr, b->sys[i].actab, b->d_itt);
+  if (int r2 = batch_join(b)) return r2;
+  if (rc) return rc;
   for (int n = 1; n <= N; ++n) {
-    for (size_t i = 0; i < ns; ++i) {
+    rc = batch_fork(b);
+    for (size_t i = 0; i < ns && !rc; ++i) {
       salmon_hip_t c = b->h[i];
       void *hh[kMaxComb + 1], *out;
       batch_buffers(b, (int)i, p, hh, &out);
-      if (c->natom > 0)
-        if (int rc = launch_proj(c, hh[n - 1], 1)) return rc;
+      if (c->natom > 0) rc = launch_proj(c, hh[n - 1], 1);
     }
-    if (int rc = batch_stencil(b, p, n)) return rc;
-    for (size_t i = 0; i < ns; ++i) {
+    if (int r2 = batch_join(b)) return r2;
+    if (rc) return rc;
+    if (int r2 = batch_stencil(b, p, n)) return r2;
+    rc = batch_fork(b);
+    for (size_t i = 0; i < ns && !rc; ++i) {
       salmon_hip_t c = b->h[i];
       void *hh[kMaxComb + 1], *out;
       batch_buffers(b, (int)i, p, hh, &out);
-      if (int rc = join_proj(c)) return rc;
-      if (c->natom > 0 && c->nlma > 0) {
+      rc = join_proj(c);
+      if (!rc && c->natom > 0 && c->nlma > 0) {
         const double2 sc = n == N ? make_double2(b->zc[2 * (N - 1)], b->zc[2 * (N - 1) + 1]) : make_double2(1.0, 0.0);
-        if (int rc = launch_scatter(c, n == N ? out : hh[n], sc)) return rc;
+        rc = launch_scatter(c, n == N ? out : hh[n], sc);
       }
     }
+    if (int r2 = batch_join(b)) return r2;
+    if (rc) return rc;
   }
-  for (size_t i = 0; i < ns; ++i) {
+  rc = batch_fork(b);
+  for (size_t i = 0; i < ns && !rc; ++i) {
     salmon_hip_t c = b->h[i];
     void *out = b->sys[i].psi[1 - p];
-    if (int rc = salmon_hip_density(c, out, b->sys[i].rho)) return rc;
-    if (int rc = current_impl(c, nullptr, b->sys[i].actab, b->d_itt, out, b->sys[i].curr_s)) return rc;
-    if (int rc = salmon_hip_dipole(c, b->sys[i].rho, b->sys[i].dip_s)) return rc;
+    rc = salmon_hip_density(c, out, b->sys[i].rho);
+    if (!rc) rc = current_impl(c, nullptr, b->sys[i].actab, b->d_itt, out, b->sys[i].curr_s);
+    if (!rc) rc = salmon_hip_dipole(c, b->sys[i].rho, b->sys[i].dip_s);
   }
+  if (int r2 = batch_join(b)) return r2;
+  if (rc) return rc;
   hipLaunchKernelGGL(k_record, dim3(1), dim3(std::max<size_t>(64, (ns + 63) / 64 * 64)), 0, b->stream, b->d_csrc,
                      b->d_cdst, b->d_cw, b->d_dsrc, b->d_ddst, b->d_nt, (int)ns, b->d_itt);
   LAUNCHCHK("k_record");
@@ -2580,6 +2616,24 @@ int salmon_hip_batch_create(const salmon_hip_t *handles, int n, void *stream, sa
     handles[i]->stream = b->stream;
   }
   b->sys.resize(n);
+  for (int k = 0; k < std::min(n, kLanes); ++k) {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      salmon_hip_batch_destroy(b);
+      return fail(SALMON_HIP_EHIP, "batch lane stream");
+    }
+    b->lanes.push_back(st);
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      salmon_hip_batch_destroy(b);
+      return fail(SALMON_HIP_EHIP, "batch lane event");
+    }
+    b->lane_ev.push_back(ev);
+  }
+  if (hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess) {
+    salmon_hip_batch_destroy(b);
+    return fail(SALMON_HIP_EHIP, "batch fork event");
+  }
   int rc = balloc(b, &b->d_itt, 1);
   if (!rc) rc = balloc(b, &b->d_csrc, n);
   if (!rc) rc = balloc(b, &b->d_cdst, n);
@@ -2599,10 +2653,15 @@ int salmon_hip_batch_create(const salmon_hip_t *handles, int n, void *stream, sa
 int salmon_hip_batch_destroy(salmon_hip_batch_t b) {
   if (!b) return 0;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (hipStream_t st : b->lanes) (void)hipStreamSynchronize(st);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (auto &g : b->groups) { dfree(g.d_tiles[0]); dfree(g.d_tiles[1]); dfree(g.d_args); }
   for (void *p : b->allocs) (void)hipFree(p);
+  for (hipStream_t st : b->lanes) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : b->lane_ev) (void)hipEventDestroy(ev);
+  if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+  for (salmon_hip_t c : b->h) c->stream = b->stream;
   delete b;
   return 0;
 }
