@@ -32,7 +32,8 @@ def hipcc() -> str:
 
 
 def build_hip(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, "salmon_hip.hip"), os.path.join(CSRC, "salmon_unet.hip")]
+    srcs = [os.path.join(CSRC, "salmon_hip.hip"), os.path.join(CSRC, "salmon_unet.hip"),
+            os.path.join(CSRC, "salmon_pp.cpp")]
     tmp = LIB + ".tmp"
     cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), *srcs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -432,6 +432,39 @@ def make_si_system(
     )
 
 
+def make_si_pp_system(ncell=(1, 1, 1), n=12, no=16, nkgrid=(2, 2, 2), seed=20261212, dt=0.08, nt=100, A0=0.01,
+                      omega=0.5, n_hamil=4, name="si_pp", pad=1, pp_file=None, lmax_ps=2, lloc_ps=2):
+    """Diamond-Si supercell with the tables of a real pseudopotential (default SALMON's
+    testsuites/pseudo/Si_rps.dat with Lmax_ps = Lloc_ps = 2, as its bulk-Si tests use):
+    V_local = Vpsl and the KB projectors come from the projector-table builder
+    (include/salmon_pp.h); the orbitals are seeded random as in make_si_system."""
+    from . import pp as _pp
+
+    ncell = tuple(ncell)
+    nn = (n, n, n) if np.isscalar(n) else tuple(n)
+    al = np.array([SI_A * ncell[0], SI_A * ncell[1], SI_A * ncell[2]])
+    hgs = al / np.array(nn, dtype=float)
+    hvol = float(al[0] * al[1] * al[2]) / (nn[0] * nn[1] * nn[2])
+    rion = si_positions(ncell)
+    kion = np.ones(len(rion), dtype=np.int32)
+    P = _pp.PseudoPotential([pp_file or _pp.SI_RPS], lloc_ps=[lloc_ps], lmax_ps=[lmax_ps])
+    mps, nps, jxyz, rxyz, nlma, ia_tbl, rinv, uv = P.nonlocal_tables(kion, rion, al, nn)
+    V = P.vpsl(kion, rion, al, nn)
+    P.close()
+    rng = np.random.default_rng(seed)
+    lap, nab, lap0 = stencil_coefs(hgs)
+    vec_k, wtk = mp_kpoints(nkgrid, al)
+    nk = len(wtk)
+    nspin = 1
+    ly = nn[1] + pad
+    psi = random_orbitals(rng, nn, ly, no, nk, nspin, hvol)
+    return System(name=name, nx=nn[0], ny=nn[1], nz=nn[2], ly=ly, al=al, hgs=hgs, hvol=hvol, nspin=nspin, no=no,
+                  nk=nk, vec_k=vec_k, wtk=wtk, rocc=np.full((nspin, nk, no), 2.0), coef_lap=lap, coef_nab=nab,
+                  coef_lap0=lap0, vlocal=V[None].copy(), rion=rion, natom=len(rion), nps=nps, mps=mps,
+                  jxyz=jxyz, rxyz=rxyz, nlma=nlma, ia_tbl=ia_tbl, rinv_uvu=rinv, uv=uv, psi=psi, dt=dt, nt=nt,
+                  n_hamil=n_hamil, pulse=Pulse(A0=A0, omega=omega, tw=nt * dt), seed=seed)
+
+
 # Named workloads (seed = 20261202 + configuration number)
 def config(name: str, **over) -> System:
     """Named workloads.

@@ -15,12 +15,12 @@ def declared(header="salmon_hip.h", prefix="salmon_hip_"):
 
 
 def all_declared():
-    return declared() + declared("salmon_unet.h", "salmon_unet_")
+    return declared() + declared("salmon_unet.h", "salmon_unet_") + declared("salmon_pp.h", "salmon_pp_")
 
 
 def test_headers_are_all_covered():
     assert sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")) == [
-        "salmon_hip.h", "salmon_unet.h"]
+        "salmon_hip.h", "salmon_pp.h", "salmon_unet.h"]
 
 
 def test_header_declares_the_binding_list():
@@ -28,6 +28,9 @@ def test_header_declares_the_binding_list():
 
     assert set(declared()) == set(hip.EXPORTS)
     assert set(declared("salmon_unet.h", "salmon_unet_")) == set(unet.EXPORTS)
+    from salmon2_amd import pp
+
+    assert set(declared("salmon_pp.h", "salmon_pp_")) == set(pp.EXPORTS)
 
 
 def test_library_exports_every_declared_symbol():

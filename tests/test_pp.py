@@ -1,0 +1,225 @@
+"""Projector-table builder (include/salmon_pp.h) against the oracle's restatement of SALMON's
+init_ps / input_pp, and the oracle pinned by what the mathematics fixes.
+
+Runs on the CPU: the builder is host code of the library (no GPU call).
+
+Pins of the oracle (oracle/pp_oracle.py):
+  * spline: scipy's natural cubic spline on the same knots;
+  * Ylm: the addition theorem sum_m Ylm^2 = r^(2l) (SALMON's Ylm = sqrt(4pi/(2l+1)) r^l Y_lm);
+  * sphere points: brute-force count of grid points (all periodic images) within rps;
+  * a synthetic KY file with closed-form channels (Gaussian potentials and radial
+    functions, an erf-screened Coulomb local part): anorm^2 = A int r^(2l+2) e^(-r^2/s^2) dr,
+    uv = the closed-form radial function times Ylm, and Vpsl = the lattice sum of
+    -Z erf(r/sigma)/r over the DFT's G set (its transform is -4 pi Z e^(-G^2 sigma^2/4)/G^2,
+    G = 0 term natom pi Z sigma^2 / volume).
+The reference's Si_rps.dat (SALMON testsuites/pseudo, a data file) is read by both.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pp_oracle as P
+from salmon2_amd import pp, system
+
+AA = P.AU_LENGTH_AA
+EV = P.AU_ENERGY_EV
+
+
+def _si(n, ncell=(1, 1, 1), shift=(0.0, 0.0, 0.0)):
+    al = np.array([system.SI_A * c for c in ncell])
+    rion = system.si_positions(ncell) + np.array(shift)
+    return al, rion, [n] * 3 if np.isscalar(n) else list(n)
+
+
+def test_spline_matches_scipy():
+    from scipy.interpolate import CubicSpline
+
+    rng = np.random.default_rng(3)
+    x = np.cumsum(rng.uniform(0.05, 0.2, 40))
+    y = np.sin(3 * x) + 0.1 * rng.standard_normal(40)
+    an, bn, cn, dn = P.spline(x, y)
+    cs = CubicSpline(x, y, bc_type="natural")
+    # scipy: c[0] x^3 + c[1] x^2 + c[2] x + c[3] on [x_i, x_{i+1})
+    for coef, ref in ((an, cs.c[0]), (bn, cs.c[1]), (cn, cs.c[2]), (dn, cs.c[3])):
+        assert np.abs(coef - ref).max() <= 1e-12 * np.abs(ref).max() + 1e-13
+
+
+def test_ylm_addition_theorem():
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        x, y, z = rng.standard_normal(3)
+        r2 = x * x + y * y + z * z
+        for l in range(3):
+            s = sum(P.ylm(x, y, z, l, m) ** 2 for m in range(-l, l + 1))
+            assert abs(s - r2 ** l) <= 1e-12 * max(1.0, r2 ** l)
+
+
+def test_reference_si_file():
+    e = P.read_ps_ky(pp.SI_RPS, lmax_ps=2, lloc_ps=2)
+    assert e["mr"] == 2000 and e["zps"] == 4 and e["mlps"] == 2 and e["lref"] == 2
+    assert abs(e["rps"] - 1.2 / AA) < 1e-14  # rrc = 1.2 Angstrom for every channel
+    assert e["rad"][e["nrps"]] > e["rps"] >= e["rad"][e["nrps"] - 1]
+    assert list(e["inorm"]) == [1, 1, 0]  # the d channel is the local one
+    lib = pp.PseudoPotential([pp.SI_RPS], lloc_ps=[2], lmax_ps=[2])
+    got = lib.element(1)
+    assert got == dict(zps=4, mlps=2, lref=2, rps=e["rps"], nrps=e["nrps"], nlma_atom=4)
+
+
+@pytest.mark.parametrize("case", ["si8_12", "si8_16", "si16_ragged", "si8_shifted"])
+def test_builder_matches_oracle(case):
+    geo = {
+        "si8_12": _si(12),
+        "si8_16": _si(16),
+        "si16_ragged": _si((20, 12, 14), ncell=(2, 1, 1)),
+        "si8_shifted": _si(12, shift=(-0.7, 0.3, 9.9)),  # atoms outside [0, L): two images per side
+    }[case]
+    al, rion, n = geo
+    e = P.read_ps_ky(pp.SI_RPS, lmax_ps=2, lloc_ps=2)
+    lib = pp.PseudoPotential([pp.SI_RPS], lloc_ps=[2], lmax_ps=[2])
+    kion = np.ones(len(rion), dtype=np.int32)
+    mps, nps, jxyz, rxyz, nlma, ia_tbl, rinv, uv = lib.nonlocal_tables(kion, rion, al, n)
+    om, oj, orx, oia, orinv, ouv = P.nonlocal_tables([e], [0] * len(rion), rion.tolist(), list(al), n)
+    assert np.array_equal(mps, om) and nps == oj.shape[1]
+    assert np.array_equal(jxyz, oj) and np.array_equal(ia_tbl, oia)
+    assert np.abs(rxyz - orx).max() <= 1e-13
+    assert np.abs(rinv - orinv).max() <= 1e-15 * np.abs(orinv).max()
+    assert np.abs(uv - ouv).max() <= 1e-12 * np.abs(ouv).max()
+    v = lib.vpsl(kion, rion, al, n)
+    ov = P.vpsl_periodic([e], [0] * len(rion), rion.tolist(), list(al), n)
+    assert np.abs(v - ov).max() <= 1e-12 * np.abs(ov).max()
+
+
+def test_sphere_points_brute_force():
+    al, rion, n = _si((14, 12, 10), ncell=(1, 1, 1), shift=(0.4, -0.2, 0.1))
+    rps = P.read_ps_ky(pp.SI_RPS, lmax_ps=2, lloc_ps=2)["rps"]
+    sp = P.spheres([rps], [0] * len(rion), rion.tolist(), list(al), n)
+    h = al / np.array(n)
+    g = np.stack(np.meshgrid(*[np.arange(n[d]) * h[d] for d in range(3)], indexing="ij"), -1).reshape(-1, 3)
+    for ia, R in enumerate(rion):
+        cnt = 0
+        for img in np.ndindex(5, 5, 5):
+            d = g - (R + (np.array(img) - 2) * al)
+            cnt += int((np.sqrt((d * d).sum(1)) < rps + 1e-12).sum())
+        assert cnt == len(sp[ia])
+
+
+# ---- synthetic KY file with closed-form channels ------------------------------------
+Z, SIG = 4, 0.6
+A0, B0, C0 = -1.3, 1.1, 1.4  # s: dv = A0 exp(-r^2/B0^2), u = r exp(-r^2/C0^2)
+A1, B1, C1 = 0.9, 1.0, 1.2  # p: dv = A1 exp(-r^2/B1^2), u = r^2 exp(-r^2/C1^2)
+RPS, STEP, MR = 3.0, 0.01, 900  # bohr
+
+
+def _vloc(r):
+    return np.where(r > 0, -Z * np.vectorize(math.erf)(r / SIG) / np.maximum(r, 1e-300), -Z * 2 / (SIG * math.sqrt(math.pi)))
+
+
+def _write_ky(path):
+    r = np.arange(MR + 1) * STEP
+    vl = _vloc(r)
+    v0 = vl + A0 * np.exp(-r * r / B0 ** 2)
+    v1 = vl + A1 * np.exp(-r * r / B1 ** 2)
+    u0 = r * np.exp(-r * r / C0 ** 2)
+    u1 = r * r * np.exp(-r * r / C1 ** 2)
+    u2 = r ** 3 * np.exp(-r * r)
+    with open(path, "w") as f:
+        f.write(f"{MR} {STEP * AA:.17e} 2 {Z:.1f}\n")
+        f.write(f"0 {RPS * AA:.17e} {RPS * AA:.17e} {RPS * AA:.17e}\n")
+        for i in range(MR + 1):
+            f.write(f"{r[i] * AA:.17e} 0.0 {v0[i] * EV:.17e} {v1[i] * EV:.17e} {vl[i] * EV:.17e}\n")
+        for i in range(MR + 1):
+            s = math.sqrt(AA)
+            f.write(f"{r[i] * AA:.17e} {u0[i] / s:.17e} {u1[i] / s:.17e} {u2[i] / s:.17e}\n")
+
+
+@pytest.fixture(scope="module")
+def synthetic(tmp_path_factory):
+    path = str(tmp_path_factory.mktemp("pp") / "X_rps.dat")
+    _write_ky(path)
+    return path
+
+
+def test_synthetic_anorm(synthetic):
+    e = P.read_ps_ky(synthetic, lmax_ps=2, lloc_ps=2)
+    s0 = (2 / C0 ** 2 + 1 / B0 ** 2) ** -0.5
+    s1 = (2 / C1 ** 2 + 1 / B1 ** 2) ** -0.5
+    a0 = abs(A0) * math.sqrt(math.pi) / 4 * s0 ** 3
+    a1 = abs(A1) * 3 * math.sqrt(math.pi) / 8 * s1 ** 5
+    assert list(e["inorm"]) == [-1, 1, 0]
+    assert abs(e["anorm"][0] ** 2 - a0) <= 1e-4 * a0
+    assert abs(e["anorm"][1] ** 2 - a1) <= 1e-4 * a1
+    assert e["nrps"] == int(round(RPS / STEP)) + 2
+
+
+def test_synthetic_uv(synthetic):
+    e = P.read_ps_ky(synthetic, lmax_ps=2, lloc_ps=2)
+    al, rion, n = _si(16)
+    mps, jxyz, rxyz, ia_tbl, rinv, uv = P.nonlocal_tables([e], [0] * 8, rion.tolist(), list(al), n)
+    hvol = al.prod() / np.prod(n)
+    assert np.allclose(rinv.reshape(8, 4), np.array([-1, 1, 1, 1]) * hvol, rtol=1e-15)
+    for ia in range(8):
+        x, y, z = rxyz[ia, : mps[ia]].T
+        r2 = x * x + y * y + z * z
+        # u/r^(l+1) at r = 0 is taken from the first radial point (upp(0) = upp(1),
+        # input_pp.f90:1150): at the atom's own grid point the radial factor uses r = step
+        ru2 = np.where(r2 < STEP ** 2, STEP ** 2, r2)
+        rad0 = A0 * np.exp(-r2 / B0 ** 2) * np.exp(-ru2 / C0 ** 2) / math.sqrt(4 * math.pi) / e["anorm"][0]
+        rad1 = A1 * np.exp(-r2 / B1 ** 2) * math.sqrt(3 / (4 * math.pi)) * np.exp(-ru2 / C1 ** 2) / e["anorm"][1]
+        ref = [rad0, -y * rad1, z * rad1, -x * rad1]
+        for k in range(4):
+            got = uv[4 * ia + k, : mps[ia]]
+            assert np.abs(got - ref[k]).max() <= 1e-6 * np.abs(ref[k]).max(), (ia, k)
+    lib = pp.PseudoPotential([synthetic], lloc_ps=[2], lmax_ps=[2])
+    got = lib.nonlocal_tables(np.ones(8, np.int32), rion, al, n)
+    assert np.abs(got[-1] - uv).max() <= 1e-12 * np.abs(uv).max()
+
+
+def _vpsl_closed_form(al, rion, n):
+    vol = al.prod()
+    k = [np.fft.fftfreq(n[d], 1.0 / n[d]) * 2 * math.pi / al[d] for d in range(3)]
+    gz, gy, gx = np.meshgrid(k[2], k[1], k[0], indexing="ij")
+    g2 = gx * gx + gy * gy + gz * gz
+    sf = sum(np.exp(-1j * (gx * R[0] + gy * R[1] + gz * R[2])) for R in rion)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        vg = np.where(g2 > 0, -4 * math.pi * Z * np.exp(-g2 * SIG ** 2 / 4) / g2, 0.0) * sf / vol
+    vg[0, 0, 0] = len(rion) * math.pi * Z * SIG ** 2 / vol
+    return (np.fft.ifftn(vg) * vg.size).real
+
+
+def test_synthetic_vpsl(synthetic, tmp_path):
+    """The radial integrals of calc_vpsl_periodic are midpoint sums: the oracle agrees with
+    the closed form to O(step^2) (8e-5 of max|V| at step 0.01 bohr), and the difference
+    falls fourfold when the radial step is halved, so what remains is the quadrature."""
+    global STEP, MR
+    al, rion, n = _si(12)
+    ref = _vpsl_closed_form(al, rion, n)
+    e = P.read_ps_ky(synthetic, lmax_ps=2, lloc_ps=2)
+    v = P.vpsl_periodic([e], [0] * 8, rion.tolist(), list(al), n)
+    err1 = np.abs(v - ref).max() / np.abs(ref).max()
+    assert err1 <= 2e-4
+    step0, mr0 = STEP, MR
+    try:
+        STEP, MR = step0 / 2, 2 * mr0
+        fine = str(tmp_path / "X_fine_rps.dat")
+        _write_ky(fine)
+        v2 = P.vpsl_periodic([P.read_ps_ky(fine, lmax_ps=2, lloc_ps=2)], [0] * 8, rion.tolist(), list(al), n)
+    finally:
+        STEP, MR = step0, mr0
+    err2 = np.abs(v2 - ref).max() / np.abs(ref).max()
+    assert err2 <= 0.3 * err1
+    lib = pp.PseudoPotential([synthetic], lloc_ps=[2], lmax_ps=[2])
+    got = lib.vpsl(np.ones(8, np.int32), rion, al, n)
+    assert np.abs(got - v).max() <= 1e-12 * np.abs(v).max()
+
+
+def test_builder_rejects_bad_input(tmp_path):
+    with pytest.raises(Exception, match="cannot open"):
+        pp.PseudoPotential([str(tmp_path / "missing.dat")], lloc_ps=[0])
+    with pytest.raises(Exception, match="Lloc_ps"):
+        pp.PseudoPotential([pp.SI_RPS], lloc_ps=[3], lmax_ps=[2])
+    lib = pp.PseudoPotential([pp.SI_RPS], lloc_ps=[2], lmax_ps=[2])
+    al, rion, n = _si(12)
+    with pytest.raises(Exception, match="kion"):
+        lib.nonlocal_tables(np.full(8, 2, np.int32), rion, al, n)
