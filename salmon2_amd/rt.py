@@ -188,6 +188,33 @@ class RTPropagator:
         return dict(psi=self.psi_a, curr=curr, dip=dip[:, :3], rne=dip[:, 3], ac_ext=self.ac_ext[1 : n + 1],
                     ac_tot=self.ac[1 : n + 1], e_ext=e_ext, e_tot=e_tot)
 
+    def checkpoint(self, directory, sliced_block=0):
+        """The current orbitals, occupations and step count in SALMON's restart layout
+        (checkpoint_rt, src/io/checkpoint_restart.f90:231-275; salmon2_amd.restart).  The
+        induced field of trans_longi = 'lo' and the learned predictor's history are not
+        part of it."""
+        from . import restart
+
+        torch.cuda.synchronize(self.dev)
+        restart.write_checkpoint(directory, self.psi_a.cpu().numpy(), self.s.ny, self.s.rocc, self.itt,
+                                 sliced_block=sliced_block)
+
+    @classmethod
+    def from_restart(cls, sysm, directory, nt, **kw):
+        """Continue from a restart directory (restart_rt, src/io/checkpoint_restart.f90:277-306):
+        orbitals and occupations from its files, the step count from info.bin; nt = the total
+        number of steps of the run (outputs are indexed from step 1)."""
+        import dataclasses
+
+        from . import restart
+
+        d = restart.read_restart(directory, (sysm.nx, sysm.ny, sysm.nz), nspin=sysm.nspin, no=sysm.no, nk=sysm.nk,
+                                 ly=sysm.ly)
+        p = cls(dataclasses.replace(sysm, rocc=d["rocc"]), psi=d["psi"], **kw)
+        p.setup_outputs(nt)
+        p.itt = d["iter"]
+        return p
+
     def write_rt_data(self, path, res=None):
         """SYSname_rt.data of the steps run so far (write_rt_data_3d, unpolarised, atomic units)."""
         res = self.results() if res is None else res

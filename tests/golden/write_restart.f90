@@ -1,0 +1,70 @@
+! Writes a small restart directory with the Fortran I/O statements SALMON uses for its
+! restart data (src/io/checkpoint_restart.f90: info.bin and occupation.bin sequential
+! unformatted, wfn.bin stream access in (ix,iy,iz,is,io,ik) order, Vh_stock.bin two
+! sequential records), with values given by closed formulas the test recomputes.
+! Usage: write_restart <directory> [real]
+program write_restart
+  implicit none
+  integer, parameter :: nx = 5, ny = 4, nz = 3, nspin = 1, no = 3, nk = 2
+  complex(8) :: zwf(nx, ny, nz, nspin, no, nk)
+  real(8) :: rwf(nx, ny, nz, nspin, no, nk)
+  real(8) :: rocc(no, nk, nspin), vh1(nx, ny, nz), vh2(nx, ny, nz)
+  integer :: ix, iy, iz, is, io, ik
+  character(256) :: dir, mode
+  logical :: if_real
+
+  call get_command_argument(1, dir)
+  call get_command_argument(2, mode)
+  if_real = trim(mode) == 'real'
+  do ik = 1, nk
+  do io = 1, no
+  do is = 1, nspin
+  do iz = 1, nz
+  do iy = 1, ny
+  do ix = 1, nx
+    zwf(ix, iy, iz, is, io, ik) = cmplx(ix + 10*iy + 100*iz + 0.5d0*is, io + 10*ik + 0.25d0*(ix - iy), kind=8)
+    rwf(ix, iy, iz, is, io, ik) = ix - 2*iy + 3*iz + 0.125d0*io - 7*ik
+  end do
+  end do
+  end do
+  rocc(io, ik, is) = 2d0 - 0.1d0*io - 0.01d0*ik
+  end do
+  end do
+  end do
+  do iz = 1, nz
+  do iy = 1, ny
+  do ix = 1, nx
+    vh1(ix, iy, iz) = 0.5d0*ix - iy + 0.25d0*iz
+    vh2(ix, iy, iz) = -vh1(ix, iy, iz) + 1d0/3d0
+  end do
+  end do
+  end do
+
+  open(97, file=trim(dir)//'/info.bin', form='unformatted')
+  write(97) nk
+  write(97) no
+  write(97) 1234
+  write(97) 4
+  write(97) if_real
+  close(97)
+  open(97, file=trim(dir)//'/occupation.bin', form='unformatted')
+  write(97) rocc(1:no, 1:nk, 1:nspin)
+  close(97)
+  open(87, file=trim(dir)//'/wfn.bin', form='unformatted', access='stream')
+  do ik = 1, nk
+  do io = 1, no
+  do is = 1, nspin
+    if (if_real) then
+      write(87) rwf(1:nx, 1:ny, 1:nz, is, io, ik)
+    else
+      write(87) zwf(1:nx, 1:ny, 1:nz, is, io, ik)
+    end if
+  end do
+  end do
+  end do
+  close(87)
+  open(97, file=trim(dir)//'/Vh_stock.bin', form='unformatted')
+  write(97) vh1
+  write(97) vh2
+  close(97)
+end program write_restart
