@@ -124,6 +124,19 @@ int salmon_hip_taylor(salmon_hip_t h, const double *zc, int n_hamil, void *tspsi
                       void *tspsi_out, void *sshtpsi);
 
 /*
+ * The Taylor step of tspsi_in when only V_local changed since the last salmon_hip_taylor
+ * call on the same tspsi_in and k-vector (no counterpart in the reference): the corrector
+ * of the predictor-corrector (time_evolution_step.f90:354-423) repeats the predictor's
+ * Taylor step with the midpoint potential, and H_mid psi = H psi + (V_mid - V) psi, so the
+ * first H application is replaced by h_1 += dv * psi with dv_dev = V_mid - V (device array
+ * (nx, ny, nz, nspin)); the later terms run as in salmon_hip_taylor with the handle's
+ * current V_local, which must already be V_mid.  h_1 is sshtpsi in FP64 mode (it must
+ * still hold H tspsi_in from that call) and the handle's FP32 copy in mixed mode.
+ */
+int salmon_hip_taylor_reuse(salmon_hip_t h, const double *zc, int n_hamil, void *tspsi_in, void *tspsi_out,
+                            void *sshtpsi, const double *dv_dev);
+
+/*
  * Arithmetic of salmon_hip_taylor (no counterpart in the reference, which is
  * FP64 throughout).  SALMON_HIP_PREC_FP64 (default): as above.
  * SALMON_HIP_PREC_MIXED: H psi, H^2 psi, ... are formed and kept as complex FP32

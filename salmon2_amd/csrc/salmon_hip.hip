@@ -995,6 +995,30 @@ __global__ __launch_bounds__(kBlock) void k_nl_scatter(ScatterArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// h += dv * psi over the orbital blocks (dv per point and spin; padding rows untouched):
+// turns H_old psi into H_new psi when only V_local changed (salmon_hip_taylor_reuse).
+// ---------------------------------------------------------------------------
+template <typename TH>
+__global__ __launch_bounds__(kBlock) void k_vshift(TH *h, const double2 *psi, const double *dv, size_t osz, int nx,
+                                                   int ny, int ly, int nz, int nspin, size_t ntot) {
+  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ntot) return;
+  const size_t b = i / osz;
+  const size_t r = i % osz;
+  const int x = (int)(r % nx);
+  const int y = (int)((r / nx) % ly);
+  const int z = (int)(r / ((size_t)nx * ly));
+  if (y >= ny || z >= nz) return;
+  const int ispin = (int)(b % nspin);
+  const double d = dv[(size_t)ispin * nx * ny * nz + x + (size_t)nx * (y + (size_t)ny * z)];
+  const double2 p = psi[i];
+  double2 v = cvt2<double2>(h[i]);
+  v.x += d * p.x;
+  v.y += d * p.y;
+  h[i] = cvt2<TH>(v);
+}
+
+// ---------------------------------------------------------------------------
 // Density: rho(p) = sum_{ik, io} (|psi|^2 rocc) wtk, orbitals in (ik, io) order,
 // in groups of kDensGroup summed in order, groups combined in order.
 // ---------------------------------------------------------------------------
@@ -2014,7 +2038,8 @@ int salmon_hip_hpsi(salmon_hip_t c, const void *tpsi, void *htpsi) {
 // Mixed-precision Taylor step: h_n = H^n psi in complex FP32 (stencil and
 // local-potential products in FP32; projections, the KB term and the Taylor sum
 // in FP64), out = psi + sum_n zc(n) h_n formed in FP64 by the last stencil pass.
-static int taylor_mixed(salmon_hip_t c, const double *zc, int n_hamil, void *tin, void *tout) {
+// first = 1: the whole chain; first = 2: h_1 (the handle's FP32 H psi) is already current
+static int taylor_mixed(salmon_hip_t c, const double *zc, int n_hamil, void *tin, void *tout, int first = 1) {
   if (n_hamil < 2) return fail(SALMON_HIP_EUNSUPPORTED, "mixed precision needs n_hamil >= 2");
   while ((int)c->tscr32.size() < n_hamil - 1) {
     float2 *p = nullptr;
@@ -2024,7 +2049,7 @@ static int taylor_mixed(salmon_hip_t c, const double *zc, int n_hamil, void *tin
   void *h[kMaxComb + 1];
   h[0] = tin;
   for (int n = 1; n < n_hamil; ++n) h[n] = c->tscr32[n - 1];
-  for (int n = 1; n < n_hamil; ++n) {
+  for (int n = first; n < n_hamil; ++n) {
     if (c->natom > 0)
       if (int rc = launch_proj(c, h[n - 1], 1, n > 1)) return rc;
     const int kind = n == 1 ? SALMON_HIP_KERNEL_HPSI_TAYLOR1 : SALMON_HIP_KERNEL_HPSI_TAYLOR;
@@ -2045,15 +2070,9 @@ int salmon_hip_set_precision(salmon_hip_t c, int mode) {
   return 0;
 }
 
-int salmon_hip_taylor(salmon_hip_t c, const double *zc, int n_hamil, void *tspsi_in, void *tspsi_out,
-                      void *sshtpsi) {
-  if (int rc = check(c)) return rc;
-  if (!zc || n_hamil < 1 || n_hamil > kMaxComb + 1)
-    return fail(SALMON_HIP_EINVAL, "zc null or n_hamil=%d outside 1..%d", n_hamil, kMaxComb + 1);
-  if (!tspsi_in || !tspsi_out || !sshtpsi || tspsi_in == tspsi_out || tspsi_in == sshtpsi || tspsi_out == sshtpsi)
-    return fail(SALMON_HIP_EINVAL, "taylor buffers invalid or aliased");
-  if (int rc = ready(c)) return rc;
-  if (c->prec == SALMON_HIP_PREC_MIXED) return taylor_mixed(c, zc, n_hamil, tspsi_in, tspsi_out);
+// first = 1: the whole chain; first = 2: h_1 = sshtpsi is already current
+static int taylor_fp64(salmon_hip_t c, const double *zc, int n_hamil, void *tspsi_in, void *tspsi_out, void *sshtpsi,
+                       int first) {
   // h_1 = H psi -> sshtpsi, h_2 .. h_{N-1} -> internal scratch
   const int nscr = std::max(0, n_hamil - 2);
   while ((int)c->tscr.size() < nscr) {
@@ -2065,7 +2084,7 @@ int salmon_hip_taylor(salmon_hip_t c, const double *zc, int n_hamil, void *tspsi
   h[0] = tspsi_in;
   if (n_hamil >= 2) h[1] = sshtpsi;
   for (int n = 2; n < n_hamil; ++n) h[n] = c->tscr[n - 2];
-  for (int n = 1; n < n_hamil; ++n) {
+  for (int n = first; n < n_hamil; ++n) {
     if (c->natom > 0)
       if (int rc = launch_proj(c, h[n - 1], 1)) return rc;
     const int kind = n == 1 ? SALMON_HIP_KERNEL_HPSI_TAYLOR1 : SALMON_HIP_KERNEL_HPSI_TAYLOR;
@@ -2075,6 +2094,45 @@ int salmon_hip_taylor(salmon_hip_t c, const double *zc, int n_hamil, void *tspsi
     if (int rc = launch_proj(c, h[n_hamil - 1], 1)) return rc;
   return launch_hpsi(c, SALMON_HIP_KERNEL_HPSI_TAYLOR_LAST, h[n_hamil - 1], tspsi_out, true, h, n_hamil - 1, zc);
 }
+
+int salmon_hip_taylor(salmon_hip_t c, const double *zc, int n_hamil, void *tspsi_in, void *tspsi_out,
+                      void *sshtpsi) {
+  if (int rc = check(c)) return rc;
+  if (!zc || n_hamil < 1 || n_hamil > kMaxComb + 1)
+    return fail(SALMON_HIP_EINVAL, "zc null or n_hamil=%d outside 1..%d", n_hamil, kMaxComb + 1);
+  if (!tspsi_in || !tspsi_out || !sshtpsi || tspsi_in == tspsi_out || tspsi_in == sshtpsi || tspsi_out == sshtpsi)
+    return fail(SALMON_HIP_EINVAL, "taylor buffers invalid or aliased");
+  if (int rc = ready(c)) return rc;
+  if (c->prec == SALMON_HIP_PREC_MIXED) return taylor_mixed(c, zc, n_hamil, tspsi_in, tspsi_out);
+  return taylor_fp64(c, zc, n_hamil, tspsi_in, tspsi_out, sshtpsi, 1);
+}
+
+int salmon_hip_taylor_reuse(salmon_hip_t c, const double *zc, int n_hamil, void *tspsi_in, void *tspsi_out,
+                            void *sshtpsi, const double *dv_dev) {
+  if (int rc = check(c)) return rc;
+  if (!zc || n_hamil < 2 || n_hamil > kMaxComb + 1)
+    return fail(SALMON_HIP_EINVAL, "zc null or n_hamil=%d outside 2..%d", n_hamil, kMaxComb + 1);
+  if (!tspsi_in || !tspsi_out || !dv_dev || tspsi_in == tspsi_out)
+    return fail(SALMON_HIP_EINVAL, "taylor_reuse buffers invalid or aliased");
+  if (int rc = ready(c)) return rc;
+  const bool mixed = c->prec == SALMON_HIP_PREC_MIXED;
+  void *h1 = mixed ? (c->tscr32.empty() ? nullptr : (void *)c->tscr32[0]) : sshtpsi;
+  if (!h1 || (!mixed && (sshtpsi == tspsi_in || sshtpsi == tspsi_out)))
+    return fail(SALMON_HIP_ESTATE, "taylor_reuse needs H psi of a previous taylor call (sshtpsi, or the FP32 chain)");
+  const size_t ntot = c->osz * c->norb;
+  const dim3 grid((unsigned)((ntot + kBlock - 1) / kBlock));
+  if (mixed)
+    hipLaunchKernelGGL(k_vshift<float2>, grid, dim3(kBlock), 0, c->stream, (float2 *)h1, (const double2 *)tspsi_in,
+                       dv_dev, c->osz, c->nx, c->ny, c->ly, c->nz, c->nspin, ntot);
+  else
+    hipLaunchKernelGGL(k_vshift<double2>, grid, dim3(kBlock), 0, c->stream, (double2 *)h1, (const double2 *)tspsi_in,
+                       dv_dev, c->osz, c->nx, c->ny, c->ly, c->nz, c->nspin, ntot);
+  LAUNCHCHK("k_vshift");
+  if (mixed) return taylor_mixed(c, zc, n_hamil, tspsi_in, tspsi_out, 2);
+  return taylor_fp64(c, zc, n_hamil, tspsi_in, tspsi_out, sshtpsi, 2);
+}
+
+
 
 int salmon_hip_autotune(salmon_hip_t c, int reps, int *choice, double *best_ms) {
   if (int rc = check(c)) return rc;

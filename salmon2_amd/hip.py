@@ -29,6 +29,7 @@ EXPORTS = (
     "salmon_hip_set_vec_ac",
     "salmon_hip_hpsi",
     "salmon_hip_taylor",
+    "salmon_hip_taylor_reuse",
     "salmon_hip_set_precision",
     "salmon_hip_autotune",
     "salmon_hip_set_launch_shape",
@@ -94,6 +95,7 @@ def load():
     L.salmon_hip_set_vec_ac.argtypes = [C.c_void_p, _dp]
     L.salmon_hip_hpsi.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_taylor.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.salmon_hip_taylor_reuse.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_set_precision.argtypes = [C.c_void_p, C.c_int]
     L.salmon_hip_autotune.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
     L.salmon_hip_set_launch_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -215,6 +217,13 @@ class Handle:
         z = np.ascontiguousarray(np.asarray(zc, dtype=np.complex128)).view(np.float64)
         _chk(load().salmon_hip_taylor(self.h, _p(z), len(z) // 2, C.c_void_p(_ptr(tin)), C.c_void_p(_ptr(tout)),
                                       C.c_void_p(_ptr(work))), "taylor")
+
+    def taylor_reuse(self, zc, tin, tout, work, dv):
+        """The Taylor step of tin when only V_local changed since the last taylor call on tin:
+        H psi is updated by dv * psi (dv = V_now - V_then, device array) instead of recomputed."""
+        z = np.ascontiguousarray(np.asarray(zc, dtype=np.complex128)).view(np.float64)
+        _chk(load().salmon_hip_taylor_reuse(self.h, _p(z), len(z) // 2, C.c_void_p(_ptr(tin)), C.c_void_p(_ptr(tout)),
+                                            C.c_void_p(_ptr(work)), C.c_void_p(_ptr(dv))), "taylor_reuse")
 
     PREC = {"fp64": 0, "mixed": 1}
 

@@ -80,6 +80,7 @@ class RTPropagator:
             self.h.density(self.psi_a, self.rho)
             self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
             self.vmid = torch.zeros_like(self.vpsl)  # the predictor-corrector midpoint potential of the last step
+            self.dv = torch.zeros_like(self.vpsl)  # V_mid - V of the predictor (corrector from the predictor's H psi)
         # learned predictor (salmon2_amd.learned.Predictor): needs the self-consistent predictor-corrector step
         self.learned = learned
         if learned is not None:
@@ -121,7 +122,9 @@ class RTPropagator:
                 self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
                 self.h.density(self.psi_b, self.rho)
                 self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)
-            self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
+                self._corrector(self.vcur)
+            else:
+                self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
             self.h.density(self.psi_b, self.rho)
             if self.scf:
                 self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
@@ -138,6 +141,12 @@ class RTPropagator:
             self.ac_ind[itt + 1] = 2.0 * self.ac_ind[itt] - self.ac_ind[itt - 1] - 4.0 * math.pi * j * self.s.dt ** 2
             self.ac[itt + 1] = self.ac_ext[itt + 1] + self.ac_ind[itt + 1]
         self.psi_a, self.psi_b = self.psi_b, self.psi_a
+
+    def _corrector(self, v_pred):
+        """The corrector Taylor step with V_mid (the handle's current potential) after a
+        predictor with v_pred: its first H application is the predictor's plus (V_mid - v_pred) psi."""
+        torch.sub(self.vmid, v_pred, out=self.dv)
+        self.h.taylor_reuse(self.zc, self.psi_a, self.psi_b, self.work, self.dv)
 
     def _learned_step(self):
         """Predictor step by the network; True when the step is complete (accepted or fallen back)."""
@@ -159,13 +168,14 @@ class RTPropagator:
         if self.fallback == "reuse":
             # the attempt serves as the predictor: V = (V(t) + V[rho'])/2, then the corrector
             self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)
+            self._corrector(self.vpred)
         else:
             # the plain predictor: a Taylor step with V(t)
             self.h.set_vlocal_device(self.vcur)
             self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
             self.h.density(self.psi_b, self.rho)
             self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)
-        self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
+            self._corrector(self.vcur)
         self.h.density(self.psi_b, self.rho)
         self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
         return True

@@ -104,3 +104,34 @@ def test_rt_scf_run(torch_dev, oracle_lib, name, pc, nt):
     e_v = rel(vfin, v_ref)
     print(f"[parity] {name} final psi {e_psi:.3e} V_local {e_v:.3e}", flush=True)
     assert e_psi <= 1e-10 and e_v <= 1e-10
+
+
+@pytest.mark.parametrize("name,prec", [("c1", "fp64"), ("si64_32cube", "fp64"), ("c1", "mixed")])
+def test_taylor_reuse_equals_full_step(torch_dev, name, prec):
+    """The corrector's shortcut: H psi updated by (V2 - V1) psi, then the remaining terms,
+    equals a Taylor step computed from scratch with V2 (FP64: to round-off; mixed: within
+    the FP32 chain's own rounding)."""
+    torch = torch_dev
+    s = CASES[name]()
+    print(f"[gpu] taylor_reuse case={name} prec={prec}", flush=True)
+    from salmon2_amd.hip import Handle
+
+    rng = np.random.default_rng(8)
+    v1 = s.vlocal[0]
+    v2 = v1 + 0.05 * rng.standard_normal(v1.shape)
+    h = Handle(s, stream=torch.cuda.current_stream())
+    h.set_precision(prec)
+    h.update_kvector(np.array([0.01, 0.0, -0.02]))
+    psi = torch.from_numpy(s.psi).cuda()
+    out1, out2, w1, w2 = (torch.zeros_like(psi) for _ in range(4))
+    dv = torch.from_numpy(np.ascontiguousarray(v2 - v1)).cuda()
+    h.set_vlocal(v1)
+    h.taylor(s.zc(), psi, out1, w1)
+    h.set_vlocal(v2)
+    h.taylor_reuse(s.zc(), psi, out1, w1, dv)
+    h.taylor(s.zc(), psi, out2, w2)
+    torch.cuda.synchronize()
+    e = rel(out1.cpu().numpy()[..., : s.ny, :], out2.cpu().numpy()[..., : s.ny, :])
+    print(f"[parity] taylor_reuse {name} {prec}: rel err {e:.3e}", flush=True)
+    assert e <= (1e-13 if prec == "fp64" else 2e-7)
+    h.close()
