@@ -93,6 +93,12 @@ class RTPropagator:
             self.hist[0].copy_(self.vcur)
             self.nhist = 1
             self.vpred = torch.zeros_like(self.vpsl)
+            # the next step's inference runs on a side stream while this step's current and dipole
+            # kernels (memory-bound) run on the main one
+            self.side = torch.cuda.Stream(self.dev)
+            self.ev_v = torch.cuda.Event()
+            self.ev_pred = torch.cuda.Event()
+            self.pred_ready = False
             self.vnew = torch.zeros_like(self.vpsl)
             self.resid = torch.zeros(1, dtype=torch.float64, device=self.dev)
             self.accepted = 0
@@ -132,6 +138,15 @@ class RTPropagator:
             self.hist = [self.hist[3]] + self.hist[:3]
             self.hist[0].copy_(self.vcur)
             self.nhist = min(self.nhist + 1, 4)
+            if self.nhist >= 4:
+                P = self.learned
+                self.ev_v.record(self.stream)
+                self.side.wait_event(self.ev_v)
+                P.net.set_stream(self.side)
+                P.predict(self.hist, self.vpsl, self.vpred)
+                P.net.set_stream(self.stream)
+                self.ev_pred.record(self.side)
+                self.pred_ready = True
         if self.record is not None:
             self.record.append((self.vcur.clone(), self.vmid.clone()))
         self.h.current(self.ac[itt], self.psi_b, self.curr_t[itt - 1])
@@ -151,7 +166,11 @@ class RTPropagator:
     def _learned_step(self):
         """Predictor step by the network; True when the step is complete (accepted or fallen back)."""
         P = self.learned
-        P.predict(self.hist, self.vpsl, self.vpred)
+        if self.pred_ready:
+            self.stream.wait_event(self.ev_pred)
+            self.pred_ready = False
+        else:
+            P.predict(self.hist, self.vpsl, self.vpred)
         self.h.set_vlocal_device(self.vpred)
         self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
         self.h.density(self.psi_b, self.rho)

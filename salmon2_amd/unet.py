@@ -67,11 +67,15 @@ class UNet:
         self.h = h
         self.shape = (nz, ny, nx)
         if stream is not None:
-            raw = getattr(stream, "cuda_stream", stream)
-            hip._chk(L.salmon_unet_set_stream(self.h, C.c_void_p(raw or None)), "salmon_unet_set_stream")
+            self.set_stream(stream)
         n = C.c_long()
         hip._chk(L.salmon_unet_num_params(self.h, C.byref(n)), "salmon_unet_num_params")
         self.nparam = n.value
+
+    def set_stream(self, stream):
+        """Later launches go to `stream` (a torch stream or a raw handle; 0/None = the null stream)."""
+        raw = getattr(stream, "cuda_stream", stream)
+        hip._chk(_lib().salmon_unet_set_stream(self.h, C.c_void_p(raw or None)), "salmon_unet_set_stream")
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
