@@ -857,6 +857,173 @@ __device__ __forceinline__ void stencil_plane(const StencilArgs &a, int bid, dou
   wait_vm<0>();
 }
 
+// ---------------------------------------------------------------------------
+// Terms without the Taylor sum on 256-point y-tiles (TY rows, 4 + 4 halo rows, nx <= 32),
+// with every LDS address fixed at compile time: one ring slot holds a plane's TY + 8 rows
+// (tile and halo together, so y neighbours are row offsets in one slot), the ring has 8
+// slots and the z loop is unrolled by 8, so slot offsets become immediate offsets of the
+// LDS reads; z neighbours above come from the ring, those below from a 4-deep register
+// queue whose rotation is resolved by the unrolling (no moves).  Issue order per plane and
+// wave: 2 x W tile DMAs (plane z+7), 2 V dword DMAs (plane z+3), 1 store; the wait at the
+// top of a plane leaves the last two planes' operations in flight.  Needs TZ % 8 == 0.
+// Arithmetic and its order are those of stencil_tile.
+// ---------------------------------------------------------------------------
+template <int P> struct Ph { static constexpr int v = P; };
+
+template <typename TI, typename R, typename TO>
+__device__ __forceinline__ void stencil_tile8(const StencilArgs &a, int bid, double2 *lds) {
+  using CH = typename Cx<R>::T;
+  constexpr int BS = kTile;
+  constexpr int WI = DmaW<TI>::W;
+  constexpr unsigned EB = sizeof(TI);
+  constexpr int SLOT = 2 * BS;                // elements per slot: (TY + 8) * nx <= 512
+  constexpr int kOps = 2 * WI + 2 + 1;
+  TI *tileR = (TI *)lds;                      // 8 x SLOT
+  double *vR = (double *)(tileR + 8 * SLOT);  // 4 x BS
+
+  const int ytile = bid % a.nyt;
+  const int zt = (bid / a.nyt) % a.nzt;
+  const int b = bid / (a.nyt * a.nzt);
+  const int ik = b / a.nb;
+  const int ispin = b % a.nspin;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, ly = a.ly;
+  const int TY = a.TY;
+  const int y0 = ytile * TY;
+  const int z0 = zt * a.TZ;
+  const int z1 = min(z0 + a.TZ, nz);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int ntile = TY * nx;
+  const bool active = tid < ntile;
+  const int te = active ? tid : 0;
+  const int tx = te % nx, ty = te / nx;
+  const int y = y0 + ty;
+  const bool outY = active && y < ny;
+  const size_t zs = (size_t)nx * ly;
+  const size_t orbOff = (size_t)b * a.osz;
+  const TI *in = (const TI *)a.in + orbOff;
+  const size_t colOff = (size_t)tx + (size_t)nx * wrapi(y, ny);
+  const size_t gplane = (size_t)nx * ny;
+  const int nslot = (TY + 8) * nx;
+  // slot element e -> in-plane source offset (row r = global row y0 - 4 + r)
+  auto sofs = [&](int e) {
+    const int ec = e < nslot ? e : 0;
+    return (size_t)(ec % nx) + (size_t)nx * wrapi(y0 - 4 + ec / nx, ny);
+  };
+  size_t toff[2][2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+      toff[k][kk] = sofs(64 * (wave + 4 * k) + (WI == 1 ? lane : 32 * kk + (lane >> 1)));
+  size_t voff[2];
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk) {
+    const int e = 64 * wave + 32 * kk + (lane >> 1);
+    const int ec = e < ntile ? e : 0;
+    voff[kk] = 2 * ((size_t)(ec % nx) + (size_t)nx * wrapi(y0 + ec / nx, ny)) + (lane & 1);
+  }
+  const double *Vb = a.V + (size_t)ispin * gplane * nz;
+  const unsigned tileA = __builtin_amdgcn_readfirstlane(lds_addr(tileR) + EB * 64u * wave);
+  const unsigned vA = __builtin_amdgcn_readfirstlane(lds_addr(vR) + 8u * 64u * wave);
+  auto dma_tile = [&](int plane, int slot) {
+    const TI *src = in + zs * wrapi(plane, nz);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) dma_chunk<TI>(src, toff[k], tileA + EB * (unsigned)(SLOT * slot + 4 * 64 * k), lane);
+  };
+  auto dma_v = [&](int plane, int slot) {
+    const float *vb = (const float *)Vb + 2 * gplane * wrapi(plane, nz);
+    dma4(vb + voff[0], vA + 8u * BS * slot);
+    dma4(vb + voff[1], vA + 8u * BS * slot + 256u);
+  };
+  // element indices in a slot: own point, x and y neighbours
+  const int own = (ty + 4) * nx + tx;
+  int xo[8], yo[8];
+#pragma unroll
+  for (int n = 1; n <= 4; ++n) {
+    xo[n - 1] = (ty + 4) * nx + wrapi(tx + n, nx);
+    xo[3 + n] = (ty + 4) * nx + wrapi(tx - n, nx);
+    yo[n - 1] = (ty + 4 + n) * nx + tx;
+    yo[3 + n] = (ty + 4 - n) * nx + tx;
+  }
+  const double *kp = a.kpar + 16 * ik;
+  const R klap0 = (R)kp[0];
+  R nab[12], lap[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    nab[i] = (R)kp[1 + i];
+    lap[i] = (R)a.lap[i];
+  }
+  CH q[4];  // planes z-4 .. z-1 at the own point; plane z-4+k sits in q[(phase + k) & 3]
+#pragma unroll
+  for (int i = 0; i < 4; ++i) q[i] = cvt2<CH>(in[colOff + zs * wrapi(z0 - 4 + i, nz)]);
+  __builtin_amdgcn_s_waitcnt(0);
+  for (int p = 0; p < 7; ++p) dma_tile(z0 + p, p);
+  for (int p = 0; p < 3; ++p) dma_v(z0 + p, p);
+  wait_vm<0>();
+
+  auto body = [&](int z, auto ph) {
+    constexpr int P = decltype(ph)::v;
+    wait_vm<2 * kOps>();
+    __syncthreads();
+    dma_tile(min(z + 7, z1 + 3), (P + 7) & 7);
+    dma_v(min(z + 3, z1 - 1), (P + 3) & 3);
+    const TI *T = tileR + SLOT * P;
+    CH v, w;
+    {
+      const CH p1 = cvt2<CH>(T[xo[0]]), p2 = cvt2<CH>(T[xo[1]]), p3 = cvt2<CH>(T[xo[2]]), p4 = cvt2<CH>(T[xo[3]]);
+      const CH m1 = cvt2<CH>(T[xo[4]]), m2 = cvt2<CH>(T[xo[5]]), m3 = cvt2<CH>(T[xo[6]]), m4 = cvt2<CH>(T[xo[7]]);
+      v.x = lap[0] * (p1.x + m1.x) + lap[1] * (p2.x + m2.x) + lap[2] * (p3.x + m3.x) + lap[3] * (p4.x + m4.x);
+      v.y = lap[0] * (p1.y + m1.y) + lap[1] * (p2.y + m2.y) + lap[2] * (p3.y + m3.y) + lap[3] * (p4.y + m4.y);
+      w.x = nab[0] * (p1.x - m1.x) + nab[1] * (p2.x - m2.x) + nab[2] * (p3.x - m3.x) + nab[3] * (p4.x - m4.x);
+      w.y = nab[0] * (p1.y - m1.y) + nab[1] * (p2.y - m2.y) + nab[2] * (p3.y - m3.y) + nab[3] * (p4.y - m4.y);
+    }
+    {
+      const CH p1 = cvt2<CH>(T[yo[0]]), p2 = cvt2<CH>(T[yo[1]]), p3 = cvt2<CH>(T[yo[2]]), p4 = cvt2<CH>(T[yo[3]]);
+      const CH m1 = cvt2<CH>(T[yo[4]]), m2 = cvt2<CH>(T[yo[5]]), m3 = cvt2<CH>(T[yo[6]]), m4 = cvt2<CH>(T[yo[7]]);
+      v.x += lap[4] * (p1.x + m1.x) + lap[5] * (p2.x + m2.x) + lap[6] * (p3.x + m3.x) + lap[7] * (p4.x + m4.x);
+      v.y += lap[4] * (p1.y + m1.y) + lap[5] * (p2.y + m2.y) + lap[6] * (p3.y + m3.y) + lap[7] * (p4.y + m4.y);
+      w.x += nab[4] * (p1.x - m1.x) + nab[5] * (p2.x - m2.x) + nab[6] * (p3.x - m3.x) + nab[7] * (p4.x - m4.x);
+      w.y += nab[4] * (p1.y - m1.y) + nab[5] * (p2.y - m2.y) + nab[6] * (p3.y - m3.y) + nab[7] * (p4.y - m4.y);
+    }
+    const CH t = cvt2<CH>(T[own]);
+    const CH u1 = cvt2<CH>(tileR[SLOT * ((P + 1) & 7) + own]), u2 = cvt2<CH>(tileR[SLOT * ((P + 2) & 7) + own]);
+    const CH u3 = cvt2<CH>(tileR[SLOT * ((P + 3) & 7) + own]), u4 = cvt2<CH>(tileR[SLOT * ((P + 4) & 7) + own]);
+    const CH d1 = q[(P + 3) & 3], d2 = q[(P + 2) & 3], d3 = q[(P + 1) & 3], d4 = q[P & 3];
+    v.x += lap[8] * (u1.x + d1.x) + lap[9] * (u2.x + d2.x) + lap[10] * (u3.x + d3.x) + lap[11] * (u4.x + d4.x);
+    v.y += lap[8] * (u1.y + d1.y) + lap[9] * (u2.y + d2.y) + lap[10] * (u3.y + d3.y) + lap[11] * (u4.y + d4.y);
+    w.x += nab[8] * (u1.x - d1.x) + nab[9] * (u2.x - d2.x) + nab[10] * (u3.x - d3.x) + nab[11] * (u4.x - d4.x);
+    w.y += nab[8] * (u1.y - d1.y) + nab[9] * (u2.y - d2.y) + nab[10] * (u3.y - d3.y) + nab[11] * (u4.y - d4.y);
+    const R vcur = (R)vR[BS * (P & 3) + te];
+    CH hh;
+    hh.x = (vcur * t.x + klap0 * t.x) - (R)0.5 * v.x + w.y;
+    hh.y = (vcur * t.y + klap0 * t.y) - (R)0.5 * v.y - w.x;
+    TO *dst = outY ? (TO *)a.hout + orbOff + colOff + zs * z : (TO *)a.sink + tid;
+    *dst = cvt2<TO>(hh);
+    q[P & 3] = t;
+  };
+  for (int z = z0; z < z1; z += 8) {
+    body(z, Ph<0>());
+    body(z + 1, Ph<1>());
+    body(z + 2, Ph<2>());
+    body(z + 3, Ph<3>());
+    body(z + 4, Ph<4>());
+    body(z + 5, Ph<5>());
+    body(z + 6, Ph<6>());
+    body(z + 7, Ph<7>());
+  }
+  wait_vm<0>();
+}
+
+template <typename TI, typename R, typename TO>
+__global__ __launch_bounds__(kTile) void k_stencil8(StencilArgs a) {
+  extern __shared__ double2 lds[];
+  int bid = blockIdx.x;
+  const int nfull = (a.nblk / 8) * 8;
+  if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;  // XCD-aware tile order
+  stencil_tile8<TI, R, TO>(a, bid, lds);
+}
+
 template <typename TI, typename R, typename TO, int HC, int NC, int BS>
 __global__ __launch_bounds__(BS) void k_stencil(StencilArgs a) {
   extern __shared__ double2 lds[];
@@ -1475,6 +1642,7 @@ struct salmon_hip_ctx {
   // launch shape of the terms without the Taylor sum (salmon_hip_autotune): tile_plane -1 = rule
   // (whole plane when it fits 256 threads), 0 = y tiles, 1 = whole plane; mid_nzt 0 = rule
   int tile_plane = -1, mid_nzt = 0;
+  bool tile8 = true;  // stencil_tile8 for the mixed middle terms where the shape allows it
   // kernel timing (salmon_hip_prof_*): start/stop event pairs per launch
   bool prof = false;
   std::vector<hipEvent_t> ev;
@@ -1887,6 +2055,26 @@ static int launch_stencil_wp(salmon_hip_ctx *c, int stage, dim3 grid, hipEvent_t
   return launch_plane_t<double2, double, double2, PY>(c, grid, e0, e1, a);
 }
 
+// y tiles with the merged halo and compile-time ring slots (stencil_tile8): the middle terms of
+// the mixed chain, rows of at most 32 points, (TY + 8) * nx <= 512, z chunks of 8k planes.
+static bool tile8_fits(const salmon_hip_ctx *c, int stage, const StencilArgs &a) {
+  return stage == kStageMixedMiddle && a.nx <= 32 && (a.TY + 8) * a.nx <= 2 * kTile && a.TY * a.nx <= kTile &&
+         a.TZ % 8 == 0 && a.nz % 8 == 0 && c->tile8;
+}
+
+template <typename TI, typename R, typename TO>
+static int launch_tile8_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
+  const size_t shmem = sizeof(TI) * 8 * 2 * kTile + sizeof(double) * 4 * kTile;
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencil8<TI, R, TO>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)shmem));
+    attr_set = true;
+  }
+  hipExtLaunchKernelGGL((k_stencil8<TI, R, TO>), grid, dim3(kTile), shmem, c->stream, e0, e1, 0, a);
+  return 0;
+}
+
 template <int HC>
 static int launch_stencil_hc(salmon_hip_ctx *c, int stage, int nc, dim3 grid, hipEvent_t e0, hipEvent_t e1,
                              const StencilArgs &a) {
@@ -2007,6 +2195,7 @@ static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, boo
   if (bs == 256) rc = launch_stencil_plane<256>(c, stage, grid, e0, e1, a);
   else if (bs == 512) rc = launch_stencil_wp<1>(c, stage, grid, e0, e1, a);
   else if (bs == 1024) rc = launch_stencil_wp<2>(c, stage, grid, e0, e1, a);
+  else if (tile8_fits(c, stage, a)) rc = launch_tile8_t<float2, float, float2>(c, grid, e0, e1, a);
   else if (hc <= 1) rc = launch_stencil_hc<1>(c, stage, nc, grid, e0, e1, a);
   else if (hc <= 2) rc = launch_stencil_hc<2>(c, stage, nc, grid, e0, e1, a);
   else if (hc <= 4) rc = launch_stencil_hc<4>(c, stage, nc, grid, e0, e1, a);
