@@ -2057,8 +2057,8 @@ static int launch_stencil_wp(salmon_hip_ctx *c, int stage, dim3 grid, hipEvent_t
 
 // y tiles with the merged halo and compile-time ring slots (stencil_tile8): the middle terms of
 // the mixed chain, rows of at most 32 points, (TY + 8) * nx <= 512, z chunks of 8k planes.
-static bool tile8_fits(const salmon_hip_ctx *c, int stage, const StencilArgs &a) {
-  return stage == kStageMixedMiddle && a.nx <= 32 && (a.TY + 8) * a.nx <= 2 * kTile && a.TY * a.nx <= kTile &&
+static bool tile8_fits(const salmon_hip_ctx *c, int stage, int nc, const StencilArgs &a) {
+  return (stage == kStageMixedMiddle || (stage == kStagePlain && nc < 0)) && a.nx <= 32 && (a.TY + 8) * a.nx <= 2 * kTile && a.TY * a.nx <= kTile &&
          a.TZ % 8 == 0 && a.nz % 8 == 0 && c->tile8;
 }
 
@@ -2195,7 +2195,9 @@ static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, boo
   if (bs == 256) rc = launch_stencil_plane<256>(c, stage, grid, e0, e1, a);
   else if (bs == 512) rc = launch_stencil_wp<1>(c, stage, grid, e0, e1, a);
   else if (bs == 1024) rc = launch_stencil_wp<2>(c, stage, grid, e0, e1, a);
-  else if (tile8_fits(c, stage, a)) rc = launch_tile8_t<float2, float, float2>(c, grid, e0, e1, a);
+  else if (tile8_fits(c, stage, nc, a))
+    rc = stage == kStagePlain ? launch_tile8_t<double2, double, double2>(c, grid, e0, e1, a)
+                              : launch_tile8_t<float2, float, float2>(c, grid, e0, e1, a);
   else if (hc <= 1) rc = launch_stencil_hc<1>(c, stage, nc, grid, e0, e1, a);
   else if (hc <= 2) rc = launch_stencil_hc<2>(c, stage, nc, grid, e0, e1, a);
   else if (hc <= 4) rc = launch_stencil_hc<4>(c, stage, nc, grid, e0, e1, a);

@@ -42,6 +42,7 @@ CASES = {
     "si64_32cube": lambda: system.make_si_system(ncell=(2, 2, 2), n=32, no=12, nkgrid=(1, 1, 2), seed=11),
     "ragged_20x26x28": lambda: system.make_si_system(n=(20, 26, 28), no=5, nkgrid=(1, 1, 1), seed=13),
     "si_48x40x36_nopad": lambda: system.make_si_system(ncell=(1, 1, 1), n=(48, 40, 36), no=3, pad=0, seed=17),
+    "ragged_24x22x16": lambda: system.make_si_system(n=(24, 22, 16), no=6, nkgrid=(1, 1, 1), seed=19),
 }
 
 

@@ -40,6 +40,8 @@ def _cases():
         "si64_32cube": lambda: system.make_si_system(ncell=(2, 2, 2), n=32, no=12, nkgrid=(1, 1, 2), seed=11),
         "ragged_20x26x28": lambda: system.make_si_system(n=(20, 26, 28), no=5, nkgrid=(1, 1, 1), seed=13),
         "si_48x40x36_nopad": lambda: system.make_si_system(ncell=(1, 1, 1), n=(48, 40, 36), no=3, pad=0, seed=17),
+        # 10-row y tiles with a partial last tile (rows 20-21), z chunks of 8 and 16 planes
+        "ragged_24x22x16": lambda: system.make_si_system(n=(24, 22, 16), no=6, nkgrid=(1, 1, 1), seed=19),
     }
 
 
@@ -167,7 +169,7 @@ def test_rt_c1_full_run(torch_dev, oracle_lib):
     assert np.abs(cur_ref).max() > 1e-6
 
 
-@pytest.mark.parametrize("name", ["si64_32cube", "ragged_20x26x28", "c1"])
+@pytest.mark.parametrize("name", ["si64_32cube", "ragged_20x26x28", "ragged_24x22x16", "c1"])
 def test_launch_shapes_keep_parity(torch_dev, oracle_lib, name):
     """Every stencil launch shape the autotuner may pick gives the same hpsi and Taylor step."""
     torch = torch_dev
