@@ -18,6 +18,9 @@ Forms timed on the same initial state (each for warmup + steps RT steps):
                self-consistency residual, else the plain predictor-corrector step
   learned-mixed-reuse  the same, but a rejected attempt serves as the predictor
                (one corrector step instead of the plain predictor-corrector)
+  learned-mixed-backoff  learned-mixed-reuse that, after 8 rejected attempts in a row,
+               attempts once every 32 steps until one is accepted (plain
+               predictor-corrector steps in between)
   learned-mixed-open   every predicted step accepted (no gate): the speed the learned
                predictor offers and the error it costs on this workload
 The batched form (salmon2_amd/batch.py) runs separately on 24 seeded uneven systems
@@ -28,7 +31,7 @@ over the whole run.  `value` is the steps/s of the fastest learned+mixed form wh
 current stays within 1e-6 (relative, max over the run) of pc-fp64's.
 
 Prints ONE JSON line.  The roofline entry is for the dominant kernel, the
-stencil of the middle Taylor terms (k_stencil) in pc-fp64, timed live with
+stencil of the middle Taylor terms (k_stencil8) in pc-fp64, timed live with
 start/stop events recorded by its own launches over the timed region.
 """
 from __future__ import annotations
@@ -51,11 +54,13 @@ FORMS = {
     "pc-mixed": dict(scf=True, pc=True, precision="mixed"),
     "learned-mixed": dict(scf=True, pc=True, precision="mixed", learned=True),
     "learned-mixed-reuse": dict(scf=True, pc=True, precision="mixed", learned=True, fallback="reuse"),
+    "learned-mixed-backoff": dict(scf=True, pc=True, precision="mixed", learned=True, fallback="reuse",
+                                  backoff=(8, 32)),
     "learned-mixed-open": dict(scf=True, pc=True, precision="mixed", learned=True, tol=float("inf")),
 }
 BASE_FORM = "pc-fp64"
 # `value` is the fastest learned+mixed form whose current stays within CUR_BAR of pc-fp64's
-HEADLINE_FORMS = ("learned-mixed", "learned-mixed-reuse", "learned-mixed-open")
+HEADLINE_FORMS = ("learned-mixed", "learned-mixed-reuse", "learned-mixed-backoff", "learned-mixed-open")
 CUR_BAR = 1e-6
 
 
@@ -302,12 +307,13 @@ def main():
         "forms": form_out,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_stencil (middle Taylor terms of pc-fp64: 25-point stencil + V + k.grad, LDS-DMA fed)",
+            "kernel": "k_stencil8 (middle Taylor terms of pc-fp64: 25-point stencil + V + k.grad on y tiles "
+                      "with merged halo rows, LDS-DMA fed)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.config, "k_stencil<d2,double,d2,1,-1,256>"),
+            "traffic": measured_traffic(args.config, "k_stencil8<d2,double,d2>"),
             "alg_bytes_per_launch": alg_bytes,
             "avg_launch_ms": kern["hpsi_taylor"]["avg_ms"],
         },

@@ -1642,7 +1642,7 @@ struct salmon_hip_ctx {
   // launch shape of the terms without the Taylor sum (salmon_hip_autotune): tile_plane -1 = rule
   // (whole plane when it fits 256 threads), 0 = y tiles, 1 = whole plane; mid_nzt 0 = rule
   int tile_plane = -1, mid_nzt = 0;
-  bool tile8 = true;  // stencil_tile8 for the mixed middle terms where the shape allows it
+  bool tile8 = true;  // stencil_tile8 for the terms without the Taylor sum where the shape allows it
   // kernel timing (salmon_hip_prof_*): start/stop event pairs per launch
   bool prof = false;
   std::vector<hipEvent_t> ev;

@@ -25,7 +25,9 @@ step by the network's midpoint potential: one Taylor step with it, accepted
 when the self-consistency residual max|V_pred - (V(t) + V(t+dt))/2| is within
 the predictor's tolerance; otherwise ``fallback="plain"`` redoes the step
 with the plain predictor-corrector and ``"reuse"`` takes the attempt as the
-predictor (one corrector Taylor step).
+predictor (one corrector Taylor step).  ``backoff=(n, period)`` stops attempting after n
+rejections in a row and then attempts once every ``period`` steps until one is accepted (the
+steps in between are plain predictor-corrector steps); None attempts every step.
 
 ``precision="mixed"`` runs the Taylor chain H^n psi in complex FP32 with the
 projections, the Taylor sum, density and current in FP64.
@@ -52,7 +54,7 @@ from .hip import Handle
 
 class RTPropagator:
     def __init__(self, sysm, device="cuda:0", psi=None, scf=False, pc=False, precision="fp64", trans_longi="tr",
-                 learned=None, fallback="plain", record=False):
+                 learned=None, fallback="plain", record=False, backoff=None):
         self.s = sysm
         self.dev = torch.device(device)
         torch.cuda.set_device(self.dev)
@@ -104,6 +106,11 @@ class RTPropagator:
             self.accepted = 0
             self.attempted = 0
             self.resid_log = []
+            if backoff is not None and (int(backoff[0]) < 1 or int(backoff[1]) < 1):
+                raise ValueError(f"backoff {backoff!r}: (rejections >= 1, period >= 1)")
+            self.backoff = None if backoff is None else (int(backoff[0]), int(backoff[1]))
+            self.fails = 0  # rejections since the last accepted attempt
+            self.last_attempt = 0
         self.record = [] if record else None
 
     def setup_outputs(self, nt):
@@ -121,7 +128,7 @@ class RTPropagator:
         itt = self.itt
         A = 0.5 * (self.ac[itt] + self.ac[itt - 1])
         self.h.update_kvector(A)
-        if self.learned is not None and self.nhist >= 4 and self._learned_step():
+        if self.learned is not None and self.nhist >= 4 and self._attempt_due(itt) and self._learned_step():
             pass
         else:
             if self.pc:
@@ -138,7 +145,7 @@ class RTPropagator:
             self.hist = [self.hist[3]] + self.hist[:3]
             self.hist[0].copy_(self.vcur)
             self.nhist = min(self.nhist + 1, 4)
-            if self.nhist >= 4:
+            if self.nhist >= 4 and self._attempt_due(itt + 1):
                 P = self.learned
                 self.ev_v.record(self.stream)
                 self.side.wait_event(self.ev_v)
@@ -163,9 +170,17 @@ class RTPropagator:
         torch.sub(self.vmid, v_pred, out=self.dv)
         self.h.taylor_reuse(self.zc, self.psi_a, self.psi_b, self.work, self.dv)
 
+    def _attempt_due(self, itt):
+        """Whether step itt tries the learned predictor (every step, or under back-off once
+        every period steps after the set number of rejections in a row)."""
+        if self.backoff is None or self.fails < self.backoff[0]:
+            return True
+        return itt - self.last_attempt >= self.backoff[1]
+
     def _learned_step(self):
         """Predictor step by the network; True when the step is complete (accepted or fallen back)."""
         P = self.learned
+        self.last_attempt = self.itt
         if self.pred_ready:
             self.stream.wait_event(self.ev_pred)
             self.pred_ready = False
@@ -181,9 +196,11 @@ class RTPropagator:
         self.resid_log.append(r)
         if r <= getattr(P, "tol_run", P.tol):
             self.accepted += 1
+            self.fails = 0
             self.vmid.copy_(self.vpred)
             self.vcur, self.vnew = self.vnew, self.vcur
             return True
+        self.fails += 1
         if self.fallback == "reuse":
             # the attempt serves as the predictor: V = (V(t) + V[rho'])/2, then the corrector
             self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)

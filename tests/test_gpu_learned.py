@@ -5,7 +5,8 @@
   side stream while the current kernels run; it must not disturb anything);
 * gate open (tol = inf): every step after the four-step history is accepted, the run is
   reproducible bit for bit, and an accepted step equals one Taylor step of the oracle with
-  the predicted potential (to the plain path's 1e-12)."""
+  the predicted potential (to the plain path's 1e-12);
+* back-off: attempts only on the steps its schedule names, the others plain steps."""
 import dataclasses
 
 import numpy as np
@@ -36,11 +37,11 @@ def _predictor(torch, s, tol):
     return P
 
 
-def _run(torch, s, tol, fallback="plain", record=False):
+def _run(torch, s, tol, fallback="plain", record=False, backoff=None):
     from salmon2_amd.rt import RTPropagator
 
     rt = RTPropagator(s, scf=True, pc=True, precision="fp64", learned=_predictor(torch, s, tol), fallback=fallback,
-                      record=record)
+                      record=record, backoff=backoff)
     rt.setup_outputs(NT)
     snaps = []
     for _ in range(NT):
@@ -62,6 +63,21 @@ def test_closed_gate_equals_plain_pc(torch_dev):
         d = float(np.abs(res[k] - ref[k]).max())
         print(f"[parity] learned-closed {k}: max abs diff {d:.3e}", flush=True)
         assert d == 0.0, k
+
+
+def test_backoff_attempts_and_equals_plain_pc(torch_dev):
+    """Back-off (2 rejections, then every 3 steps): attempts at steps 4, 5 and 8 of 10; every
+    step, attempted or not, is the plain predictor-corrector step."""
+    torch = torch_dev
+    from salmon2_amd.rt import RTPropagator
+
+    s = system.config("c1", nt=NT)
+    print("[gpu] learned back-off, gate closed", flush=True)
+    ref = RTPropagator(s, scf=True, pc=True).run(NT)
+    rt, res, _ = _run(torch, s, tol=0.0, backoff=(2, 3))
+    assert rt.attempted == 3 and rt.accepted == 0 and rt.last_attempt == 8
+    for k in ("curr", "dip", "rne"):
+        assert np.array_equal(res[k], ref[k]), k
 
 
 def test_open_gate_reproducible_and_accepted_step_matches_oracle(torch_dev, oracle_lib):
