@@ -218,8 +218,13 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
   __shared__ double2 sp[kProjJC][kProjOrb];
   __shared__ double2 zsh[kProjJC][kMaxLmaPerAtom];
   __shared__ double rsh[kProjJC][3];
-  const int grp = blockIdx.x % a.ngrp;
-  const int t = blockIdx.x / a.ngrp;
+  // XCD-aware order: the blocks of one XCD take a contiguous range of (group, atom, k), so the
+  // cache lines that neighbouring spheres share are read through one L2
+  int bid = blockIdx.x;
+  const int nfull = ((int)gridDim.x / 8) * 8;
+  if (bid < nfull) bid = (bid % 8) * ((int)gridDim.x / 8) + bid / 8;
+  const int grp = bid % a.ngrp;
+  const int t = bid / a.ngrp;
   const int ia = t % a.natom;
   const int ik = t / a.natom;
   const int2 al = a.atom_lma[ia];
