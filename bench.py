@@ -31,7 +31,7 @@ over the whole run.  `value` is the steps/s of the fastest learned+mixed form wh
 current stays within 1e-6 (relative, max over the run) of pc-fp64's.
 
 Prints ONE JSON line.  The roofline entry is for the dominant kernel, the
-stencil of the middle Taylor terms (k_stencil8) in pc-fp64, timed live with
+stencil of the middle Taylor terms (k_stencilz) in pc-fp64, timed live with
 start/stop events recorded by its own launches over the timed region.
 """
 from __future__ import annotations
@@ -307,13 +307,13 @@ def main():
         "forms": form_out,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_stencil8 (middle Taylor terms of pc-fp64: 25-point stencil + V + k.grad on y tiles "
-                      "with merged halo rows, LDS-DMA fed)",
+            "kernel": "k_stencilz (middle Taylor terms of pc-fp64: 25-point stencil + V + k.grad on y tiles "
+                      "with merged halo rows and the z part formed 4 planes late, LDS-DMA fed)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.config, "k_stencil8<d2,double,d2>"),
+            "traffic": measured_traffic(args.config, "k_stencilz<d2,double,d2>"),
             "alg_bytes_per_launch": alg_bytes,
             "avg_launch_ms": kern["hpsi_taylor"]["avg_ms"],
         },

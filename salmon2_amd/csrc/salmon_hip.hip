@@ -1020,6 +1020,184 @@ __device__ __forceinline__ void stencil_tile8(const StencilArgs &a, int bid, dou
   wait_vm<0>();
 }
 
+// ---------------------------------------------------------------------------
+// The same terms with the z part of each output plane formed 4 planes late, so that a plane's
+// LDS slot (tile and halo rows) is needed only while that plane is the current one: the ring
+// has 4 slots (plane p read, p+1 and p+2 in flight, p+3 being written) and the block takes
+// 40 KB of LDS in FP64 instead of 72 KB (four blocks per CU instead of two, twice the bytes in
+// flight).  At plane p the kernel forms the x and y part and the centre term of plane p from
+// the slot (kept in a 4-deep register queue), pushes p's own value into a 9-deep queue of own
+// values (planes p-8 .. p) and completes plane p-4 with its z part.  The centre, x/y and z
+// partial sums are added in that order (hh = ((V t + klap0 t) - v_xy/2 + i w_xy) - v_z/2 + i w_z),
+// a regrouping of stencil_tile's sum.  Needs TZ % 4 == 0; the z loop runs TZ + 4 planes,
+// unrolled by 4 so that slot and queue indices are compile-time.
+// ---------------------------------------------------------------------------
+template <typename TI, typename R, typename TO>
+__device__ __forceinline__ void stencil_tilez(const StencilArgs &a, int bid, double2 *lds) {
+  using CH = typename Cx<R>::T;
+  constexpr int BS = kTile;
+  constexpr int WI = DmaW<TI>::W;
+  constexpr unsigned EB = sizeof(TI);
+  constexpr int SLOT = 2 * BS;                // elements per slot: (TY + 8) * nx <= 512
+  constexpr int kOps = 2 * WI + 2 + 1;
+  TI *tileR = (TI *)lds;                      // 4 x SLOT
+  double *vR = (double *)(tileR + 4 * SLOT);  // 4 x BS
+
+  const int ytile = bid % a.nyt;
+  const int zt = (bid / a.nyt) % a.nzt;
+  const int b = bid / (a.nyt * a.nzt);
+  const int ik = b / a.nb;
+  const int ispin = b % a.nspin;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, ly = a.ly;
+  const int TY = a.TY;
+  const int y0 = ytile * TY;
+  const int z0 = zt * a.TZ;
+  const int z1 = min(z0 + a.TZ, nz);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int ntile = TY * nx;
+  const bool active = tid < ntile;
+  const int te = active ? tid : 0;
+  const int tx = te % nx, ty = te / nx;
+  const int y = y0 + ty;
+  const bool outY = active && y < ny;
+  const size_t zs = (size_t)nx * ly;
+  const size_t orbOff = (size_t)b * a.osz;
+  const TI *in = (const TI *)a.in + orbOff;
+  const size_t colOff = (size_t)tx + (size_t)nx * wrapi(y, ny);
+  const size_t gplane = (size_t)nx * ny;
+  const int nslot = (TY + 8) * nx;
+  auto sofs = [&](int e) {
+    const int ec = e < nslot ? e : 0;
+    return (size_t)(ec % nx) + (size_t)nx * wrapi(y0 - 4 + ec / nx, ny);
+  };
+  size_t toff[2][2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+      toff[k][kk] = sofs(64 * (wave + 4 * k) + (WI == 1 ? lane : 32 * kk + (lane >> 1)));
+  size_t voff[2];
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk) {
+    const int e = 64 * wave + 32 * kk + (lane >> 1);
+    const int ec = e < ntile ? e : 0;
+    voff[kk] = 2 * ((size_t)(ec % nx) + (size_t)nx * wrapi(y0 + ec / nx, ny)) + (lane & 1);
+  }
+  const double *Vb = a.V + (size_t)ispin * gplane * nz;
+  const unsigned tileA = __builtin_amdgcn_readfirstlane(lds_addr(tileR) + EB * 64u * wave);
+  const unsigned vA = __builtin_amdgcn_readfirstlane(lds_addr(vR) + 8u * 64u * wave);
+  auto dma_tile = [&](int plane, int slot) {
+    const TI *src = in + zs * wrapi(plane, nz);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) dma_chunk<TI>(src, toff[k], tileA + EB * (unsigned)(SLOT * slot + 4 * 64 * k), lane);
+  };
+  auto dma_v = [&](int plane, int slot) {
+    const float *vb = (const float *)Vb + 2 * gplane * wrapi(plane, nz);
+    dma4(vb + voff[0], vA + 8u * BS * slot);
+    dma4(vb + voff[1], vA + 8u * BS * slot + 256u);
+  };
+  const int own = (ty + 4) * nx + tx;
+  int xo[8], yo[8];
+#pragma unroll
+  for (int n = 1; n <= 4; ++n) {
+    xo[n - 1] = (ty + 4) * nx + wrapi(tx + n, nx);
+    xo[3 + n] = (ty + 4) * nx + wrapi(tx - n, nx);
+    yo[n - 1] = (ty + 4 + n) * nx + tx;
+    yo[3 + n] = (ty + 4 - n) * nx + tx;
+  }
+  const double *kp = a.kpar + 16 * ik;
+  const R klap0 = (R)kp[0];
+  R nab[12], lap[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    nab[i] = (R)kp[1 + i];
+    lap[i] = (R)a.lap[i];
+  }
+  CH q[9];   // own values of planes p-8 .. p (q[8] = p) after the push at plane p
+  CH hq[4];  // centre + x/y part of planes p-3 .. p; plane p's sits in hq[(p - z0) & 3]
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    q[i] = cvt2<CH>(make_double2(0.0, 0.0));
+    hq[i] = q[i];
+    q[5 + i] = cvt2<CH>(in[colOff + zs * wrapi(z0 - 4 + i, nz)]);
+  }
+  q[4] = q[0];
+  __builtin_amdgcn_s_waitcnt(0);
+  for (int p = 0; p < 3; ++p) {
+    dma_tile(z0 + p, p);
+    dma_v(z0 + p, p);
+  }
+  wait_vm<0>();
+
+  auto body = [&](int p, auto ph) {
+    constexpr int P = decltype(ph)::v;
+    wait_vm<2 * kOps>();
+    __syncthreads();
+    dma_tile(min(p + 3, z1 + 3), (P + 3) & 3);
+    dma_v(min(p + 3, z1 - 1), (P + 3) & 3);
+    const TI *T = tileR + SLOT * P;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) q[i] = q[i + 1];
+    q[8] = cvt2<CH>(T[own]);
+    const CH hp = hq[P];  // plane p - 4
+    if (p < z1) {
+      CH v, w;
+      {
+        const CH p1 = cvt2<CH>(T[xo[0]]), p2 = cvt2<CH>(T[xo[1]]), p3 = cvt2<CH>(T[xo[2]]), p4 = cvt2<CH>(T[xo[3]]);
+        const CH m1 = cvt2<CH>(T[xo[4]]), m2 = cvt2<CH>(T[xo[5]]), m3 = cvt2<CH>(T[xo[6]]), m4 = cvt2<CH>(T[xo[7]]);
+        v.x = lap[0] * (p1.x + m1.x) + lap[1] * (p2.x + m2.x) + lap[2] * (p3.x + m3.x) + lap[3] * (p4.x + m4.x);
+        v.y = lap[0] * (p1.y + m1.y) + lap[1] * (p2.y + m2.y) + lap[2] * (p3.y + m3.y) + lap[3] * (p4.y + m4.y);
+        w.x = nab[0] * (p1.x - m1.x) + nab[1] * (p2.x - m2.x) + nab[2] * (p3.x - m3.x) + nab[3] * (p4.x - m4.x);
+        w.y = nab[0] * (p1.y - m1.y) + nab[1] * (p2.y - m2.y) + nab[2] * (p3.y - m3.y) + nab[3] * (p4.y - m4.y);
+      }
+      {
+        const CH p1 = cvt2<CH>(T[yo[0]]), p2 = cvt2<CH>(T[yo[1]]), p3 = cvt2<CH>(T[yo[2]]), p4 = cvt2<CH>(T[yo[3]]);
+        const CH m1 = cvt2<CH>(T[yo[4]]), m2 = cvt2<CH>(T[yo[5]]), m3 = cvt2<CH>(T[yo[6]]), m4 = cvt2<CH>(T[yo[7]]);
+        v.x += lap[4] * (p1.x + m1.x) + lap[5] * (p2.x + m2.x) + lap[6] * (p3.x + m3.x) + lap[7] * (p4.x + m4.x);
+        v.y += lap[4] * (p1.y + m1.y) + lap[5] * (p2.y + m2.y) + lap[6] * (p3.y + m3.y) + lap[7] * (p4.y + m4.y);
+        w.x += nab[4] * (p1.x - m1.x) + nab[5] * (p2.x - m2.x) + nab[6] * (p3.x - m3.x) + nab[7] * (p4.x - m4.x);
+        w.y += nab[4] * (p1.y - m1.y) + nab[5] * (p2.y - m2.y) + nab[6] * (p3.y - m3.y) + nab[7] * (p4.y - m4.y);
+      }
+      const CH t = q[8];
+      const R vcur = (R)vR[BS * P + te];
+      CH hc;
+      hc.x = (vcur * t.x + klap0 * t.x) - (R)0.5 * v.x + w.y;
+      hc.y = (vcur * t.y + klap0 * t.y) - (R)0.5 * v.y - w.x;
+      hq[P] = hc;
+    }
+    CH v, w;
+    v.x = lap[8] * (q[5].x + q[3].x) + lap[9] * (q[6].x + q[2].x) + lap[10] * (q[7].x + q[1].x) + lap[11] * (q[8].x + q[0].x);
+    v.y = lap[8] * (q[5].y + q[3].y) + lap[9] * (q[6].y + q[2].y) + lap[10] * (q[7].y + q[1].y) + lap[11] * (q[8].y + q[0].y);
+    w.x = nab[8] * (q[5].x - q[3].x) + nab[9] * (q[6].x - q[2].x) + nab[10] * (q[7].x - q[1].x) + nab[11] * (q[8].x - q[0].x);
+    w.y = nab[8] * (q[5].y - q[3].y) + nab[9] * (q[6].y - q[2].y) + nab[10] * (q[7].y - q[1].y) + nab[11] * (q[8].y - q[0].y);
+    CH hh;
+    hh.x = hp.x - (R)0.5 * v.x + w.y;
+    hh.y = hp.y - (R)0.5 * v.y - w.x;
+    // every lane stores every plane (the vm count per plane is fixed): the first four planes
+    // of the chunk and lanes without a point store into the sink
+    const bool outP = outY && p >= z0 + 4;
+    TO *dst = outP ? (TO *)a.hout + orbOff + colOff + zs * (p - 4) : (TO *)a.sink + tid;
+    *dst = cvt2<TO>(hh);
+  };
+  for (int p = z0; p < z1 + 4; p += 4) {
+    body(p, Ph<0>());
+    body(p + 1, Ph<1>());
+    body(p + 2, Ph<2>());
+    body(p + 3, Ph<3>());
+  }
+  wait_vm<0>();
+}
+
+template <typename TI, typename R, typename TO>
+__global__ __launch_bounds__(kTile) void k_stencilz(StencilArgs a) {
+  extern __shared__ double2 lds[];
+  int bid = blockIdx.x;
+  const int nfull = (a.nblk / 8) * 8;
+  if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;  // XCD-aware tile order
+  stencil_tilez<TI, R, TO>(a, bid, lds);
+}
+
 template <typename TI, typename R, typename TO>
 __global__ __launch_bounds__(kTile) void k_stencil8(StencilArgs a) {
   extern __shared__ double2 lds[];
@@ -1648,6 +1826,7 @@ struct salmon_hip_ctx {
   // (whole plane when it fits 256 threads), 0 = y tiles, 1 = whole plane; mid_nzt 0 = rule
   int tile_plane = -1, mid_nzt = 0;
   bool tile8 = true;  // stencil_tile8 for the terms without the Taylor sum where the shape allows it
+  bool tilez = true;  // stencil_tilez for the FP64 ones and the mixed first term (before stencil_tile8)
   // kernel timing (salmon_hip_prof_*): start/stop event pairs per launch
   bool prof = false;
   std::vector<hipEvent_t> ev;
@@ -2067,6 +2246,26 @@ static bool tile8_fits(const salmon_hip_ctx *c, int stage, int nc, const Stencil
          a.TZ % 8 == 0 && a.nz % 8 == 0 && c->tile8;
 }
 
+// y tiles with the delayed z part (stencil_tilez): the FP64 terms without the Taylor sum and the
+// mixed first term (FP64 input), rows of at most 32 points, z chunks of 4k planes.
+static bool tilez_fits(const salmon_hip_ctx *c, int stage, int nc, const StencilArgs &a) {
+  return ((stage == kStagePlain && nc < 0) || stage == kStageMixedFirst) && a.nx <= 32 &&
+         (a.TY + 8) * a.nx <= 2 * kTile && a.TY * a.nx <= kTile && a.TZ % 4 == 0 && a.nz % 4 == 0 && c->tilez;
+}
+
+template <typename TI, typename R, typename TO>
+static int launch_tilez_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
+  const size_t shmem = sizeof(TI) * 4 * 2 * kTile + sizeof(double) * 4 * kTile;
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencilz<TI, R, TO>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)shmem));
+    attr_set = true;
+  }
+  hipExtLaunchKernelGGL((k_stencilz<TI, R, TO>), grid, dim3(kTile), shmem, c->stream, e0, e1, 0, a);
+  return 0;
+}
+
 template <typename TI, typename R, typename TO>
 static int launch_tile8_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
   const size_t shmem = sizeof(TI) * 8 * 2 * kTile + sizeof(double) * 4 * kTile;
@@ -2200,6 +2399,9 @@ static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, boo
   if (bs == 256) rc = launch_stencil_plane<256>(c, stage, grid, e0, e1, a);
   else if (bs == 512) rc = launch_stencil_wp<1>(c, stage, grid, e0, e1, a);
   else if (bs == 1024) rc = launch_stencil_wp<2>(c, stage, grid, e0, e1, a);
+  else if (tilez_fits(c, stage, nc, a))
+    rc = stage == kStagePlain ? launch_tilez_t<double2, double, double2>(c, grid, e0, e1, a)
+                              : launch_tilez_t<double2, float, float2>(c, grid, e0, e1, a);
   else if (tile8_fits(c, stage, nc, a))
     rc = stage == kStagePlain ? launch_tile8_t<double2, double, double2>(c, grid, e0, e1, a)
                               : launch_tile8_t<float2, float, float2>(c, grid, e0, e1, a);
