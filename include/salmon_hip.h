@@ -281,6 +281,7 @@ int salmon_hip_batch_replay(salmon_hip_batch_t b, int npairs);
 #define SALMON_HIP_KERNEL_PROJ 4           /* k_proj, KB projection */
 #define SALMON_HIP_KERNEL_PROJ_CURRENT 5   /* k_proj, nonlocal-current moments */
 #define SALMON_HIP_KERNEL_NL_SCATTER 6     /* k_nl_scatter, KB term at sphere points */
+#define SALMON_HIP_KERNEL_KB_FUSED 7       /* KB term of a Taylor term fused with its projection */
 int salmon_hip_prof_start(salmon_hip_t h, int max_launches);
 int salmon_hip_prof_stop(salmon_hip_t h, int kind, double *total_ms, int *count);
 int salmon_hip_prof_end(salmon_hip_t h);

@@ -360,6 +360,182 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// KB term of one H application fused with the projection of its result: for a Taylor term
+// h = H h_prev the pass adds s * sum_lma c_prev(lma) zekr(j, lma) at the sphere points that one
+// (atom, point) pair covers alone and projects the completed h onto the atom's projectors (the
+// c of the next term), reading h once.  Block = (k, atom, 64 orbitals) as in k_proj<1>: each
+// thread owns one sphere point of every 32-point chunk and 8 of the orbitals; after the chunk is
+// staged in LDS it completes its points (exclusive ones: KB sum in ascending ilma, in FP64, then
+// rounded to TP as k_nl_scatter stores it), writes them back and puts the rounded values into the
+// LDS tile the MFMAs read.  Points covered by several pairs are completed beforehand by
+// k_nl_scatter over their list, and no point is written by two blocks (an exclusive point
+// belongs to one atom, and the blocks of one atom split the orbitals), so the pass is
+// deterministic and race-free.  c_prev for the block's orbitals is staged in dynamic LDS
+// (cmax x 64 elements).
+// ---------------------------------------------------------------------------
+struct ProjScatArgs {
+  ProjArgs p;              // projection of h (p.psi = h, p.out = its c, scaled by rinv)
+  const double2 *cin;      // c of h_prev (nb, nlma, nk), scaled by rinv
+  const int *jexcl;        // (nps, natom): offset within an orbital block if one pair covers the point, else -1
+  double2 s;               // h += s * sum (1 for the terms without the Taylor sum)
+  int cmax;                // projectors per atom, at most (the row count of the staged c)
+};
+
+template <typename TP, int LM>
+__global__ __launch_bounds__(kBlock) void k_proj_scatter(ProjScatArgs A) {
+  static_assert(LM <= kMaxLmaPerAtom, "LM bounds the projectors of one atom");
+  const ProjArgs &a = A.p;
+  constexpr int MAXT = (kMaxLmaPerAtom + 7) / 8;
+  // rows padded by one element: the per-point accesses of the completion step (lanes on points)
+  // then fall on different banks
+  __shared__ double2 sp[kProjJC][kProjOrb + 1];
+  __shared__ double2 zsh[kProjJC][kMaxLmaPerAtom + 1];
+  extern __shared__ double2 csh[];  // cmax x kProjOrb
+  int bid = blockIdx.x;
+  const int nfull = ((int)gridDim.x / 8) * 8;
+  if (bid < nfull) bid = (bid % 8) * ((int)gridDim.x / 8) + bid / 8;
+  const int grp = bid % a.ngrp;
+  const int t = bid / a.ngrp;
+  const int ia = t % a.natom;
+  const int ik = t / a.natom;
+  const int2 al = a.atom_lma[ia];
+  const int m = a.mps[ia];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int nlw = al.y;
+  const int ntile = (nlw + 7) >> 3;
+  const size_t kbase = (size_t)ik * a.nb;
+
+  for (int e = threadIdx.x; e < al.y * kProjOrb; e += kBlock) {
+    const int l = e / kProjOrb, o = e % kProjOrb, ob = grp * kProjOrb + o;
+    csh[e] = ob < a.nb ? A.cin[((size_t)ik * a.nlma + al.x + l) * a.nb + ob] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+
+  typedef double __attribute__((ext_vector_type(4))) d4;
+  d4 acc[MAXT];
+#pragma unroll
+  for (int tt = 0; tt < MAXT; ++tt) acc[tt] = d4{0.0, 0.0, 0.0, 0.0};
+  const int kq = lane >> 4;
+  const int comp = kq & 1;
+  const int oc = lane & 15;
+
+  constexpr int kPer = (kProjJC * kProjOrb) / kBlock;
+  constexpr int kZPer = (kProjJC * kMaxLmaPerAtom + kBlock - 1) / kBlock;
+  TP pre[kPer];
+  double2 zpre[kZPer];
+  TP *h = (TP *)a.psi;
+  const int *jx = a.jidx + (size_t)a.nps * ia;
+  const int *jxe = A.jexcl + (size_t)a.nps * ia;
+  const double2 *zk = a.zekr + (size_t)a.nps * ((size_t)al.x + (size_t)a.nlma * ik);
+  static_assert(kBlock % kProjJC == 0, "one sphere point per thread and chunk");
+  const int jt = threadIdx.x % kProjJC;
+  const int o0 = threadIdx.x / kProjJC;  // this thread's orbitals: o0 + (kBlock / kProjJC) r
+  int jnext = jt < m ? jx[jt] : 0;
+  int xnext = jt < m ? jxe[jt] : -1;
+  int jload = 0, xload = -1;  // point offset and exclusive offset of the chunk last loaded
+  double2 zown[LM];           // zekr of this thread's point of that chunk (exclusive points only)
+  auto load_chunk = [&](int j0) {
+    jload = jnext;
+    xload = xnext;
+#pragma unroll
+    for (int l = 0; l < LM; ++l)
+      zown[l] = (xload >= 0 && l < al.y) ? zk[(size_t)(j0 + jt) + (size_t)a.nps * l] : make_double2(0.0, 0.0);
+    const bool more = j0 + kProjJC + jt < m;
+    jnext = more ? jx[j0 + kProjJC + jt] : 0;
+    xnext = more ? jxe[j0 + kProjJC + jt] : -1;
+#pragma unroll
+    for (int r = 0; r < kZPer; ++r) {
+      const int e = threadIdx.x + kBlock * r;
+      const int jj = e % kProjJC, l = e / kProjJC;
+      const int j = j0 + jj;
+      zpre[r] = (j < m && l < al.y) ? zk[(size_t)j + (size_t)a.nps * l] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const int ob = grp * kProjOrb + o0 + (kBlock / kProjJC) * r;
+      TP v;
+      v.x = 0;
+      v.y = 0;
+      if (j0 + jt < m && ob < a.nb) v = h[(kbase + ob) * a.osz + jload];
+      pre[r] = v;
+    }
+  };
+  load_chunk(0);
+  for (int j0 = 0; j0 < m; j0 += kProjJC) {
+    // complete this thread's point if one pair covers it (xload >= 0 implies j0 + jt < m) while
+    // staging the chunk
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const int o = o0 + (kBlock / kProjJC) * r;
+      const int ob = grp * kProjOrb + o;
+      double2 v = cvt2<double2>(pre[r]);
+      if (xload >= 0 && ob < a.nb) {
+        double2 sum = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int l = 0; l < LM; ++l) {
+          if (l < al.y) {
+            const double2 cv = csh[l * kProjOrb + o];
+            sum.x += cv.x * zown[l].x - cv.y * zown[l].y;
+            sum.y += cv.x * zown[l].y + cv.y * zown[l].x;
+          }
+        }
+        v.x += A.s.x * sum.x - A.s.y * sum.y;
+        v.y += A.s.x * sum.y + A.s.y * sum.x;
+        const TP vt = cvt2<TP>(v);
+        h[(kbase + ob) * a.osz + jload] = vt;
+        v = cvt2<double2>(vt);
+      }
+      sp[jt][o] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < kZPer; ++r) {
+      const int e = threadIdx.x + kBlock * r;
+      if (e < kProjJC * kMaxLmaPerAtom) zsh[e % kProjJC][e / kProjJC] = zpre[r];
+    }
+    __syncthreads();
+    if (j0 + kProjJC < m) load_chunk(j0 + kProjJC);
+#pragma unroll 2
+    for (int ks = 0; ks < kProjJC / 2; ++ks) {
+      const int jp = 2 * ks + (kq >> 1);
+      const int j = j0 + jp;
+      const double2 pv = sp[jp][wave * 16 + oc];
+      const double bval = comp ? pv.y : pv.x;
+      const bool jv = j < m;
+#pragma unroll
+      for (int tt = 0; tt < MAXT; ++tt) {
+        if (tt < ntile) {
+          const int part = (oc >> 2) & 1;
+          const int lw = tt * 8 + (oc >> 3) * 4 + (oc & 3);
+          double aval = 0.0;
+          if (jv && lw < nlw) {
+            const double2 z = zsh[jp][lw];
+            aval = part == 0 ? (comp ? z.y : z.x) : (comp ? z.x : -z.y);
+          }
+          acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(aval, bval, acc[tt], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const int ob = grp * kProjOrb + wave * 16 + oc;
+  if (ob >= a.nb) return;
+#pragma unroll
+  for (int tt = 0; tt < MAXT; ++tt) {
+    if (tt >= ntile) break;
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+      const int lw = tt * 8 + hh * 4 + kq;
+      if (lw >= nlw) continue;
+      const int ilma = al.x + lw;
+      const double re = acc[tt][2 * hh], im = acc[tt][2 * hh + 1];
+      const double sc = a.rinv[ilma];
+      a.out[((size_t)ik * a.nlma + ilma) * a.nb + ob] = make_double2(re * sc, im * sc);
+    }
+  }
+}
+
 constexpr int kMaxComb = 5;   // Taylor order up to 6
 
 // ---------------------------------------------------------------------------
@@ -1804,10 +1980,17 @@ struct salmon_hip_ctx {
   int *d_pt_lma = nullptr, *d_entof = nullptr;
   int4 *d_tp_info = nullptr;  // per touched point: orbital-block offset, first entry, entry count, its pt_lma
   int ntp = 0;
+  // fused KB pass (k_proj_scatter): per sphere point its offset if one (atom, point) pair covers
+  // it, else -1; the touched points with several pairs (completed by k_nl_scatter first)
+  int *d_jexcl = nullptr;
+  int4 *d_tp_shared = nullptr;
+  int ntp_shared = 0;
+  bool kb_fused = true;
   double2 *d_sink = nullptr;
   int2 *d_atom_lma = nullptr;
   double *d_rxyz = nullptr, *d_uv = nullptr, *d_rinv = nullptr;
   double2 *d_zekr = nullptr, *d_zent = nullptr, *d_c = nullptr, *d_curnl = nullptr;
+  double2 *d_c2 = nullptr;  // the fused pass writes the next term's c here; the two are then swapped
   // Taylor scratch: H^2 psi .. H^(N-2) psi (allocated on first use); FP32 chain h_1 .. h_{N-1}
   std::vector<double2 *> tscr;
   std::vector<float2 *> tscr32;
@@ -1924,6 +2107,7 @@ int salmon_hip_destroy(salmon_hip_t c) {
   dfree(c->d_mps); dfree(c->d_jidx); dfree(c->d_lma_atom); dfree(c->d_pt_off);
   dfree(c->d_atom_lma); dfree(c->d_pt_lma); dfree(c->d_entof); dfree(c->d_rxyz); dfree(c->d_uv);
   dfree(c->d_rinv); dfree(c->d_zekr); dfree(c->d_zent); dfree(c->d_c); dfree(c->d_curnl);
+  dfree(c->d_c2); dfree(c->d_jexcl); dfree(c->d_tp_shared);
   dfree(c->d_tp_info); dfree(c->d_sink);
   dfree(c->d_curpart); dfree(c->d_contrib); dfree(c->d_denspart);
   for (int d = 0; d < 3; ++d) dfree(c->d_tw[d]);
@@ -1956,8 +2140,9 @@ int salmon_hip_set_nonlocal(salmon_hip_t c, int natom, int nps, const int *mps, 
   dfree(c->d_mps); dfree(c->d_jidx); dfree(c->d_lma_atom); dfree(c->d_pt_off);
   dfree(c->d_atom_lma); dfree(c->d_pt_lma); dfree(c->d_entof); dfree(c->d_rxyz); dfree(c->d_uv);
   dfree(c->d_rinv); dfree(c->d_zekr); dfree(c->d_zent); dfree(c->d_c); dfree(c->d_curnl);
+  dfree(c->d_c2); dfree(c->d_jexcl); dfree(c->d_tp_shared);
   dfree(c->d_tp_info);
-  c->natom = c->nps = c->nlma = c->nent = c->cmax = c->ntp = 0;
+  c->natom = c->nps = c->nlma = c->nent = c->cmax = c->ntp = c->ntp_shared = 0;
   c->have_k = false;
   if (natom == 0 || nlma == 0) return 0;
   if (natom < 0 || nps < 1 || nlma < 0 || !mps || !jxyz || !rxyz || !ia_tbl || !rinv_uvu || !uv)
@@ -2021,11 +2206,31 @@ int salmon_hip_set_nonlocal(salmon_hip_t c, int natom, int nps, const int *mps, 
         if (cnt[g + 1] > cnt[g])
           tp_info.push_back(make_int4(x + c->nx * (y + c->ly * z), cnt[g], cnt[g + 1] - cnt[g], pt_lma[cnt[g]]));
       }
+  // fused-pass tables: a sphere point's offset where its grid point has one entry, else -1;
+  // the touched points with several entries
+  std::vector<int> jexcl((size_t)natom * nps, -1);
+  for (int ia = 0; ia < natom; ++ia) {
+    if (atom_lma[ia].y == 0) continue;
+    for (int j = 0; j < mps[ia]; ++j) {
+      const int *p = jxyz + 3 * ((size_t)j + (size_t)nps * ia);
+      const size_t g = (size_t)(p[0] - 1) + (size_t)c->nx * ((size_t)(p[1] - 1) + (size_t)c->ny * (p[2] - 1));
+      if (cnt[g + 1] - cnt[g] == 1) jexcl[(size_t)nps * ia + j] = jidx[(size_t)nps * ia + j];
+    }
+  }
+  std::vector<int4> tp_shared;
+  for (const int4 &t : tp_info)
+    if (t.z > 1) tp_shared.push_back(t);
   int rc = 0;
   if (!tp_info.empty()) {
     if ((rc = dalloc(&c->d_tp_info, tp_info.size()))) return rc;
     HIPCHK(hipMemcpy(c->d_tp_info, tp_info.data(), sizeof(int4) * tp_info.size(), hipMemcpyHostToDevice));
   }
+  if (!tp_shared.empty()) {
+    if ((rc = dalloc(&c->d_tp_shared, tp_shared.size()))) return rc;
+    HIPCHK(hipMemcpy(c->d_tp_shared, tp_shared.data(), sizeof(int4) * tp_shared.size(), hipMemcpyHostToDevice));
+  }
+  if ((rc = dalloc(&c->d_jexcl, jexcl.size())) || (rc = dalloc(&c->d_c2, (size_t)nlma * c->norb))) return rc;
+  HIPCHK(hipMemcpy(c->d_jexcl, jexcl.data(), sizeof(int) * jexcl.size(), hipMemcpyHostToDevice));
   if ((rc = dalloc(&c->d_mps, natom)) || (rc = dalloc(&c->d_jidx, (size_t)natom * nps)) ||
       (rc = dalloc(&c->d_lma_atom, nlma)) || (rc = dalloc(&c->d_pt_off, c->ngrid + 1)) ||
       (rc = dalloc(&c->d_atom_lma, natom)) || (rc = dalloc(&c->d_pt_lma, pt_lma.size())) ||
@@ -2051,6 +2256,7 @@ int salmon_hip_set_nonlocal(salmon_hip_t c, int natom, int nps, const int *mps, 
   c->nent = nent;
   c->cmax = cmax;
   c->ntp = (int)tp_info.size();
+  c->ntp_shared = (int)tp_shared.size();
   return 0;
 }
 
@@ -2306,8 +2512,11 @@ static int launch_stencil_hc(salmon_hip_ctx *c, int stage, int nc, dim3 grid, hi
 
 extern "C" {
 
-static int launch_scatter(salmon_hip_t c, void *hout, double2 s, bool f32 = false) {
-  if (c->ntp == 0) return 0;
+// shared_only: the touched points with several (atom, point) pairs (before k_proj_scatter);
+// e0/e1 override the profiling events (nullptr: this launch's own)
+static int launch_scatter(salmon_hip_t c, void *hout, double2 s, bool f32 = false, bool shared_only = false,
+                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, bool own_events = true) {
+  if ((shared_only ? c->ntp_shared : c->ntp) == 0) return 0;
   ScatterArgs a;
   a.hout = hout;
   a.c = c->d_c;
@@ -2321,11 +2530,15 @@ static int launch_scatter(salmon_hip_t c, void *hout, double2 s, bool f32 = fals
   a.nlma = c->nlma;
   a.nent = c->nent;
   a.cmax = c->cmax;
+  hipEvent_t e0 = ev0, e1 = ev1;
+  if (own_events) prof_events(c, SALMON_HIP_KERNEL_NL_SCATTER, &e0, &e1);
+  if (shared_only) {
+    a.tp_info = c->d_tp_shared;
+    a.ntp = c->ntp_shared;
+  }
   const int G = f32 ? kScatG32 : kScatG;
   a.ngrp = (c->nb + G - 1) / G;
-  hipEvent_t e0, e1;
-  prof_events(c, SALMON_HIP_KERNEL_NL_SCATTER, &e0, &e1);
-  dim3 grid((unsigned)((c->ntp + kBlock - 1) / kBlock), (unsigned)(c->nk * a.ngrp));
+  dim3 grid((unsigned)((a.ntp + kBlock - 1) / kBlock), (unsigned)(c->nk * a.ngrp));
   if (f32) {
     if (c->cmax <= kScatCM32)
       hipExtLaunchKernelGGL((k_nl_scatter<float2, kScatG32, kScatCM32>), grid, dim3(kBlock), 0, c->stream, e0, e1, 0, a);
@@ -2340,6 +2553,63 @@ static int launch_scatter(salmon_hip_t c, void *hout, double2 s, bool f32 = fals
   LAUNCHCHK("k_nl_scatter");
   return 0;
 }
+
+// The KB term of hout = H in (c of `in` current in d_c) fused with the projection of hout: the
+// points with several pairs by k_nl_scatter, then k_proj_scatter; afterwards d_c holds the c of
+// hout.  Profiled as one SALMON_HIP_KERNEL_KB_FUSED interval.
+static int launch_kb_fused(salmon_hip_t c, void *hout, bool f32) {
+  hipEvent_t e0, e1;
+  prof_events(c, SALMON_HIP_KERNEL_KB_FUSED, &e0, &e1);
+  const bool shared = c->ntp_shared > 0;
+  if (shared)
+    if (int rc = launch_scatter(c, hout, make_double2(1.0, 0.0), f32, true, e0, nullptr, false)) return rc;
+  ProjScatArgs A;
+  ProjArgs &a = A.p;
+  a.psi = hout;
+  a.zekr = c->d_zekr;
+  a.rxyz = c->d_rxyz;
+  a.jidx = c->d_jidx;
+  a.mps = c->d_mps;
+  a.atom_lma = c->d_atom_lma;
+  a.rinv = c->d_rinv;
+  a.out = c->d_c2;
+  a.osz = c->osz;
+  a.nb = c->nb;
+  a.nk = c->nk;
+  a.natom = c->natom;
+  a.nps = c->nps;
+  a.nlma = c->nlma;
+  a.ngrp = (c->nb + kProjOrb - 1) / kProjOrb;
+  A.cin = c->d_c;
+  A.jexcl = c->d_jexcl;
+  A.s = make_double2(1.0, 0.0);
+  A.cmax = c->cmax;
+  const size_t shmem = sizeof(double2) * (size_t)c->cmax * kProjOrb;
+  dim3 grid((unsigned)(a.ngrp * c->natom * c->nk));
+  hipEvent_t s0 = shared ? nullptr : e0;
+  // LM: registers for the projector values of one point (s+p: 4, up to d: 9, up to f: 16)
+  if (f32) {
+    if (c->cmax <= 4)
+      hipExtLaunchKernelGGL((k_proj_scatter<float2, 4>), grid, dim3(kBlock), shmem, c->stream, s0, e1, 0, A);
+    else if (c->cmax <= 9)
+      hipExtLaunchKernelGGL((k_proj_scatter<float2, 9>), grid, dim3(kBlock), shmem, c->stream, s0, e1, 0, A);
+    else
+      hipExtLaunchKernelGGL((k_proj_scatter<float2, 16>), grid, dim3(kBlock), shmem, c->stream, s0, e1, 0, A);
+  } else {
+    if (c->cmax <= 4)
+      hipExtLaunchKernelGGL((k_proj_scatter<double2, 4>), grid, dim3(kBlock), shmem, c->stream, s0, e1, 0, A);
+    else if (c->cmax <= 9)
+      hipExtLaunchKernelGGL((k_proj_scatter<double2, 9>), grid, dim3(kBlock), shmem, c->stream, s0, e1, 0, A);
+    else
+      hipExtLaunchKernelGGL((k_proj_scatter<double2, 16>), grid, dim3(kBlock), shmem, c->stream, s0, e1, 0, A);
+  }
+  LAUNCHCHK("k_proj_scatter");
+  std::swap(c->d_c, c->d_c2);
+  return 0;
+}
+
+// The fused KB pass is used inside Taylor chains when the projections run on the handle's stream.
+static bool kb_fused_on(const salmon_hip_ctx *c) { return c->kb_fused && !c->overlap && c->natom > 0 && c->nlma > 0; }
 
 // One H application: hout = H in (fin = false), or the last Taylor term
 // hout = psi + c1 h1 + ... + c_N H in (fin = true, comb = [psi, h1 .. h_{N-2}]).
@@ -2386,8 +2656,9 @@ static StencilArgs stencil_args(salmon_hip_t c, const void *in, void *hout, bool
   return a;
 }
 
+// fuse_next: finish with the fused KB pass (hout's KB term and its projection) instead of the scatter.
 static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, bool fin, void *const *comb, int ncomb,
-                       const double *zc, int stage = kStagePlain) {
+                       const double *zc, int stage = kStagePlain, bool fuse_next = false) {
   int bs = 0;
   const int nc = fin ? ncomb : -1;
   StencilArgs a = stencil_args(c, in, hout, fin, comb, ncomb, zc, &bs, false);
@@ -2413,6 +2684,7 @@ static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, boo
   LAUNCHCHK("k_stencil");
   const bool f32out = stage == kStageMixedFirst || stage == kStageMixedMiddle;
   if (int rc = join_proj(c)) return rc;
+  if (fuse_next && !fin && kb_fused_on(c)) return launch_kb_fused(c, hout, f32out);
   if (c->natom > 0 && c->nlma > 0)
     return launch_scatter(c, hout, fin ? a.zc[ncomb] : make_double2(1.0, 0.0), f32out);
   return 0;
@@ -2447,14 +2719,18 @@ static int taylor_mixed(salmon_hip_t c, const double *zc, int n_hamil, void *tin
   void *h[kMaxComb + 1];
   h[0] = tin;
   for (int n = 1; n < n_hamil; ++n) h[n] = c->tscr32[n - 1];
+  // with the fused KB pass each term's projection comes out of the previous term's KB pass
+  const bool fuse = kb_fused_on(c);
+  bool have_c = false;
   for (int n = first; n < n_hamil; ++n) {
-    if (c->natom > 0)
+    if (c->natom > 0 && !have_c)
       if (int rc = launch_proj(c, h[n - 1], 1, n > 1)) return rc;
     const int kind = n == 1 ? SALMON_HIP_KERNEL_HPSI_TAYLOR1 : SALMON_HIP_KERNEL_HPSI_TAYLOR;
     const int stage = n == 1 ? kStageMixedFirst : kStageMixedMiddle;
-    if (int rc = launch_hpsi(c, kind, h[n - 1], h[n], false, nullptr, 0, nullptr, stage)) return rc;
+    if (int rc = launch_hpsi(c, kind, h[n - 1], h[n], false, nullptr, 0, nullptr, stage, fuse)) return rc;
+    have_c = fuse;
   }
-  if (c->natom > 0)
+  if (c->natom > 0 && !have_c)
     if (int rc = launch_proj(c, h[n_hamil - 1], 1, true)) return rc;
   return launch_hpsi(c, SALMON_HIP_KERNEL_HPSI_TAYLOR_LAST, h[n_hamil - 1], tout, true, h, n_hamil - 1, zc,
                      kStageMixedFinal);
@@ -2482,13 +2758,16 @@ static int taylor_fp64(salmon_hip_t c, const double *zc, int n_hamil, void *tsps
   h[0] = tspsi_in;
   if (n_hamil >= 2) h[1] = sshtpsi;
   for (int n = 2; n < n_hamil; ++n) h[n] = c->tscr[n - 2];
+  const bool fuse = kb_fused_on(c);
+  bool have_c = false;
   for (int n = first; n < n_hamil; ++n) {
-    if (c->natom > 0)
+    if (c->natom > 0 && !have_c)
       if (int rc = launch_proj(c, h[n - 1], 1)) return rc;
     const int kind = n == 1 ? SALMON_HIP_KERNEL_HPSI_TAYLOR1 : SALMON_HIP_KERNEL_HPSI_TAYLOR;
-    if (int rc = launch_hpsi(c, kind, h[n - 1], h[n], false, nullptr, 0, nullptr)) return rc;
+    if (int rc = launch_hpsi(c, kind, h[n - 1], h[n], false, nullptr, 0, nullptr, kStagePlain, fuse)) return rc;
+    have_c = fuse;
   }
-  if (c->natom > 0)
+  if (c->natom > 0 && !have_c)
     if (int rc = launch_proj(c, h[n_hamil - 1], 1)) return rc;
   return launch_hpsi(c, SALMON_HIP_KERNEL_HPSI_TAYLOR_LAST, h[n_hamil - 1], tspsi_out, true, h, n_hamil - 1, zc);
 }

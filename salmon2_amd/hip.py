@@ -276,7 +276,7 @@ class Handle:
              "update_potential")
 
     KERNELS = {"hpsi": 0, "hpsi_taylor1": 1, "hpsi_taylor": 2, "hpsi_taylor_last": 3, "proj": 4,
-               "proj_current": 5, "nl_scatter": 6}
+               "proj_current": 5, "nl_scatter": 6, "kb_fused": 7}
 
     def prof_start(self, max_launches=100000):
         _chk(load().salmon_hip_prof_start(self.h, int(max_launches)), "prof_start")
