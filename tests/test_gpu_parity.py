@@ -42,6 +42,10 @@ def _cases():
         "si_48x40x36_nopad": lambda: system.make_si_system(ncell=(1, 1, 1), n=(48, 40, 36), no=3, pad=0, seed=17),
         # 10-row y tiles with a partial last tile (rows 20-21), z chunks of 8 and 16 planes
         "ragged_24x22x16": lambda: system.make_si_system(n=(24, 22, 16), no=6, nkgrid=(1, 1, 1), seed=19),
+        # projector spheres of radius 3.6: 58% of the touched points covered twice
+        "overlap_rc3.6": lambda: system.make_si_system(n=12, no=4, nkgrid=(1, 1, 2), rcut=3.6, seed=23),
+        # radius 5.3 > a/2: every touched point covered 2-10 times, also by images of the same atom
+        "selfimage_rc5.3": lambda: system.make_si_system(n=12, no=4, nkgrid=(1, 1, 2), rcut=5.3, seed=23),
     }
 
 
