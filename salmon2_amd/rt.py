@@ -112,6 +112,13 @@ class RTPropagator:
             self.fails = 0  # rejections since the last accepted attempt
             self.last_attempt = 0
         self.record = [] if record else None
+        # the current of a predictor-corrector step runs on a second stream while the density and
+        # the potential update run on the first (independent reads of the new orbitals); joined at
+        # the end of the step, before the next phase-table update
+        self.cur_stream = torch.cuda.Stream(self.dev)
+        self.ev_psi = torch.cuda.Event()
+        self.ev_cur = torch.cuda.Event()
+        self.cur_async = False
 
     def setup_outputs(self, nt):
         s = self.s
@@ -136,6 +143,7 @@ class RTPropagator:
                 self.h.density(self.psi_b, self.rho)
                 self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)
                 self._corrector(self.vcur)
+                self._current_async(itt)
             else:
                 self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
             self.h.density(self.psi_b, self.rho)
@@ -156,13 +164,29 @@ class RTPropagator:
                 self.pred_ready = True
         if self.record is not None:
             self.record.append((self.vcur.clone(), self.vmid.clone()))
-        self.h.current(self.ac[itt], self.psi_b, self.curr_t[itt - 1])
+        if not self.cur_async:
+            self.h.current(self.ac[itt], self.psi_b, self.curr_t[itt - 1])
         self.h.dipole(self.rho, self.dip_t[itt - 1])
+        if self.cur_async:
+            self.stream.wait_event(self.ev_cur)
+            self.cur_async = False
         if self.trans_longi == "lo" and itt + 1 < len(self.ac):
             j = self.curr_t[itt - 1].sum(dim=0).cpu().numpy()  # synchronises this step
             self.ac_ind[itt + 1] = 2.0 * self.ac_ind[itt] - self.ac_ind[itt - 1] - 4.0 * math.pi * j * self.s.dt ** 2
             self.ac[itt + 1] = self.ac_ext[itt + 1] + self.ac_ind[itt + 1]
         self.psi_a, self.psi_b = self.psi_b, self.psi_a
+
+    def _current_async(self, itt):
+        """The current of the orbitals just completed (psi_b), on the second stream."""
+        self.ev_psi.record(self.stream)
+        self.cur_stream.wait_event(self.ev_psi)
+        self.h.set_stream(self.cur_stream)
+        try:
+            self.h.current(self.ac[itt], self.psi_b, self.curr_t[itt - 1])
+        finally:
+            self.h.set_stream(self.stream)
+        self.ev_cur.record(self.cur_stream)
+        self.cur_async = True
 
     def _corrector(self, v_pred):
         """The corrector Taylor step with V_mid (the handle's current potential) after a
@@ -212,6 +236,7 @@ class RTPropagator:
             self.h.density(self.psi_b, self.rho)
             self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)
             self._corrector(self.vcur)
+        self._current_async(self.itt)
         self.h.density(self.psi_b, self.rho)
         self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
         return True
