@@ -1047,7 +1047,9 @@ __device__ __forceinline__ void stencil_plane(const StencilArgs &a, int bid, dou
 // queue whose rotation is resolved by the unrolling (no moves).  Issue order per plane and
 // wave: 2 x W tile DMAs (plane z+7), 2 V dword DMAs (plane z+3), 1 store; the wait at the
 // top of a plane leaves the last two planes' operations in flight.  Needs TZ % 8 == 0.
-// Arithmetic and its order are those of stencil_tile.
+// Used for the middle terms of the mixed chain (instruction-issue bound): the pair sums of
+// the x, y and z neighbours go into one fused multiply-add chain per component, a regrouping
+// of stencil_tile's sum that saves two instructions per component.
 // ---------------------------------------------------------------------------
 template <int P> struct Ph { static constexpr int v = P; };
 
@@ -1150,35 +1152,40 @@ __device__ __forceinline__ void stencil_tile8(const StencilArgs &a, int bid, dou
     dma_tile(min(z + 7, z1 + 3), (P + 7) & 7);
     dma_v(min(z + 3, z1 - 1), (P + 3) & 3);
     const TI *T = tileR + SLOT * P;
-    CH v, w;
+    // one accumulation chain per component (x, then y, then z pairs: 12 fused multiply-adds), then
+    // hh = (V t + klap0 t) - v/2 + i w
+    R vx, vy, wx, wy;
     {
       const CH p1 = cvt2<CH>(T[xo[0]]), p2 = cvt2<CH>(T[xo[1]]), p3 = cvt2<CH>(T[xo[2]]), p4 = cvt2<CH>(T[xo[3]]);
       const CH m1 = cvt2<CH>(T[xo[4]]), m2 = cvt2<CH>(T[xo[5]]), m3 = cvt2<CH>(T[xo[6]]), m4 = cvt2<CH>(T[xo[7]]);
-      v.x = lap[0] * (p1.x + m1.x) + lap[1] * (p2.x + m2.x) + lap[2] * (p3.x + m3.x) + lap[3] * (p4.x + m4.x);
-      v.y = lap[0] * (p1.y + m1.y) + lap[1] * (p2.y + m2.y) + lap[2] * (p3.y + m3.y) + lap[3] * (p4.y + m4.y);
-      w.x = nab[0] * (p1.x - m1.x) + nab[1] * (p2.x - m2.x) + nab[2] * (p3.x - m3.x) + nab[3] * (p4.x - m4.x);
-      w.y = nab[0] * (p1.y - m1.y) + nab[1] * (p2.y - m2.y) + nab[2] * (p3.y - m3.y) + nab[3] * (p4.y - m4.y);
+      vx = lap[0] * (p1.x + m1.x);
+      vy = lap[0] * (p1.y + m1.y);
+      wx = nab[0] * (p1.x - m1.x);
+      wy = nab[0] * (p1.y - m1.y);
+      vx += lap[1] * (p2.x + m2.x); vy += lap[1] * (p2.y + m2.y); wx += nab[1] * (p2.x - m2.x); wy += nab[1] * (p2.y - m2.y);
+      vx += lap[2] * (p3.x + m3.x); vy += lap[2] * (p3.y + m3.y); wx += nab[2] * (p3.x - m3.x); wy += nab[2] * (p3.y - m3.y);
+      vx += lap[3] * (p4.x + m4.x); vy += lap[3] * (p4.y + m4.y); wx += nab[3] * (p4.x - m4.x); wy += nab[3] * (p4.y - m4.y);
     }
     {
       const CH p1 = cvt2<CH>(T[yo[0]]), p2 = cvt2<CH>(T[yo[1]]), p3 = cvt2<CH>(T[yo[2]]), p4 = cvt2<CH>(T[yo[3]]);
       const CH m1 = cvt2<CH>(T[yo[4]]), m2 = cvt2<CH>(T[yo[5]]), m3 = cvt2<CH>(T[yo[6]]), m4 = cvt2<CH>(T[yo[7]]);
-      v.x += lap[4] * (p1.x + m1.x) + lap[5] * (p2.x + m2.x) + lap[6] * (p3.x + m3.x) + lap[7] * (p4.x + m4.x);
-      v.y += lap[4] * (p1.y + m1.y) + lap[5] * (p2.y + m2.y) + lap[6] * (p3.y + m3.y) + lap[7] * (p4.y + m4.y);
-      w.x += nab[4] * (p1.x - m1.x) + nab[5] * (p2.x - m2.x) + nab[6] * (p3.x - m3.x) + nab[7] * (p4.x - m4.x);
-      w.y += nab[4] * (p1.y - m1.y) + nab[5] * (p2.y - m2.y) + nab[6] * (p3.y - m3.y) + nab[7] * (p4.y - m4.y);
+      vx += lap[4] * (p1.x + m1.x); vy += lap[4] * (p1.y + m1.y); wx += nab[4] * (p1.x - m1.x); wy += nab[4] * (p1.y - m1.y);
+      vx += lap[5] * (p2.x + m2.x); vy += lap[5] * (p2.y + m2.y); wx += nab[5] * (p2.x - m2.x); wy += nab[5] * (p2.y - m2.y);
+      vx += lap[6] * (p3.x + m3.x); vy += lap[6] * (p3.y + m3.y); wx += nab[6] * (p3.x - m3.x); wy += nab[6] * (p3.y - m3.y);
+      vx += lap[7] * (p4.x + m4.x); vy += lap[7] * (p4.y + m4.y); wx += nab[7] * (p4.x - m4.x); wy += nab[7] * (p4.y - m4.y);
     }
     const CH t = cvt2<CH>(T[own]);
     const CH u1 = cvt2<CH>(tileR[SLOT * ((P + 1) & 7) + own]), u2 = cvt2<CH>(tileR[SLOT * ((P + 2) & 7) + own]);
     const CH u3 = cvt2<CH>(tileR[SLOT * ((P + 3) & 7) + own]), u4 = cvt2<CH>(tileR[SLOT * ((P + 4) & 7) + own]);
     const CH d1 = q[(P + 3) & 3], d2 = q[(P + 2) & 3], d3 = q[(P + 1) & 3], d4 = q[P & 3];
-    v.x += lap[8] * (u1.x + d1.x) + lap[9] * (u2.x + d2.x) + lap[10] * (u3.x + d3.x) + lap[11] * (u4.x + d4.x);
-    v.y += lap[8] * (u1.y + d1.y) + lap[9] * (u2.y + d2.y) + lap[10] * (u3.y + d3.y) + lap[11] * (u4.y + d4.y);
-    w.x += nab[8] * (u1.x - d1.x) + nab[9] * (u2.x - d2.x) + nab[10] * (u3.x - d3.x) + nab[11] * (u4.x - d4.x);
-    w.y += nab[8] * (u1.y - d1.y) + nab[9] * (u2.y - d2.y) + nab[10] * (u3.y - d3.y) + nab[11] * (u4.y - d4.y);
+    vx += lap[8] * (u1.x + d1.x); vy += lap[8] * (u1.y + d1.y); wx += nab[8] * (u1.x - d1.x); wy += nab[8] * (u1.y - d1.y);
+    vx += lap[9] * (u2.x + d2.x); vy += lap[9] * (u2.y + d2.y); wx += nab[9] * (u2.x - d2.x); wy += nab[9] * (u2.y - d2.y);
+    vx += lap[10] * (u3.x + d3.x); vy += lap[10] * (u3.y + d3.y); wx += nab[10] * (u3.x - d3.x); wy += nab[10] * (u3.y - d3.y);
+    vx += lap[11] * (u4.x + d4.x); vy += lap[11] * (u4.y + d4.y); wx += nab[11] * (u4.x - d4.x); wy += nab[11] * (u4.y - d4.y);
     const R vcur = (R)vR[BS * (P & 3) + te];
     CH hh;
-    hh.x = (vcur * t.x + klap0 * t.x) - (R)0.5 * v.x + w.y;
-    hh.y = (vcur * t.y + klap0 * t.y) - (R)0.5 * v.y - w.x;
+    hh.x = (vcur * t.x + klap0 * t.x) - (R)0.5 * vx + wy;
+    hh.y = (vcur * t.y + klap0 * t.y) - (R)0.5 * vy - wx;
     TO *dst = outY ? (TO *)a.hout + orbOff + colOff + zs * z : (TO *)a.sink + tid;
     *dst = cvt2<TO>(hh);
     q[P & 3] = t;
@@ -2008,7 +2015,7 @@ struct salmon_hip_ctx {
   // launch shape of the terms without the Taylor sum (salmon_hip_autotune): tile_plane -1 = rule
   // (whole plane when it fits 256 threads), 0 = y tiles, 1 = whole plane; mid_nzt 0 = rule
   int tile_plane = -1, mid_nzt = 0;
-  bool tile8 = true;  // stencil_tile8 for the terms without the Taylor sum where the shape allows it
+  bool tile8 = true;  // stencil_tile8 for the mixed middle terms where the shape allows it
   bool tilez = true;  // stencil_tilez for the FP64 ones and the mixed first term (before stencil_tile8)
   // kernel timing (salmon_hip_prof_*): start/stop event pairs per launch
   bool prof = false;
@@ -2448,7 +2455,8 @@ static int launch_stencil_wp(salmon_hip_ctx *c, int stage, dim3 grid, hipEvent_t
 // y tiles with the merged halo and compile-time ring slots (stencil_tile8): the middle terms of
 // the mixed chain, rows of at most 32 points, (TY + 8) * nx <= 512, z chunks of 8k planes.
 static bool tile8_fits(const salmon_hip_ctx *c, int stage, int nc, const StencilArgs &a) {
-  return (stage == kStageMixedMiddle || (stage == kStagePlain && nc < 0)) && a.nx <= 32 && (a.TY + 8) * a.nx <= 2 * kTile && a.TY * a.nx <= kTile &&
+  (void)nc;
+  return stage == kStageMixedMiddle && a.nx <= 32 && (a.TY + 8) * a.nx <= 2 * kTile && a.TY * a.nx <= kTile &&
          a.TZ % 8 == 0 && a.nz % 8 == 0 && c->tile8;
 }
 
@@ -2673,9 +2681,7 @@ static int launch_hpsi(salmon_hip_t c, int kind, const void *in, void *hout, boo
   else if (tilez_fits(c, stage, nc, a))
     rc = stage == kStagePlain ? launch_tilez_t<double2, double, double2>(c, grid, e0, e1, a)
                               : launch_tilez_t<double2, float, float2>(c, grid, e0, e1, a);
-  else if (tile8_fits(c, stage, nc, a))
-    rc = stage == kStagePlain ? launch_tile8_t<double2, double, double2>(c, grid, e0, e1, a)
-                              : launch_tile8_t<float2, float, float2>(c, grid, e0, e1, a);
+  else if (tile8_fits(c, stage, nc, a)) rc = launch_tile8_t<float2, float, float2>(c, grid, e0, e1, a);
   else if (hc <= 1) rc = launch_stencil_hc<1>(c, stage, nc, grid, e0, e1, a);
   else if (hc <= 2) rc = launch_stencil_hc<2>(c, stage, nc, grid, e0, e1, a);
   else if (hc <= 4) rc = launch_stencil_hc<4>(c, stage, nc, grid, e0, e1, a);
