@@ -3294,16 +3294,22 @@ int batch_step(salmon_hip_batch *b, int p) {
   for (size_t i = 0; i < ns && !rc; ++i) rc = update_kvector_impl(b->h[i], nullptr, b->sys[i].actab, b->d_itt);
   if (int r2 = batch_join(b)) return r2;
   if (rc) return rc;
+  // the KB term of every term but the last runs fused with the next term's projection, so a
+  // system's projection phase runs only for the first term (or without the fused pass)
   for (int n = 1; n <= N; ++n) {
-    rc = batch_fork(b);
-    for (size_t i = 0; i < ns && !rc; ++i) {
-      salmon_hip_t c = b->h[i];
-      void *hh[kMaxComb + 1], *out;
-      batch_buffers(b, (int)i, p, hh, &out);
-      if (c->natom > 0) rc = launch_proj(c, hh[n - 1], 1);
+    bool any = false;
+    for (size_t i = 0; i < ns; ++i) any = any || (b->h[i]->natom > 0 && (n == 1 || !kb_fused_on(b->h[i])));
+    if (any) {
+      rc = batch_fork(b);
+      for (size_t i = 0; i < ns && !rc; ++i) {
+        salmon_hip_t c = b->h[i];
+        void *hh[kMaxComb + 1], *out;
+        batch_buffers(b, (int)i, p, hh, &out);
+        if (c->natom > 0 && (n == 1 || !kb_fused_on(c))) rc = launch_proj(c, hh[n - 1], 1);
+      }
+      if (int r2 = batch_join(b)) return r2;
+      if (rc) return rc;
     }
-    if (int r2 = batch_join(b)) return r2;
-    if (rc) return rc;
     if (int r2 = batch_stencil(b, p, n)) return r2;
     rc = batch_fork(b);
     for (size_t i = 0; i < ns && !rc; ++i) {
@@ -3311,7 +3317,9 @@ int batch_step(salmon_hip_batch *b, int p) {
       void *hh[kMaxComb + 1], *out;
       batch_buffers(b, (int)i, p, hh, &out);
       rc = join_proj(c);
-      if (!rc && c->natom > 0 && c->nlma > 0) {
+      if (!rc && n < N && kb_fused_on(c)) {
+        rc = launch_kb_fused(c, hh[n], false);
+      } else if (!rc && c->natom > 0 && c->nlma > 0) {
         const double2 sc = n == N ? make_double2(b->zc[2 * (N - 1)], b->zc[2 * (N - 1) + 1]) : make_double2(1.0, 0.0);
         rc = launch_scatter(c, n == N ? out : hh[n], sc);
       }
