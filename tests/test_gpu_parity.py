@@ -207,3 +207,33 @@ def test_launch_shapes_keep_parity(torch_dev, oracle_lib, name):
         _report(name, f"shape {shape} hpsi/taylor", max(e1, e2))
         assert e1 <= HPSI_TOL and e2 <= HPSI_TOL
     h.close()
+
+
+@pytest.mark.parametrize("n_hamil", [1, 2, 3, 6])
+def test_taylor_orders(torch_dev, oracle_lib, n_hamil):
+    """Taylor steps of order 1 .. 6 (chains of 0 .. 5 fused KB passes) against the oracle, FP64 and,
+    from order 2, the mixed chain."""
+    torch = torch_dev
+    s = system.make_si_system(n=12, no=4, nkgrid=(1, 1, 2), rcut=3.6, seed=29, n_hamil=n_hamil)
+    print(f"[gpu] taylor order {n_hamil}", flush=True)
+    from salmon2_amd.hip import Handle
+
+    A = np.array([0.012, -0.02, 0.007])
+    O = oracle_lib.Oracle(s)
+    O.update_kvector(A)
+    ref, _, _ = O.taylor(A, s.psi)
+    for prec, tol in (("fp64", HPSI_TOL), ("mixed", 1e-6)):
+        if prec == "mixed" and n_hamil < 2:
+            continue
+        h = Handle(s, stream=torch.cuda.current_stream())
+        h.set_precision(prec)
+        h.update_kvector(A)
+        t = torch.from_numpy(s.psi).cuda()
+        out = torch.zeros_like(t)
+        w = torch.zeros_like(t)
+        h.taylor(s.zc(), t, out, w)
+        torch.cuda.synchronize()
+        e = rel(out.cpu().numpy()[..., : s.ny, :], ref[..., : s.ny, :])
+        _report(f"order{n_hamil}", f"{prec} taylor", e)
+        assert e <= tol
+        h.close()
