@@ -31,7 +31,7 @@ over the whole run.  `value` is the steps/s of the fastest learned+mixed form wh
 current stays within 1e-6 (relative, max over the run) of pc-fp64's.
 
 Prints ONE JSON line.  The roofline entry is for the dominant kernel, the
-stencil of the middle Taylor terms (k_stencilz) in pc-fp64, timed live with
+stencil of the first and middle Taylor terms (k_stencilz) in pc-fp64, timed live with
 start/stop events recorded by its own launches over the timed region.
 """
 from __future__ import annotations
@@ -279,11 +279,14 @@ def main():
     ok = [f for f in HEADLINE_FORMS if f in form_out and form_out[f]["rel_err_vs_" + BASE_FORM]["current"] <= CUR_BAR]
     head = max(ok, key=lambda f: form_out[f]["steps_per_s"]) if ok else BASE_FORM
     kern = base["kern"]
-    # algorithmic bytes of one middle-term launch: read H^(n-1) psi (16 B) and write H^n psi (16 B)
-    # per point-orbital, plus V_local once (8 B per point)
+    # algorithmic bytes of one launch of the terms without the Taylor sum (first and middle terms, the
+    # same kernel and work): read H^(n-1) psi (16 B) and write H^n psi (16 B) per point-orbital, plus
+    # V_local once (8 B per point); the average duration is over all those launches in the timed region
     alg_bytes = 32.0 * npts * norb + 8.0 * npts
-    avg_s = kern["hpsi_taylor"]["avg_ms"] / 1e3
-    achieved = alg_bytes / avg_s / 1e9
+    k1, km = kern["hpsi_taylor1"], kern["hpsi_taylor"]
+    nl = k1["launches"] + km["launches"]
+    avg_ms = (k1["avg_ms"] * k1["launches"] + km["avg_ms"] * km["launches"]) / max(nl, 1)
+    achieved = alg_bytes / (avg_ms / 1e3) / 1e9
     out = {
         "metric": METRIC,
         "value": form_out[head]["steps_per_s"],
@@ -308,15 +311,16 @@ def main():
         "forms": form_out,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_stencilz (middle Taylor terms of pc-fp64: 25-point stencil + V + k.grad on y tiles "
-                      "with merged halo rows and the z part formed 4 planes late, LDS-DMA fed)",
+            "kernel": "k_stencilz (first and middle Taylor terms of pc-fp64: 25-point stencil + V + k.grad on y "
+                      "tiles with merged halo rows and the z part formed 4 planes late, LDS-DMA fed)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(args.config, "k_stencilz<d2,double,d2>"),
             "alg_bytes_per_launch": alg_bytes,
-            "avg_launch_ms": kern["hpsi_taylor"]["avg_ms"],
+            "avg_launch_ms": avg_ms,
+            "launches": nl,
         },
         "kernels_ms": {k: round(v["avg_ms"], 5) for k, v in kern.items()},
     }
