@@ -164,8 +164,10 @@ int salmon_hip_autotune(salmon_hip_t h, int reps, int *choice, double *best_ms);
 int salmon_hip_set_launch_shape(salmon_hip_t h, int whole_plane, int z_chunks);
 
 /* Run the KB projections on a second stream of the handle, concurrently with the stencil
-   pass that reads the same orbitals (joined by events before the scatter).  Off by default:
-   at c2 the two passes share HBM and the step gains about 1%. */
+   pass that reads the same orbitals (joined by events before the scatter).  Off by default.
+   With it on, Taylor chains keep a separate projection and scatter per term instead of the
+   pass that fuses a term's KB scatter with the next term's projection (faster at 64 atoms,
+   32^3 points, 1024 orbital blocks: 177 against about 172 steps/s in FP64). */
 int salmon_hip_set_overlap(salmon_hip_t h, int on);
 
 /*
