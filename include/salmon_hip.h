@@ -137,6 +137,19 @@ int salmon_hip_taylor_reuse(salmon_hip_t h, const double *zc, int n_hamil, void 
                             void *sshtpsi, const double *dv_dev);
 
 /*
+ * Device-side gate (no counterpart in the reference; for a predictor whose acceptance is
+ * decided on the device, with no host synchronisation).  While flag_dev (a device int) is
+ * set here, the launches of salmon_hip_hpsi / _taylor / _taylor_reuse, salmon_hip_density
+ * and salmon_hip_hartree / _update_vlocal / _update_potential read *flag_dev when they run
+ * and do nothing if it is non-zero; current, dipole and the phase table are never gated.
+ * flag_dev = NULL clears it.  The flag's value is read on the device, in stream order.
+ */
+int salmon_hip_set_skip(salmon_hip_t h, const int *flag_dev);
+
+/* dst_dev[i] = src_dev[i] for i < n when *flag_dev != 0 (device arrays; on the handle's stream). */
+int salmon_hip_copy_if(salmon_hip_t h, const int *flag_dev, double *dst_dev, const double *src_dev, size_t n);
+
+/*
  * Arithmetic of salmon_hip_taylor (no counterpart in the reference, which is
  * FP64 throughout).  SALMON_HIP_PREC_FP64 (default): as above.
  * SALMON_HIP_PREC_MIXED: H psi, H^2 psi, ... are formed and kept as complex FP32
@@ -243,7 +256,7 @@ int salmon_hip_update_potential(salmon_hip_t h, const double *rho_dev, const dou
  * counter, and the per-step current curr(3*nspin) and dipole/electron count (4) are
  * written into per-system DEVICE series at row itt-1, so that a captured step can be
  * replayed.  FP64 only.  All handles are moved onto `stream` (NULL: the first
- * handle's own stream; the null stream cannot be captured).  Calls on a batch and
+ * handle's own stream).  Calls on a batch and
  * its handles must not overlap.
  *   batch_set_system: buffers of system i (psi_a holds the initial orbitals; the two
  *     orbital buffers alternate by step), Ac_tot table, series of nt rows.
@@ -252,9 +265,13 @@ int salmon_hip_update_potential(salmon_hip_t h, const double *rho_dev, const dou
  *   batch_autotune: times the packed stencil for each candidate (z-chunks per orbital:
  *     the systems' rule, 1 or 2; persistent blocks: 256, 512, 1024, 2048 or one per
  *     tile) and keeps the fastest; *choice = 100*z + blocks/256 (99 = one per tile).
- *   batch_step: enqueue one step; batch_capture: capture two consecutive steps (both
- *     buffer parities) into a hipGraph; batch_replay: launch it npairs times.
- *   batch_set_time: set the device step counter.
+ *   batch_step: enqueue one step; SALMON_HIP_EINVAL once the step counter has passed the
+ *     shortest system's nt (the field tables end there).
+ *   batch_set_time: set the step counter (1 .. nt_min + 1).
+ *   batch_capture / batch_replay: graph capture of two consecutive steps and its replay;
+ *     switched off (they return SALMON_HIP_EUNSUPPORTED): a stream capture of the step,
+ *     with its per-system launches forked onto side streams, once ended in a host
+ *     segmentation fault whose cause was not found.
  */
 typedef struct salmon_hip_batch *salmon_hip_batch_t;
 int salmon_hip_batch_create(const salmon_hip_t *handles, int n, void *stream, salmon_hip_batch_t *out);

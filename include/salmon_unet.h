@@ -72,6 +72,15 @@ int salmon_unet_target(salmon_unet_t u, const double *vmid, const double *v0, co
                        float *t_dev);
 int salmon_unet_residual(salmon_unet_t u, const double *vpred, const double *v0, const double *vnew, double *out_dev);
 
+/*
+ * Acceptance decided on the device (no host synchronisation): *flag_dev = (*resid_dev <= tol);
+ * stats_dev[0] += the flag, stats_dev[1] += 1 (accepted / attempted steps); while
+ * stats_dev[1] < log_cap, log_dev[stats_dev[1]] = the residual (log_dev may be null with
+ * log_cap = 0).  The flag is meant for salmon_hip_set_skip / salmon_hip_copy_if.
+ */
+int salmon_unet_gate(salmon_unet_t u, const double *resid_dev, double tol, int *flag_dev, int *stats_dev,
+                     double *log_dev, int log_cap);
+
 #ifdef __cplusplus
 }
 #endif

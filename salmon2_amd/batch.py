@@ -2,9 +2,11 @@
 
 Each system has its own Handle (grid, orbitals, projectors, fixed V_local) and its
 own pulse; the batch runs the fixed-potential RT step for all of them with one
-persistent stencil kernel per halo class and per Taylor term, A(t) tables and the
-step counter on the device, and (``graph=True``) two steps captured in a hipGraph
-and replayed.  torch provides the device buffers and the stream.
+persistent stencil kernel per halo class and per Taylor term, the per-system
+kernels on side streams, and A(t) tables and the step counter on the device.
+Launches are eager: the library's graph capture of the batched step is switched
+off (salmon_hip_batch_capture reports it).  torch provides the device buffers and
+the stream.
 """
 from __future__ import annotations
 
@@ -15,13 +17,15 @@ from .hip import Batch, Handle
 
 
 class BatchRunner:
-    def __init__(self, systems, nt, device="cuda:0"):
+    def __init__(self, systems, nt, device="cuda:0", psi=None):
+        """psi: optional initial orbitals per system (device tensors or host arrays), for
+        systems built with host_orbitals=False."""
         self.systems = list(systems)
+        self.psi_init = list(psi) if psi is not None else [None] * len(self.systems)
         self.nt = nt
         self.dev = torch.device(device)
         torch.cuda.set_device(self.dev)
-        # a stream of its own (the null stream cannot be captured); torch work on the batch's
-        # buffers is issued on it too
+        # a stream of its own; torch work on the batch's buffers is issued on it too
         self.stream = torch.cuda.Stream(self.dev)
         with torch.cuda.stream(self.stream):
             self._setup(nt)
@@ -32,7 +36,13 @@ class BatchRunner:
         self.buf = []
         zc = None
         for i, s in enumerate(self.systems):
-            psi_a = torch.from_numpy(np.ascontiguousarray(s.psi)).to(self.dev)
+            src = self.psi_init[i] if self.psi_init[i] is not None else s.psi
+            if src is None:
+                raise ValueError(f"system {i}: no orbitals (pass psi)")
+            if isinstance(src, torch.Tensor):
+                psi_a = src.to(self.dev, dtype=torch.complex128).contiguous().clone()
+            else:
+                psi_a = torch.from_numpy(np.ascontiguousarray(src)).to(self.dev)
             b = dict(psi0=psi_a.clone(), psi_a=psi_a, psi_b=torch.zeros_like(psi_a), work=torch.zeros_like(psi_a),
                      rho=torch.zeros((s.nspin, s.nz, s.ny, s.nx), dtype=torch.float64, device=self.dev),
                      ac=torch.from_numpy(np.ascontiguousarray(s.pulse.table(s.dt, nt) if s.pulse else
@@ -54,15 +64,15 @@ class BatchRunner:
                 b["psi_a"].copy_(b["psi0"])
             self.batch.set_time(1)
 
-    def run(self, nsteps, graph=False):
-        if graph:
-            self.batch.capture()
-            self.batch.replay(nsteps // 2)
-            if nsteps % 2:
-                self.batch.step()
-        else:
-            for _ in range(nsteps):
-                self.batch.step()
+    def run(self, nsteps):
+        for _ in range(nsteps):
+            self.batch.step()
+
+    def close(self):
+        torch.cuda.synchronize(self.dev)
+        self.batch.close()
+        for h in self.handles:
+            h.close()
 
     def results(self, nsteps):
         torch.cuda.synchronize(self.dev)

@@ -442,6 +442,20 @@ __global__ __launch_bounds__(256) void k_resid_final(const double *part, int nb,
   if (threadIdx.x == 0) *out = sh[0];
 }
 
+// The gate of a step propagated with a predicted potential, decided on the device: flag =
+// (resid <= tol); stats[0] counts accepted, stats[1] attempted steps; log[stats[1]] = resid while
+// the log has room.
+__global__ void k_gate(const double *resid, double tol, int *flag, int *stats, double *log, int log_cap) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double r = *resid;
+  const int ok = r <= tol ? 1 : 0;
+  const int n = stats[1];
+  if (log && n < log_cap) log[n] = r;
+  *flag = ok;
+  stats[0] += ok;
+  stats[1] = n + 1;
+}
+
 int ufail(int code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
@@ -855,6 +869,17 @@ int salmon_unet_residual(salmon_unet_t u, const double *vpred, const double *v0,
   ULAUNCH("k_resid_part");
   hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, u->stream, u->d_rpart, nb, out_dev);
   ULAUNCH("k_resid_final");
+  return 0;
+}
+
+int salmon_unet_gate(salmon_unet_t u, const double *resid_dev, double tol, int *flag_dev, int *stats_dev,
+                     double *log_dev, int log_cap) {
+  if (int rc = ucheck(u)) return rc;
+  if (!resid_dev || !flag_dev || !stats_dev || (log_cap > 0 && !log_dev))
+    return ufail(SALMON_HIP_EINVAL, "null pointer");
+  hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, u->stream, resid_dev, tol, flag_dev, stats_dev,
+                     log_cap > 0 ? log_dev : nullptr, log_cap);
+  ULAUNCH("k_gate");
   return 0;
 }
 

@@ -30,6 +30,8 @@ EXPORTS = (
     "salmon_hip_hpsi",
     "salmon_hip_taylor",
     "salmon_hip_taylor_reuse",
+    "salmon_hip_set_skip",
+    "salmon_hip_copy_if",
     "salmon_hip_set_precision",
     "salmon_hip_autotune",
     "salmon_hip_set_launch_shape",
@@ -96,6 +98,8 @@ def load():
     L.salmon_hip_hpsi.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_taylor.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_taylor_reuse.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.salmon_hip_set_skip.argtypes = [C.c_void_p, C.c_void_p]
+    L.salmon_hip_copy_if.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
     L.salmon_hip_set_precision.argtypes = [C.c_void_p, C.c_int]
     L.salmon_hip_autotune.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
     L.salmon_hip_set_launch_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -224,6 +228,18 @@ class Handle:
         z = np.ascontiguousarray(np.asarray(zc, dtype=np.complex128)).view(np.float64)
         _chk(load().salmon_hip_taylor_reuse(self.h, _p(z), len(z) // 2, C.c_void_p(_ptr(tin)), C.c_void_p(_ptr(tout)),
                                             C.c_void_p(_ptr(work)), C.c_void_p(_ptr(dv))), "taylor_reuse")
+
+    def set_skip(self, flag):
+        """Device-side gate: while set, hpsi/taylor, density and the potential update do nothing
+        when the device int `flag` is non-zero (None clears)."""
+        _chk(load().salmon_hip_set_skip(self.h, C.c_void_p(None if flag is None else _ptr(flag))), "set_skip")
+
+    def copy_if(self, flag, dst, src):
+        """dst = src (float64 device tensors) when the device int `flag` is non-zero."""
+        n = dst.numel()
+        assert src.numel() == n and dst.dtype == src.dtype
+        _chk(load().salmon_hip_copy_if(self.h, C.c_void_p(_ptr(flag)), C.c_void_p(_ptr(dst)), C.c_void_p(_ptr(src)),
+                                       n), "copy_if")
 
     PREC = {"fp64": 0, "mixed": 1}
 

@@ -78,8 +78,8 @@ def generate(sysm, pulses, nt, precision="fp64"):
         prop.setup_outputs(nt)
         for _ in range(nt):
             prop.step()
-        hist = [v0] + [v for v, _ in prop.record]          # V_0 .. V_nt
-        mids = [m for _, m in prop.record]                  # midpoint of step itt = 1..nt
+        hist = [v0] + [r[0] for r in prop.record]          # V_0 .. V_nt
+        mids = [r[1] for r in prop.record]                  # midpoint of step itt = 1..nt
         for itt in range(4, nt + 1):
             samples.append(([hist[itt - 1], hist[itt - 2], hist[itt - 3], hist[itt - 4]], mids[itt - 1]))
         prop.h.close()

@@ -16,8 +16,6 @@ import numpy as np
 
 from . import hip
 
-DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
-SI_RPS = os.path.join(DATA, "Si_rps.dat")  # SALMON testsuites/pseudo/Si_rps.dat
 EXPORTS = ("salmon_pp_create", "salmon_pp_destroy", "salmon_pp_element", "salmon_pp_sizes", "salmon_pp_nonlocal",
            "salmon_pp_vpsl")
 

@@ -21,13 +21,15 @@ leaves its input untouched, so the orbital copy of the reference
 (psi_tmp, :370-372, 410-412) is not needed.
 
 ``learned=Predictor`` (salmon2_amd.learned) replaces the predictor Taylor
-step by the network's midpoint potential: one Taylor step with it, accepted
-when the self-consistency residual max|V_pred - (V(t) + V(t+dt))/2| is within
-the predictor's tolerance; otherwise ``fallback="plain"`` redoes the step
-with the plain predictor-corrector and ``"reuse"`` takes the attempt as the
-predictor (one corrector Taylor step).  ``backoff=(n, period)`` stops attempting after n
-rejections in a row and then attempts once every ``period`` steps until one is accepted (the
-steps in between are plain predictor-corrector steps); None attempts every step.
+step by the network's midpoint potential V_pred: one Taylor step with it, then
+the self-consistency residual max|V_pred - (V(t) + V[rho'])/2| is compared with
+the predictor's tolerance ON THE DEVICE (salmon_unet_gate).  The flag it sets
+gates the corrector branch (salmon_hip_set_skip): when the step is rejected,
+the attempt serves as the predictor of the predictor-corrector -- V_mid =
+(V(t) + V[rho'])/2, then the corrector Taylor step from the same orbitals --
+and when it is accepted those launches do nothing and the attempt's orbitals,
+density and potential stand.  No host synchronisation in the step; accepted
+and attempted counts and the residuals are read back once (predictor_stats).
 
 ``precision="mixed"`` runs the Taylor chain H^n psi in complex FP32 with the
 projections, the Taylor sum, density and current in FP64.
@@ -54,7 +56,7 @@ from .hip import Handle
 
 class RTPropagator:
     def __init__(self, sysm, device="cuda:0", psi=None, scf=False, pc=False, precision="fp64", trans_longi="tr",
-                 learned=None, fallback="plain", record=False, backoff=None):
+                 learned=None, record=False):
         self.s = sysm
         self.dev = torch.device(device)
         torch.cuda.set_device(self.dev)
@@ -62,7 +64,12 @@ class RTPropagator:
         self.h = Handle(sysm, stream=self.stream)
         self.h.set_precision(precision)
         src = sysm.psi if psi is None else psi
-        self.psi_a = torch.from_numpy(np.ascontiguousarray(src)).to(self.dev)
+        if src is None:
+            raise ValueError("no orbitals: pass psi (system built with host_orbitals=False)")
+        if isinstance(src, torch.Tensor):
+            self.psi_a = src.to(self.dev, dtype=torch.complex128).contiguous().clone()
+        else:
+            self.psi_a = torch.from_numpy(np.ascontiguousarray(src)).to(self.dev)
         self.psi_b = torch.zeros_like(self.psi_a)
         self.work = torch.zeros_like(self.psi_a)
         self.rho = torch.zeros((sysm.nspin, sysm.nz, sysm.ny, sysm.nx), dtype=torch.float64, device=self.dev)
@@ -88,9 +95,6 @@ class RTPropagator:
         if learned is not None:
             if not self.pc:
                 raise ValueError("the learned predictor replaces the predictor of the predictor-corrector (scf=pc=True)")
-            if fallback not in ("plain", "reuse"):
-                raise ValueError(f"fallback {fallback!r}: 'plain' or 'reuse'")
-            self.fallback = fallback
             self.hist = [torch.zeros_like(self.vpsl) for _ in range(4)]  # V(t), V(t-dt), V(t-2dt), V(t-3dt)
             self.hist[0].copy_(self.vcur)
             self.nhist = 1
@@ -103,15 +107,11 @@ class RTPropagator:
             self.pred_ready = False
             self.vnew = torch.zeros_like(self.vpsl)
             self.resid = torch.zeros(1, dtype=torch.float64, device=self.dev)
-            self.accepted = 0
-            self.attempted = 0
-            self.resid_log = []
-            if backoff is not None and (int(backoff[0]) < 1 or int(backoff[1]) < 1):
-                raise ValueError(f"backoff {backoff!r}: (rejections >= 1, period >= 1)")
-            self.backoff = None if backoff is None else (int(backoff[0]), int(backoff[1]))
-            self.fails = 0  # rejections since the last accepted attempt
-            self.last_attempt = 0
+            self.flag = torch.zeros(1, dtype=torch.int32, device=self.dev)  # 1: this step's attempt accepted
+            self.stats = torch.zeros(2, dtype=torch.int32, device=self.dev)  # accepted, attempted
+            self.resid_log = torch.zeros(0, dtype=torch.float64, device=self.dev)
         self.record = [] if record else None
+        self._vpred_rec = None
         # the current of a predictor-corrector step runs on a second stream while the density and
         # the potential update run on the first (independent reads of the new orbitals); joined at
         # the end of the step, before the next phase-table update
@@ -128,6 +128,9 @@ class RTPropagator:
         self.ac = self.ac_ext.copy()  # Ac_tot
         self.curr_t = torch.zeros((nt, s.nspin, 3), dtype=torch.float64, device=self.dev)
         self.dip_t = torch.zeros((nt, 4), dtype=torch.float64, device=self.dev)
+        if self.learned is not None:
+            self.stats.zero_()
+            self.resid_log = torch.zeros(nt, dtype=torch.float64, device=self.dev)
 
     def step(self):
         """One RT step itt -> itt+1 (time_evolution_step)."""
@@ -135,8 +138,8 @@ class RTPropagator:
         itt = self.itt
         A = 0.5 * (self.ac[itt] + self.ac[itt - 1])
         self.h.update_kvector(A)
-        if self.learned is not None and self.nhist >= 4 and self._attempt_due(itt) and self._learned_step():
-            pass
+        if self.learned is not None and self.nhist >= 4:
+            self._learned_step()
         else:
             if self.pc:
                 self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
@@ -153,7 +156,7 @@ class RTPropagator:
             self.hist = [self.hist[3]] + self.hist[:3]
             self.hist[0].copy_(self.vcur)
             self.nhist = min(self.nhist + 1, 4)
-            if self.nhist >= 4 and self._attempt_due(itt + 1):
+            if self.nhist >= 4:
                 P = self.learned
                 self.ev_v.record(self.stream)
                 self.side.wait_event(self.ev_v)
@@ -163,7 +166,9 @@ class RTPropagator:
                 self.ev_pred.record(self.side)
                 self.pred_ready = True
         if self.record is not None:
-            self.record.append((self.vcur.clone(), self.vmid.clone()))
+            # (V(t+dt), the step's midpoint potential, the prediction the step tried or None)
+            self.record.append((self.vcur.clone(), self.vmid.clone(), self._vpred_rec))
+            self._vpred_rec = None
         if not self.cur_async:
             self.h.current(self.ac[itt], self.psi_b, self.curr_t[itt - 1])
         self.h.dipole(self.rho, self.dip_t[itt - 1])
@@ -194,52 +199,52 @@ class RTPropagator:
         torch.sub(self.vmid, v_pred, out=self.dv)
         self.h.taylor_reuse(self.zc, self.psi_a, self.psi_b, self.work, self.dv)
 
-    def _attempt_due(self, itt):
-        """Whether step itt tries the learned predictor (every step, or under back-off once
-        every period steps after the set number of rejections in a row)."""
-        if self.backoff is None or self.fails < self.backoff[0]:
-            return True
-        return itt - self.last_attempt >= self.backoff[1]
-
     def _learned_step(self):
-        """Predictor step by the network; True when the step is complete (accepted or fallen back)."""
+        """A step from the network's midpoint potential, gated on the device (see the module notes)."""
         P = self.learned
-        self.last_attempt = self.itt
         if self.pred_ready:
             self.stream.wait_event(self.ev_pred)
             self.pred_ready = False
         else:
             P.predict(self.hist, self.vpsl, self.vpred)
+        if self.record is not None:
+            self._vpred_rec = self.vpred.clone()
         self.h.set_vlocal_device(self.vpred)
         self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
         self.h.density(self.psi_b, self.rho)
         self.h.update_potential(self.rho, self.vpsl, vout=self.vnew)
         P.net.residual(self.vpred, self.vcur, self.vnew, self.resid)
-        r = float(self.resid.item())  # synchronises the step
-        self.attempted += 1
-        self.resid_log.append(r)
-        if r <= getattr(P, "tol_run", P.tol):
-            self.accepted += 1
-            self.fails = 0
-            self.vmid.copy_(self.vpred)
-            self.vcur, self.vnew = self.vnew, self.vcur
-            return True
-        self.fails += 1
-        if self.fallback == "reuse":
-            # the attempt serves as the predictor: V = (V(t) + V[rho'])/2, then the corrector
+        P.net.gate(self.resid, getattr(P, "tol_run", P.tol), self.flag, self.stats, self.resid_log)
+        # rejected: the attempt is the predictor, V_mid = (V(t) + V[rho'])/2, then the corrector,
+        # the density and V(t+dt); accepted: these launches do nothing
+        self.h.set_skip(self.flag)
+        try:
             self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)
             self._corrector(self.vpred)
-        else:
-            # the plain predictor: a Taylor step with V(t)
-            self.h.set_vlocal_device(self.vcur)
-            self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
             self.h.density(self.psi_b, self.rho)
-            self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)
-            self._corrector(self.vcur)
+            self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
+        finally:
+            self.h.set_skip(None)
+        # accepted: V(t+dt) = V[rho'] (the handle's V_local already), the step's midpoint = V_pred
+        self.h.copy_if(self.flag, self.vcur, self.vnew)
+        self.h.copy_if(self.flag, self.vmid, self.vpred)
         self._current_async(self.itt)
-        self.h.density(self.psi_b, self.rho)
-        self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
-        return True
+
+    def predictor_stats(self):
+        """(accepted, attempted, median residual) of the learned attempts since setup_outputs."""
+        torch.cuda.synchronize(self.dev)
+        acc, att = (int(v) for v in self.stats.cpu().numpy())
+        n = min(att, self.resid_log.numel())
+        med = float(np.median(self.resid_log[:n].cpu().numpy())) if n else None
+        return acc, att, med
+
+    @property
+    def accepted(self):
+        return self.predictor_stats()[0]
+
+    @property
+    def attempted(self):
+        return self.predictor_stats()[1]
 
     def run(self, nt=None):
         nt = self.s.nt if nt is None else nt
@@ -247,6 +252,11 @@ class RTPropagator:
         for _ in range(nt):
             self.step()
         return self.results()
+
+    def close(self):
+        """Release the device handle (buffers go with the object)."""
+        torch.cuda.synchronize(self.dev)
+        self.h.close()
 
     def results(self):
         torch.cuda.synchronize(self.dev)

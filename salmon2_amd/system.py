@@ -141,6 +141,7 @@ class System:
     n_hamil: int = 4
     pulse: Pulse | None = None
     seed: int = 0
+    psi_rng_state: dict | None = None  # generator state before the orbital draws
 
     @property
     def ngrid(self) -> int:
@@ -346,6 +347,37 @@ def random_orbitals(rng, n, ly, no, nk, nspin, hvol):
     return psi
 
 
+def device_orbitals(s, device="cuda"):
+    """The orbitals of ``random_orbitals`` for system s, drawn from the same generator state and
+    orthonormalised on `device` (torch tensor, the padded layout of ``System.psi``).
+
+    The Gram-Schmidt factor is formed as a Cholesky QR: G = Z^H Z hvol, R = chol(G)^H with a
+    positive diagonal, Q = Z R^-1 -- the same unique factorisation as the host path's Householder
+    QR with fixed column phases, to rounding (Z is a tall Gaussian matrix, condition ~1.1)."""
+    import torch
+
+    if s.psi_rng_state is None:
+        raise ValueError("system has no recorded orbital generator state")
+    rng = np.random.default_rng()
+    rng.bit_generator.state = s.psi_rng_state
+    nx, ny, nz = s.nx, s.ny, s.nz
+    N = nx * ny * nz
+    out = torch.zeros((s.nk, s.no, s.nspin, nz, s.ly, nx), dtype=torch.complex128, device=device)
+    for ik in range(s.nk):
+        for isp in range(s.nspin):
+            re = rng.standard_normal((s.no, N))
+            im = rng.standard_normal((s.no, N))
+            Zt = torch.complex(torch.from_numpy(re), torch.from_numpy(im)).to(device)  # (no, N) = Z^T
+            G = (Zt.conj() @ Zt.transpose(0, 1)).cpu().numpy() * s.hvol  # Z^H Z hvol
+            L = np.linalg.cholesky(G)  # G = L L^H, R = L^H
+            Rinv = np.linalg.inv(L.conj().T)
+            # Q^T = (Z R^-1)^T = R^-T Z^T
+            Qt = torch.from_numpy(np.ascontiguousarray(Rinv.T)).to(device) @ Zt
+            out[ik, :, isp, :, :ny, :] = Qt.reshape(s.no, nz, ny, nx)
+            del Zt, Qt
+    return out
+
+
 def make_si_system(
     ncell=(1, 1, 1),
     n=16,
@@ -362,8 +394,13 @@ def make_si_system(
     pad=1,
     rc=1.5,
     rcut=2.5,
+    host_orbitals=True,
 ):
-    """Seeded diamond-Si supercell of the kind the benchmark configurations name."""
+    """Seeded diamond-Si supercell of the kind the benchmark configurations name.
+
+    host_orbitals=False leaves ``psi`` None and keeps the generator state in
+    ``psi_rng_state``: ``device_orbitals`` then draws the same numbers and orthonormalises
+    them on a GPU (the host QR of the 216-atom workload takes minutes)."""
     ncell = tuple(ncell)
     nn = (n, n, n) if np.isscalar(n) else tuple(n)
     al = np.array([SI_A * ncell[0], SI_A * ncell[1], SI_A * ncell[2]])
@@ -391,7 +428,8 @@ def make_si_system(
         rinv = np.zeros(0)
     nspin = 1
     ly = nn[1] + pad
-    psi = random_orbitals(rng, nn, ly, no, nk, nspin, hvol)
+    rng_state = rng.bit_generator.state
+    psi = random_orbitals(rng, nn, ly, no, nk, nspin, hvol) if host_orbitals else None
     rocc = np.full((nspin, nk, no), 2.0)
     pulse = Pulse(A0=A0, omega=omega, tw=nt * dt)
     return System(
@@ -429,13 +467,15 @@ def make_si_system(
         n_hamil=n_hamil,
         pulse=pulse,
         seed=seed,
+        psi_rng_state=rng_state,
     )
 
 
-def make_si_pp_system(ncell=(1, 1, 1), n=12, no=16, nkgrid=(2, 2, 2), seed=20261212, dt=0.08, nt=100, A0=0.01,
-                      omega=0.5, n_hamil=4, name="si_pp", pad=1, pp_file=None, lmax_ps=2, lloc_ps=2):
-    """Diamond-Si supercell with the tables of a real pseudopotential (default SALMON's
-    testsuites/pseudo/Si_rps.dat with Lmax_ps = Lloc_ps = 2, as its bulk-Si tests use):
+def make_si_pp_system(pp_file, ncell=(1, 1, 1), n=12, no=16, nkgrid=(2, 2, 2), seed=20261212, dt=0.08, nt=100,
+                      A0=0.01, omega=0.5, n_hamil=4, name="si_pp", pad=1, lmax_ps=2, lloc_ps=2):
+    """Diamond-Si supercell with the tables of a real KY-format pseudopotential `pp_file`
+    (e.g. SALMON's testsuites/pseudo/Si_rps.dat with Lmax_ps = Lloc_ps = 2, as its bulk-Si
+    tests use):
     V_local = Vpsl and the KB projectors come from the projector-table builder
     (include/salmon_pp.h); the orbitals are seeded random as in make_si_system."""
     from . import pp as _pp
@@ -447,7 +487,7 @@ def make_si_pp_system(ncell=(1, 1, 1), n=12, no=16, nkgrid=(2, 2, 2), seed=20261
     hvol = float(al[0] * al[1] * al[2]) / (nn[0] * nn[1] * nn[2])
     rion = si_positions(ncell)
     kion = np.ones(len(rion), dtype=np.int32)
-    P = _pp.PseudoPotential([pp_file or _pp.SI_RPS], lloc_ps=[lloc_ps], lmax_ps=[lmax_ps])
+    P = _pp.PseudoPotential([pp_file], lloc_ps=[lloc_ps], lmax_ps=[lmax_ps])
     mps, nps, jxyz, rxyz, nlma, ia_tbl, rinv, uv = P.nonlocal_tables(kion, rion, al, nn)
     V = P.vpsl(kion, rion, al, nn)
     P.close()
@@ -484,16 +524,16 @@ def config(name: str, **over) -> System:
     }
     kw = dict(table[name])
     kw.update(over)
+
     kw.setdefault("name", name)
     return make_si_system(**kw)
 
 
-def batch_systems(n=24, seed=20261205, nt=100):
+def batch_systems(n=24, seed=20261205, nt=100, host_orbitals=True):
     """Seeded uneven systems of the batched workload: Si supercells of 8..128 atoms, grid extents
     16..40 (step 4) chosen per axis so that the spacing stays in [0.55, 1.1] bohr (the 4th-order
     Taylor step with dt = 0.08 is stable for |dt E| < 2 sqrt 2, E_max ~ 9.75/h^2 for random
-    orbitals), 16..256 orbitals, 1..27 k-points (orbitals x k-points capped at 512 to bound the
-    host-side orbital generation and memory)."""
+    orbitals), 16..256 orbitals, 1..27 k-points."""
     a = 10.26
     rng = np.random.default_rng(seed)
     cells = [(1, 1, 1), (1, 1, 2), (2, 2, 1), (2, 2, 2), (2, 2, 4)]
@@ -506,10 +546,7 @@ def batch_systems(n=24, seed=20261205, nt=100):
             cand = [g for g in range(16, 44, 4) if 0.55 <= a * m / g <= 1.1]
             grid.append(int(rng.choice(cand)))
         kg = kgrids[rng.integers(len(kgrids))]
-        nk = int(np.prod(kg))
         no = int(rng.choice([16, 32, 64, 128, 256]))
-        while no * nk > 512 and no > 16:
-            no //= 2
         out.append(make_si_system(ncell=cell, n=tuple(grid), no=no, nkgrid=kg, seed=seed + 1 + i, nt=nt,
-                                  name=f"batch{i}"))
+                                  name=f"batch{i}", host_orbitals=host_orbitals))
     return out

@@ -22,6 +22,7 @@ EXPORTS = (
     "salmon_unet_predict",
     "salmon_unet_target",
     "salmon_unet_residual",
+    "salmon_unet_gate",
     "salmon_unet_get_activation",
 )
 
@@ -48,6 +49,7 @@ def _lib():
         L.salmon_unet_predict.argtypes = [vp, vp, vp, vp, C.c_double, vp]
         L.salmon_unet_target.argtypes = [vp, vp, vp, vp, C.c_double, vp]
         L.salmon_unet_residual.argtypes = [vp, vp, vp, vp, vp]
+        L.salmon_unet_gate.argtypes = [vp, vp, C.c_double, vp, vp, vp, C.c_int]
         L.salmon_unet_get_activation.argtypes = [vp, C.c_int, fp, C.c_long]
         _bound = True
     return L
@@ -137,6 +139,13 @@ class UNet:
 
     def residual(self, vpred, v0, vnew, out):
         hip._chk(_lib().salmon_unet_residual(self.h, _dp(vpred), _dp(v0), _dp(vnew), _dp(out)), "salmon_unet_residual")
+
+    def gate(self, resid, tol, flag, stats, log=None):
+        """flag = (resid <= tol) on the device; stats[0] accepted, stats[1] attempted (device int32);
+        log[attempt] = resid while it has room."""
+        cap = 0 if log is None else int(log.numel())
+        hip._chk(_lib().salmon_unet_gate(self.h, _dp(resid), float(tol), _dp(flag), _dp(stats), _dp(log), cap),
+                 "salmon_unet_gate")
 
     def adam_step(self, lr, nsamples):
         hip._chk(_lib().salmon_unet_adam_step(self.h, float(lr), int(nsamples)), "salmon_unet_adam_step")

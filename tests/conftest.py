@@ -7,6 +7,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# SALMON's testsuites/pseudo/Si_rps.dat (a data file of the reference's tests, KY format)
+SI_RPS = os.path.join(ROOT, "tests", "data", "Si_rps.dat")
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")

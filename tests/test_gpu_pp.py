@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from salmon2_amd import system
+from tests.conftest import SI_RPS
 
 pytestmark = pytest.mark.gpu
 
@@ -28,7 +29,7 @@ def torch_dev():
 @pytest.fixture(scope="module")
 def si_pp():
     # the bulk-Si testsuite geometry: 8 atoms, a = 10.26 bohr, 12^3 grid
-    return system.make_si_pp_system(n=12, no=8, nkgrid=(1, 1, 2), nt=20)
+    return system.make_si_pp_system(SI_RPS, n=12, no=8, nkgrid=(1, 1, 2), nt=20)
 
 
 def test_hpsi_real_pp(torch_dev, oracle_lib, si_pp):

@@ -1,0 +1,142 @@
+"""Full-length RT trajectories on the GPU against the oracle's (golden fixtures).
+
+tests/golden/make_trajectories.py ran the oracle (the C restatement of SALMON's CPU loops,
+time_evolution_step.f90:116-253 driven as main_tddft.f90:89-130) on the named workloads and
+stored J(t), the dipole and the electron count of every step.  Here the same seeded inputs
+are regenerated (their fingerprint is checked first, so a changed generator is told apart
+from a wrong kernel), the HIP path runs the same number of steps, and the series must agree:
+
+  * plain FP64: <= 1e-9 relative (max over the run / max of the oracle's series), for the
+    fixed-potential step and the self-consistent predictor-corrector step;
+  * mixed precision (complex-FP32 Taylor chain, FP64 sums): its error against the FP64
+    oracle is printed and bounded by 2e-6 (current, dipole) and 1e-7 (electron count).
+
+The 216-atom workload (48^3, 4 k) runs at a reduced orbital count for its full 500 steps and
+at its full 432 orbitals for the 10 steps the oracle covers; at full size the mixed form is
+also run for all 500 steps against the FP64 GPU path, with the orbital norms checked.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests.golden import make_trajectories as MT
+
+pytestmark = pytest.mark.gpu
+
+FP64_BAR = 1e-9
+MIXED_BAR = 2e-6
+MIXED_NE_BAR = 1e-7
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
+    from salmon2_amd import hip
+
+    hip.load()
+    return torch
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
+
+
+def _golden(case):
+    p = MT.path_of(case)
+    if not os.path.exists(p):
+        pytest.skip(f"fixture {os.path.basename(p)} not generated")
+    return dict(np.load(p, allow_pickle=False))
+
+
+def _system(case, torch):
+    """The case's seeded system and its orbitals (host array, or generated on the GPU for
+    the 216-atom workload), with the fingerprint checked against the fixture's."""
+    c = MT.CASES[case]
+    big = c["config"] == "c4" and "no" not in c["over"]
+    s = MT.system_of(case, host_orbitals=not big)
+    if big:
+        from salmon2_amd import system
+
+        psi = system.device_orbitals(s, "cuda")
+    else:
+        psi = s.psi
+    return s, psi
+
+
+def _check_fp(case, s, psi, g):
+    fp = MT.fingerprint(s, psi)
+    d = np.abs(fp - g["fp"]) / np.maximum(np.abs(g["fp"]), 1e-300)
+    print(f"[parity] {case} input fingerprint rel diff {d.max():.2e}", flush=True)
+    assert d.max() <= 1e-12, (fp, g["fp"])
+
+
+def _run(s, psi, nt, scf, precision):
+    from salmon2_amd.rt import RTPropagator
+
+    p = RTPropagator(s, psi=psi, scf=scf, pc=scf, precision=precision)
+    r = p.run(nt)
+    p.h.close()
+    return r
+
+
+def _errors(r, g):
+    return rel(r["curr"], g["curr"]), rel(r["dip"], g["dip"]), float(np.abs(r["rne"] - g["rne"]).max() / g["rne"][0])
+
+
+@pytest.mark.parametrize("case", ["c2k1_fixv", "c2k1_pc", "c2_fixv", "c4o16_fixv", "c4_fixv10"])
+def test_trajectory_vs_oracle(torch_dev, case):
+    torch = torch_dev
+    g = _golden(case)
+    c = MT.CASES[case]
+    nt, scf = int(g["nt"]), bool(g["scf"])
+    print(f"[gpu] trajectory {case}: {nt} steps, scf={scf}", flush=True)
+    s, psi = _system(case, torch)
+    _check_fp(case, s, psi, g)
+    assert np.array_equal(s.pulse.table(s.dt, nt), g["ac"])
+    r = _run(s, psi, nt, scf, "fp64")
+    ec, ed, en = _errors(r, g)
+    print(f"[parity] {case} fp64 vs oracle over {nt} steps: current {ec:.3e} dipole {ed:.3e} N_e {en:.3e}",
+          flush=True)
+    assert ec <= FP64_BAR and ed <= FP64_BAR and en <= FP64_BAR
+    r = _run(s, psi, nt, scf, "mixed")
+    ec, ed, en = _errors(r, g)
+    print(f"[parity] {case} mixed vs FP64 oracle over {nt} steps: current {ec:.3e} dipole {ed:.3e} N_e {en:.3e}",
+          flush=True)
+    assert ec <= MIXED_BAR and ed <= MIXED_BAR and en <= MIXED_NE_BAR
+    del psi
+    torch.cuda.empty_cache()
+
+
+def test_c4_full_mixed_vs_fp64(torch_dev):
+    """The full 216-atom workload (48^3, 432 orbitals x 4 k) for its 500 steps: mixed against
+    the FP64 GPU path (pinned against the oracle above), and the orbital norms of both."""
+    torch = torch_dev
+    from salmon2_amd import system
+    from salmon2_amd.rt import RTPropagator
+
+    s = system.config("c4", host_orbitals=False)
+    psi = system.device_orbitals(s, "cuda")
+    nt = s.nt
+    out = {}
+    for prec in ("fp64", "mixed"):
+        print(f"[gpu] c4 full {prec}: {nt} steps", flush=True)
+        p = RTPropagator(s, psi=psi, precision=prec)
+        r = p.run(nt)
+        fin = p.psi_a
+        norms = ((fin.real ** 2 + fin.imag ** 2).sum(dim=(2, 3, 4, 5)) * s.hvol).cpu().numpy()
+        out[prec] = (r, float(np.abs(norms - 1.0).max()))
+        p.h.close()
+        del p, fin
+        torch.cuda.empty_cache()
+    r64, d64 = out["fp64"]
+    r32, d32 = out["mixed"]
+    ec, ed = rel(r32["curr"], r64["curr"]), rel(r32["dip"], r64["dip"])
+    en = float(np.abs(r32["rne"] - r64["rne"]).max() / r64["rne"][0])
+    print(f"[parity] c4 full mixed vs fp64 over {nt} steps: current {ec:.3e} dipole {ed:.3e} N_e {en:.3e}; "
+          f"orbital norm drift fp64 {d64:.3e} mixed {d32:.3e}", flush=True)
+    assert ec <= MIXED_BAR and ed <= MIXED_BAR and en <= MIXED_NE_BAR
+    # the 4th-order Taylor step is not unitary: both forms drift alike; mixed adds FP32 round-off
+    assert abs(d32 - d64) <= 1e-6, (d64, d32)

@@ -22,6 +22,7 @@ import pytest
 
 from oracle import pp_oracle as P
 from salmon2_amd import pp, system
+from tests.conftest import SI_RPS
 
 AA = P.AU_LENGTH_AA
 EV = P.AU_ENERGY_EV
@@ -57,12 +58,12 @@ def test_ylm_addition_theorem():
 
 
 def test_reference_si_file():
-    e = P.read_ps_ky(pp.SI_RPS, lmax_ps=2, lloc_ps=2)
+    e = P.read_ps_ky(SI_RPS, lmax_ps=2, lloc_ps=2)
     assert e["mr"] == 2000 and e["zps"] == 4 and e["mlps"] == 2 and e["lref"] == 2
     assert abs(e["rps"] - 1.2 / AA) < 1e-14  # rrc = 1.2 Angstrom for every channel
     assert e["rad"][e["nrps"]] > e["rps"] >= e["rad"][e["nrps"] - 1]
     assert list(e["inorm"]) == [1, 1, 0]  # the d channel is the local one
-    lib = pp.PseudoPotential([pp.SI_RPS], lloc_ps=[2], lmax_ps=[2])
+    lib = pp.PseudoPotential([SI_RPS], lloc_ps=[2], lmax_ps=[2])
     got = lib.element(1)
     assert got == dict(zps=4, mlps=2, lref=2, rps=e["rps"], nrps=e["nrps"], nlma_atom=4)
 
@@ -76,8 +77,8 @@ def test_builder_matches_oracle(case):
         "si8_shifted": _si(12, shift=(-0.7, 0.3, 9.9)),  # atoms outside [0, L): two images per side
     }[case]
     al, rion, n = geo
-    e = P.read_ps_ky(pp.SI_RPS, lmax_ps=2, lloc_ps=2)
-    lib = pp.PseudoPotential([pp.SI_RPS], lloc_ps=[2], lmax_ps=[2])
+    e = P.read_ps_ky(SI_RPS, lmax_ps=2, lloc_ps=2)
+    lib = pp.PseudoPotential([SI_RPS], lloc_ps=[2], lmax_ps=[2])
     kion = np.ones(len(rion), dtype=np.int32)
     mps, nps, jxyz, rxyz, nlma, ia_tbl, rinv, uv = lib.nonlocal_tables(kion, rion, al, n)
     om, oj, orx, oia, orinv, ouv = P.nonlocal_tables([e], [0] * len(rion), rion.tolist(), list(al), n)
@@ -93,7 +94,7 @@ def test_builder_matches_oracle(case):
 
 def test_sphere_points_brute_force():
     al, rion, n = _si((14, 12, 10), ncell=(1, 1, 1), shift=(0.4, -0.2, 0.1))
-    rps = P.read_ps_ky(pp.SI_RPS, lmax_ps=2, lloc_ps=2)["rps"]
+    rps = P.read_ps_ky(SI_RPS, lmax_ps=2, lloc_ps=2)["rps"]
     sp = P.spheres([rps], [0] * len(rion), rion.tolist(), list(al), n)
     h = al / np.array(n)
     g = np.stack(np.meshgrid(*[np.arange(n[d]) * h[d] for d in range(3)], indexing="ij"), -1).reshape(-1, 3)
@@ -218,8 +219,8 @@ def test_builder_rejects_bad_input(tmp_path):
     with pytest.raises(Exception, match="cannot open"):
         pp.PseudoPotential([str(tmp_path / "missing.dat")], lloc_ps=[0])
     with pytest.raises(Exception, match="Lloc_ps"):
-        pp.PseudoPotential([pp.SI_RPS], lloc_ps=[3], lmax_ps=[2])
-    lib = pp.PseudoPotential([pp.SI_RPS], lloc_ps=[2], lmax_ps=[2])
+        pp.PseudoPotential([SI_RPS], lloc_ps=[3], lmax_ps=[2])
+    lib = pp.PseudoPotential([SI_RPS], lloc_ps=[2], lmax_ps=[2])
     al, rion, n = _si(12)
     with pytest.raises(Exception, match="kion"):
         lib.nonlocal_tables(np.full(8, 2, np.int32), rion, al, n)
