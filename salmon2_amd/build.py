@@ -47,9 +47,11 @@ def build_hip(verbose: bool = False) -> str:
 
 FDIR = os.path.join(PKG, "fortran")
 FBUILD = os.path.join(FDIR, "_build")
-FORTRAN_SRCS = ["salmon_hip_mod.f90", "salmon_hip_structures_subset.f90", "salmon_hip_dropin.f90",
-                "test_dropin.f90"]
-FORTRAN_EXE = os.path.join(FDIR, "test_dropin")
+FORTRAN_SRCS = ["salmon_hip_mod.f90", "salmon_hip_structures_subset.f90", "salmon_hip_dropin.f90"]
+# the drop-in's test driver (test infrastructure, built beside the product objects)
+DRIVER_DIR = os.path.join(ROOT, "tests", "fortran")
+DRIVER_SRC = os.path.join(DRIVER_DIR, "test_dropin.f90")
+DRIVER_EXE = os.path.join(DRIVER_DIR, "_build", "test_dropin")
 
 
 def flang() -> str:
@@ -59,22 +61,30 @@ def flang() -> str:
     return "flang"
 
 
+def _flang(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("flang failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+
+
 def build_fortran(verbose: bool = False) -> str:
-    """iso_c_binding layer + SALMON-signature wrappers + their test driver, linked to libsalmon_hip.so."""
+    """iso_c_binding layer + SALMON-signature wrappers (objects and modules in fortran/_build),
+    and the wrappers' test driver linked to libsalmon_hip.so."""
     os.makedirs(FBUILD, exist_ok=True)
     objs = []
     for src in FORTRAN_SRCS:
         obj = os.path.join(FBUILD, src.replace(".f90", ".o"))
-        cmd = [flang(), "-O2", "-fPIC", "-module-dir", FBUILD, "-c", os.path.join(FDIR, src), "-o", obj]
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError("flang failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+        _flang([flang(), "-O2", "-fPIC", "-module-dir", FBUILD, "-c", os.path.join(FDIR, src), "-o", obj])
         objs.append(obj)
-    cmd = [flang(), "-o", FORTRAN_EXE, *objs, "-L", PKG, "-lsalmon_hip", "-Wl,-rpath,$ORIGIN/.."]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("flang link failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
-    return FORTRAN_EXE
+    if not os.path.exists(DRIVER_SRC):
+        return FBUILD
+    dbuild = os.path.dirname(DRIVER_EXE)
+    os.makedirs(dbuild, exist_ok=True)
+    dobj = os.path.join(dbuild, "test_dropin.o")
+    _flang([flang(), "-O2", "-fPIC", "-module-dir", dbuild, "-I", FBUILD, "-c", DRIVER_SRC, "-o", dobj])
+    rpath = os.path.relpath(PKG, dbuild)
+    _flang([flang(), "-o", DRIVER_EXE, *objs, dobj, "-L", PKG, "-lsalmon_hip", f"-Wl,-rpath,$ORIGIN/{rpath}"])
+    return DRIVER_EXE
 
 
 if __name__ == "__main__":

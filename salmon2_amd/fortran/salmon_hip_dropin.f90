@@ -8,10 +8,19 @@
 !   update_kvector_nonlocalpt_hip   <- update_kvector_nonlocalpt (src/common/hamiltonian.f90:847-860)
 !   calc_density_hip                <- calc_density              (src/common/density_matrix.f90:26-42)
 !   calc_current_hip                <- calc_current              (src/common/density_matrix.f90:240-300)
-! The orbital arguments are SALMON's host arrays; these wrappers copy them to
-! and from device buffers owned by this module at every call (PCIe-inclusive,
-! the simple drop-in).  A driver that keeps orbitals resident uses the
-! salmon_hip_mod interfaces directly (see the RT step in salmon2_amd/rt.py).
+!
+! Residency (the reference's OpenACC path: `!$acc enter data copyin(spsi_in,
+! spsi_out,tpsi)` in src/rt/initialization_rt.f90:234-240, then `present`
+! clauses): salmon_hip_enter_data(orb) gives an s_orbital a device mirror,
+! keyed by the address of its zwf array, and uploads it.  A wrapper whose
+! orbital arguments have mirrors works on them in place -- no host<->device
+! copy of the orbitals -- and the host arrays are then stale until
+! salmon_hip_update_host(orb) (`!$acc update host`); salmon_hip_update_device
+! re-uploads after a host write; salmon_hip_exit_data drops the mirror.
+! Orbital arguments without a mirror are copied in and out at the call (the
+! simple drop-in).  rho and curr are host outputs and are downloaded at every
+! call; V_local is uploaded by taylor_hip/hpsi_hip only when it differs from
+! the last upload.
 !
 ! Supported: periodic single-process runs (no r-space division, no halo:
 ! is_array = is), one im, orthogonal lattice, collinear spin, no DFT+U -- the
@@ -24,6 +33,8 @@ module salmon_hip_dropin
   implicit none
   private
   public :: salmon_hip_dropin_setup, salmon_hip_dropin_release
+  public :: salmon_hip_enter_data, salmon_hip_exit_data, salmon_hip_update_host, salmon_hip_update_device
+  public :: salmon_hip_dropin_sync
   public :: taylor_hip, hpsi_hip, update_kvector_nonlocalpt_hip, calc_density_hip, calc_current_hip
 
   type(c_ptr), save :: h = c_null_ptr
@@ -31,6 +42,13 @@ module salmon_hip_dropin
   type(c_ptr), save :: d_rho = c_null_ptr, d_cur = c_null_ptr
   integer(c_size_t), save :: orb_bytes = 0
   integer, save :: nx_s, ny_s, nz_s, nspin_s
+  ! device mirrors of s_orbital arrays: key = address of the host zwf
+  integer, parameter :: kmaxres = 16
+  integer(c_intptr_t), save :: res_key(kmaxres) = 0
+  type(c_ptr), save :: res_dev(kmaxres)
+  ! the last V_local uploaded (uploads are skipped while it is unchanged)
+  real(8), allocatable, save :: v_last(:,:,:,:)
+  logical, save :: have_v = .false.
 
 contains
 
@@ -66,36 +84,159 @@ contains
     end if
     orb_bytes = int(nx_s, c_size_t) * int(ly, c_size_t) * int(nz_s, c_size_t) * int(system%nspin, c_size_t) &
               * int(system%no, c_size_t) * int(system%nk, c_size_t) * 16_c_size_t
-    call salmon_hip_check(salmon_hip_malloc(d_a, orb_bytes), 'malloc')
-    call salmon_hip_check(salmon_hip_malloc(d_b, orb_bytes), 'malloc')
-    call salmon_hip_check(salmon_hip_malloc(d_w, orb_bytes), 'malloc')
     call salmon_hip_check(salmon_hip_malloc(d_rho, int(nx_s, c_size_t) * ny_s * nz_s * nspin_s * 8_c_size_t), &
         'malloc')
     call salmon_hip_check(salmon_hip_malloc(d_cur, int(3 * nspin_s, c_size_t) * 8_c_size_t), 'malloc')
+    allocate(v_last(nx_s, ny_s, nz_s, nspin_s))
+    have_v = .false.
   end subroutine salmon_hip_dropin_setup
 
   subroutine salmon_hip_dropin_release()
+    integer :: i
     if (.not. c_associated(h)) return
-    call salmon_hip_check(salmon_hip_free(d_a), 'free')
-    call salmon_hip_check(salmon_hip_free(d_b), 'free')
-    call salmon_hip_check(salmon_hip_free(d_w), 'free')
+    call salmon_hip_check(salmon_hip_synchronize(h), 'synchronize')
+    do i = 1, kmaxres
+      if (res_key(i) /= 0) call salmon_hip_check(salmon_hip_free(res_dev(i)), 'free')
+      res_key(i) = 0
+    end do
+    if (c_associated(d_a)) call salmon_hip_check(salmon_hip_free(d_a), 'free')
+    if (c_associated(d_b)) call salmon_hip_check(salmon_hip_free(d_b), 'free')
+    if (c_associated(d_w)) call salmon_hip_check(salmon_hip_free(d_w), 'free')
+    d_a = c_null_ptr; d_b = c_null_ptr; d_w = c_null_ptr
     call salmon_hip_check(salmon_hip_free(d_rho), 'free')
     call salmon_hip_check(salmon_hip_free(d_cur), 'free')
     call salmon_hip_check(salmon_hip_destroy(h), 'destroy')
+    if (allocated(v_last)) deallocate(v_last)
     h = c_null_ptr
   end subroutine salmon_hip_dropin_release
 
-  ! V_local(ispin)%f(is:ie) -> one (nx,ny,nz,nspin) block on the device
+  ! Wait for every enqueued device step (host timers, host reads of resident results).
+  subroutine salmon_hip_dropin_sync()
+    if (.not. c_associated(h)) stop 'salmon_hip: salmon_hip_dropin_setup not called'
+    call salmon_hip_check(salmon_hip_synchronize(h), 'synchronize')
+  end subroutine salmon_hip_dropin_sync
+
+  integer(c_intptr_t) function key_of(orb)
+    type(s_orbital), intent(in), target :: orb
+    key_of = transfer(c_loc(orb%zwf), 0_c_intptr_t)
+  end function key_of
+
+  ! Device mirror of orb, or c_null_ptr.
+  type(c_ptr) function mirror(orb)
+    type(s_orbital), intent(in), target :: orb
+    integer(c_intptr_t) :: k
+    integer :: i
+    mirror = c_null_ptr
+    if (.not. allocated(orb%zwf)) return
+    k = key_of(orb)
+    do i = 1, kmaxres
+      if (res_key(i) == k) then
+        mirror = res_dev(i)
+        return
+      end if
+    end do
+  end function mirror
+
+  subroutine check_shape(orb)
+    type(s_orbital), intent(in) :: orb
+    if (.not. allocated(orb%zwf)) stop 'salmon_hip: orbital array not allocated'
+    if (int(size(orb%zwf), c_size_t) * 16_c_size_t /= orb_bytes) stop 'salmon_hip: orbital array shape'
+  end subroutine check_shape
+
+  ! `!$acc enter data copyin(orb)`: a device mirror of orb%zwf, uploaded.
+  subroutine salmon_hip_enter_data(orb)
+    type(s_orbital), intent(in), target :: orb
+    integer :: i
+    if (.not. c_associated(h)) stop 'salmon_hip: salmon_hip_dropin_setup before enter_data'
+    call check_shape(orb)
+    if (c_associated(mirror(orb))) return
+    do i = 1, kmaxres
+      if (res_key(i) == 0) then
+        call salmon_hip_check(salmon_hip_malloc(res_dev(i), orb_bytes), 'malloc (enter_data)')
+        call salmon_hip_check(salmon_hip_memcpy_htod(h, res_dev(i), orb%zwf, orb_bytes), 'upload (enter_data)')
+        res_key(i) = key_of(orb)
+        return
+      end if
+    end do
+    stop 'salmon_hip: too many resident orbital arrays'
+  end subroutine salmon_hip_enter_data
+
+  ! `!$acc exit data`: drop the mirror (copyout = .true.: download it first).
+  subroutine salmon_hip_exit_data(orb, copyout)
+    type(s_orbital), intent(inout), target :: orb
+    logical, intent(in), optional :: copyout
+    integer(c_intptr_t) :: k
+    integer :: i
+    if (.not. c_associated(h)) return
+    k = key_of(orb)
+    do i = 1, kmaxres
+      if (res_key(i) == k) then
+        if (present(copyout)) then
+          if (copyout) call salmon_hip_check(salmon_hip_memcpy_dtoh(h, orb%zwf, res_dev(i), orb_bytes), 'download')
+        end if
+        call salmon_hip_check(salmon_hip_synchronize(h), 'synchronize')
+        call salmon_hip_check(salmon_hip_free(res_dev(i)), 'free (exit_data)')
+        res_key(i) = 0
+        return
+      end if
+    end do
+  end subroutine salmon_hip_exit_data
+
+  ! `!$acc update host(orb)`
+  subroutine salmon_hip_update_host(orb)
+    type(s_orbital), intent(inout), target :: orb
+    type(c_ptr) :: d
+    d = mirror(orb)
+    if (.not. c_associated(d)) stop 'salmon_hip: update_host of an array without a device mirror'
+    call salmon_hip_check(salmon_hip_memcpy_dtoh(h, orb%zwf, d, orb_bytes), 'update_host')
+  end subroutine salmon_hip_update_host
+
+  ! `!$acc update device(orb)`
+  subroutine salmon_hip_update_device(orb)
+    type(s_orbital), intent(in), target :: orb
+    type(c_ptr) :: d
+    d = mirror(orb)
+    if (.not. c_associated(d)) stop 'salmon_hip: update_device of an array without a device mirror'
+    call salmon_hip_check(salmon_hip_memcpy_htod(h, d, orb%zwf, orb_bytes), 'update_device')
+  end subroutine salmon_hip_update_device
+
+  ! Device address of orb: its mirror, else `scratch` (allocated on first use) with orb
+  ! uploaded when `upload`.
+  type(c_ptr) function on_device(orb, scratch, upload) result(d)
+    type(s_orbital), intent(in), target :: orb
+    type(c_ptr), intent(inout) :: scratch
+    logical, intent(in) :: upload
+    d = mirror(orb)
+    if (c_associated(d)) return
+    call check_shape(orb)
+    if (.not. c_associated(scratch)) call salmon_hip_check(salmon_hip_malloc(scratch, orb_bytes), 'malloc')
+    d = scratch
+    if (upload) call salmon_hip_check(salmon_hip_memcpy_htod(h, d, orb%zwf, orb_bytes), 'upload')
+  end function on_device
+
+  ! Download orb from d unless d is orb's own mirror.
+  subroutine to_host(orb, d)
+    type(s_orbital), intent(inout), target :: orb
+    type(c_ptr), intent(in) :: d
+    if (c_associated(mirror(orb), d)) return
+    call salmon_hip_check(salmon_hip_memcpy_dtoh(h, orb%zwf, d, orb_bytes), 'download')
+  end subroutine to_host
+
+  ! V_local(ispin)%f(is:ie) -> one (nx,ny,nz,nspin) block on the device, when it changed
   subroutine upload_vlocal(V_local)
     type(s_scalar), intent(in) :: V_local(:)
-    real(8), allocatable :: v(:,:,:,:)
     integer :: is
-    allocate(v(nx_s, ny_s, nz_s, nspin_s))
+    logical :: same
+    same = have_v
     do is = 1, nspin_s
-      v(:,:,:,is) = V_local(is)%f
+      if (same) same = all(v_last(:,:,:,is) == V_local(is)%f)
     end do
-    call salmon_hip_check(salmon_hip_set_vlocal(h, v), 'set_vlocal')
-    deallocate(v)
+    if (same) return
+    do is = 1, nspin_s
+      v_last(:,:,:,is) = V_local(is)%f
+    end do
+    call salmon_hip_check(salmon_hip_set_vlocal(h, v_last), 'set_vlocal')
+    have_v = .true.
   end subroutine upload_vlocal
 
   subroutine update_kvector_nonlocalpt_hip(ik_s, ik_e, system, ppg)
@@ -113,19 +254,22 @@ contains
     type(s_parallel_info), intent(in) :: info
     type(s_stencil), intent(in) :: stencil
     type(s_sendrecv_grid), intent(inout) :: srg
-    type(s_orbital), intent(inout) :: tspsi_in
-    type(s_orbital), intent(inout) :: tspsi_out
-    type(s_orbital), intent(inout) :: sshtpsi
+    type(s_orbital), intent(inout), target :: tspsi_in
+    type(s_orbital), intent(inout), target :: tspsi_out
+    type(s_orbital), intent(inout), target :: sshtpsi
     type(s_pp_grid), intent(in) :: ppg
     type(s_scalar), intent(in) :: V_local(system%nspin)
     type(s_rt), intent(in) :: rt
+    type(c_ptr) :: din, dout, dw
     call salmon_hip_dropin_setup(mg, system, info, stencil, ppg)
     call upload_vlocal(V_local)
     call salmon_hip_check(salmon_hip_set_vec_ac(h, system%vec_Ac), 'set_vec_ac')
-    call salmon_hip_check(salmon_hip_memcpy_htod(h, d_a, tspsi_in%zwf, orb_bytes), 'upload tspsi_in')
-    call salmon_hip_check(salmon_hip_taylor(h, rt%zc, int(size(rt%zc), c_int), d_a, d_b, d_w), 'taylor')
-    call salmon_hip_check(salmon_hip_memcpy_dtoh(h, tspsi_out%zwf, d_b, orb_bytes), 'download tspsi_out')
-    call salmon_hip_check(salmon_hip_memcpy_dtoh(h, sshtpsi%zwf, d_w, orb_bytes), 'download sshtpsi')
+    din = on_device(tspsi_in, d_a, .true.)
+    dout = on_device(tspsi_out, d_b, .false.)
+    dw = on_device(sshtpsi, d_w, .false.)
+    call salmon_hip_check(salmon_hip_taylor(h, rt%zc, int(size(rt%zc), c_int), din, dout, dw), 'taylor')
+    call to_host(tspsi_out, dout)
+    call to_host(sshtpsi, dw)
   end subroutine taylor_hip
 
   subroutine hpsi_hip(tpsi, htpsi, info, mg, V_local, system, stencil, srg, ppg, ttpsi)
@@ -136,17 +280,21 @@ contains
     type(s_stencil), intent(in) :: stencil
     type(s_sendrecv_grid), intent(inout) :: srg
     type(s_pp_grid), intent(in) :: ppg
-    type(s_orbital) :: tpsi, htpsi
+    type(s_orbital), target :: tpsi, htpsi
     type(s_orbital), optional :: ttpsi
+    type(c_ptr) :: din, dout
     integer :: is, io, ik, im
     call salmon_hip_dropin_setup(mg, system, info, stencil, ppg)
     call upload_vlocal(V_local)
     call salmon_hip_check(salmon_hip_set_vec_ac(h, system%vec_Ac), 'set_vec_ac')
-    call salmon_hip_check(salmon_hip_memcpy_htod(h, d_a, tpsi%zwf, orb_bytes), 'upload tpsi')
-    call salmon_hip_check(salmon_hip_hpsi(h, d_a, d_b), 'hpsi')
-    call salmon_hip_check(salmon_hip_memcpy_dtoh(h, htpsi%zwf, d_b, orb_bytes), 'download htpsi')
+    din = on_device(tpsi, d_a, .true.)
+    dout = on_device(htpsi, d_b, .false.)
+    call salmon_hip_check(salmon_hip_hpsi(h, din, dout), 'hpsi')
+    call to_host(htpsi, dout)
     if (present(ttpsi)) then
-      ! ttpsi = htpsi - V_local tpsi (hamiltonian.f90:396-413)
+      ! ttpsi = htpsi - V_local tpsi (hamiltonian.f90:396-413), on the host arrays
+      if (c_associated(mirror(tpsi)) .or. c_associated(mirror(htpsi))) &
+        stop 'salmon_hip: hpsi with ttpsi needs host orbitals (no device mirror)'
       do im = lbound(tpsi%zwf, 7), ubound(tpsi%zwf, 7)
       do ik = lbound(tpsi%zwf, 6), ubound(tpsi%zwf, 6)
       do io = lbound(tpsi%zwf, 5), ubound(tpsi%zwf, 5)
@@ -165,13 +313,14 @@ contains
     type(s_dft_system), intent(in) :: system
     type(s_parallel_info), intent(in) :: info
     type(s_rgrid), intent(in) :: mg
-    type(s_orbital), intent(in) :: psi
+    type(s_orbital), intent(in), target :: psi
     type(s_scalar) :: rho(system%nspin, info%im_s:info%im_e)
     real(8), allocatable :: r(:,:,:,:)
+    type(c_ptr) :: d
     integer :: is
     if (.not. c_associated(h)) stop 'salmon_hip: salmon_hip_dropin_setup not called'
-    call salmon_hip_check(salmon_hip_memcpy_htod(h, d_a, psi%zwf, orb_bytes), 'upload psi')
-    call salmon_hip_check(salmon_hip_density(h, d_a, d_rho), 'density')
+    d = on_device(psi, d_a, .true.)
+    call salmon_hip_check(salmon_hip_density(h, d, d_rho), 'density')
     allocate(r(nx_s, ny_s, nz_s, nspin_s))
     call salmon_hip_check(salmon_hip_memcpy_dtoh(h, r, d_rho, int(size(r), c_size_t) * 8_c_size_t), 'download rho')
     do is = 1, system%nspin
@@ -186,13 +335,14 @@ contains
     type(s_stencil), intent(in) :: stencil
     type(s_parallel_info), intent(in) :: info
     type(s_sendrecv_grid) :: srg
-    type(s_orbital) :: psi
+    type(s_orbital), target :: psi
     type(s_pp_grid), intent(in) :: ppg
     real(8) :: curr(3, system%nspin, info%im_s:info%im_e)
     real(8) :: c(3, system%nspin)
+    type(c_ptr) :: d
     if (.not. c_associated(h)) stop 'salmon_hip: salmon_hip_dropin_setup not called'
-    call salmon_hip_check(salmon_hip_memcpy_htod(h, d_a, psi%zwf, orb_bytes), 'upload psi')
-    call salmon_hip_check(salmon_hip_current(h, system%vec_Ac, d_a, d_cur), 'current')
+    d = on_device(psi, d_a, .true.)
+    call salmon_hip_check(salmon_hip_current(h, system%vec_Ac, d, d_cur), 'current')
     call salmon_hip_check(salmon_hip_memcpy_dtoh(h, c, d_cur, int(size(c), c_size_t) * 8_c_size_t), 'download curr')
     curr(:, :, info%im_s) = c
   end subroutine calc_current_hip
