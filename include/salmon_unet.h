@@ -6,7 +6,7 @@
  * The network is a 3-level U-Net on the periodic real-space grid: 3x3x3 periodic
  * convolutions with ReLU, 16/32/64 channels, 2x2x2 average-pool down, nearest
  * upsampling, skip concatenation, a 1x1x1 head.  Weights and activations are
- * BF16, products accumulate in FP32 on MFMA (v_mfma_f32_16x16x16_bf16), master
+ * BF16, products accumulate in FP32 on MFMA (gfx950 v_mfma_f32_16x16x32_bf16), master
  * weights and the Adam state are FP32.  Grid extents must be multiples of 4.
  *
  * Tensors handed over are DEVICE arrays, channels-last: in[p][c] (c = 0..3) and

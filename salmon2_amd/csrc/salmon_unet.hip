@@ -16,11 +16,12 @@
 // conv = 3x3x3 periodic, weights [27][co][ci]; `up` = nearest upsampling, read on the fly.
 //
 // Kernels:
-//   k_conv<NT>     implicit GEMM on v_mfma_f32_16x16x16_bf16: M = 16 points per wave,
-//                  N = NT x 16 output channels, K = 16 input channels of one tap; also the
-//                  data gradient (flipped, transposed weights, FP32 output)
+//   k_conv<NT>     implicit GEMM on gfx950's double-K v_mfma_f32_16x16x32_bf16: M = 16
+//                  points per wave, N = NT x 16 output channels, K = 32 of the flattened
+//                  (tap, input channel) index (two taps per instruction at 16 channels);
+//                  also the data gradient (flipped, transposed weights, FP32 output)
 //   k_wgrad        weight gradient per (tap, co tile, ci tile) as MFMA over point chunks
-//                  (M = co, N = ci, K = points), FP32 partials reduced in fixed order
+//                  (M = co, N = ci, K = 32 points), FP32 partials reduced in fixed order
 //   small kernels  pooling, gradient routing (mask, down-sum, pool-back), head, MSE, Adam
 #include <hip/hip_runtime.h>
 
@@ -40,8 +41,15 @@ void salmon_hip_internal_error(const char *msg);  // salmon_hip.hip: sets salmon
 namespace {
 
 typedef unsigned short bf16_t;
-typedef short s4v __attribute__((ext_vector_type(4)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+typedef __bf16 b8v __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+
+// v_mfma_f32_16x16x32_bf16 (gfx950): lane L holds A[L%16][8(L/16) + e] and
+// B[8(L/16) + e][L%16], e = 0..7; D[4(L/16) + r][L%16] lands in register r.
+__device__ __forceinline__ f4v mfma32(s8v a, s8v b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8v, a), __builtin_bit_cast(b8v, b), c, 0, 0, 0);
+}
 
 int ufail(int code, const char *fmt, ...);
 
@@ -65,11 +73,12 @@ __device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((unsign
 
 // ---------------------------------------------------------------------------
 // 3x3x3 periodic convolution as implicit GEMM.  Wave w of block b owns points
-// p0 = 16 (4b + w) .. p0 + 15.  MFMA operand layout (16x16x16 bf16): lane L holds
-// A[L%16][4(L/16) + e] and B[4(L/16) + e][L%16], e = 0..3; the result lands as
-// D[4(L/16) + r][L%16] in register r.  A row = point, k = input channel, B col =
-// output channel.  The input is [up(src1), src2] (channel concatenation, src1 at
-// half resolution when up1), every 16-channel chunk lies in one source.
+// p0 = 16 (4b + w) .. p0 + 15; A row = point, B col = output channel, K = the
+// flattened index f = tap * ci + input channel.  One MFMA takes 32 consecutive f:
+// lane group q = L/16 supplies f0 + 8q .. f0 + 8q + 7, which lie in one tap and one
+// 16-channel chunk (ci is a multiple of 16), i.e. one 16-byte load of activations at
+// the tap's neighbour and one of weights.  The input is [up(src1), src2] (channel
+// concatenation, src1 at half resolution when up1).
 // ---------------------------------------------------------------------------
 struct ConvArgs {
   const bf16_t *src1, *src2;
@@ -94,29 +103,37 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int x = pp % nx, y = (pp / nx) % ny, z = pp / (nx * ny);
   const int ci = a.c1 + a.c2;
+  const int K = 27 * ci;
   f4v acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
-  for (int tap = 0; tap < 27; ++tap) {
-    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-    const int xn = (x + dx + nx) % nx, yn = (y + dy + ny) % ny, zn = (z + dz + nz) % nz;
-    const size_t pn = (size_t)xn + (size_t)nx * ((size_t)yn + (size_t)ny * zn);
-    const size_t ph = a.up1 ? (size_t)(xn >> 1) + (size_t)(nx >> 1) * ((size_t)(yn >> 1) + (size_t)(ny >> 1) * (zn >> 1))
-                            : pn;
-    const bf16_t *wt = a.w + (size_t)tap * a.co * ci;
-    for (int c0 = 0; c0 < ci; c0 += 16) {
-      const int c = c0 + 4 * kq;
-      s4v av = s4v{0, 0, 0, 0};
+  for (int f0 = 0; f0 < K; f0 += 32) {
+    const int f = f0 + 8 * kq;
+    s8v av = s8v{0, 0, 0, 0, 0, 0, 0, 0};
+    s8v bv[NT];
+    if (f < K) {
+      const int tap = f / ci, c = f - tap * ci;
+      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+      const int xn = (x + dx + nx) % nx, yn = (y + dy + ny) % ny, zn = (z + dz + nz) % nz;
       if (pv) {
-        if (c0 < a.c1) av = *(const s4v *)(a.src1 + ph * a.c1 + c);
-        else av = *(const s4v *)(a.src2 + pn * a.c2 + (c - a.c1));
+        if (c < a.c1) {
+          const size_t ph = a.up1 ? (size_t)(xn >> 1) + (size_t)(nx >> 1) * ((size_t)(yn >> 1) + (size_t)(ny >> 1) * (zn >> 1))
+                                  : (size_t)xn + (size_t)nx * ((size_t)yn + (size_t)ny * zn);
+          av = *(const s8v *)(a.src1 + ph * a.c1 + c);
+        } else {
+          const size_t pn = (size_t)xn + (size_t)nx * ((size_t)yn + (size_t)ny * zn);
+          av = *(const s8v *)(a.src2 + pn * a.c2 + (c - a.c1));
+        }
       }
+      const bf16_t *wt = a.w + (size_t)tap * a.co * ci + c;
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const s4v bv = *(const s4v *)(wt + (size_t)(t * 16 + row) * ci + c);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
-      }
+      for (int t = 0; t < NT; ++t) bv[t] = *(const s8v *)(wt + (size_t)(t * 16 + row) * ci);
+    } else {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) bv[t] = s8v{0, 0, 0, 0, 0, 0, 0, 0};
     }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = mfma32(av, bv[t], acc[t]);
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t)
@@ -139,7 +156,8 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 // ---------------------------------------------------------------------------
 // Weight gradient dW[tap][co][ci] = sum_p dy[p][co] x[p + d(tap)][ci] over one chunk of
 // points per grid row y; wave = (tap, co tile, ci tile); A = dy^T (M = co, K = points),
-// B = shifted input (K = points, N = ci).  part[(chunk*27 + tap)*co + co_][ci_].
+// B = shifted input (K = points, N = ci), 32 points per MFMA (lane group q takes points
+// pb + 8q .. pb + 8q + 7).  part[(chunk*27 + tap)*co + co_][ci_].
 // ---------------------------------------------------------------------------
 struct WgArgs {
   const bf16_t *dy;
@@ -164,11 +182,11 @@ __global__ __launch_bounds__(256) void k_wgrad(WgArgs a) {
   const int cc = cit * 16 + row;
   const bool first = cit * 16 < a.c1;
   f4v acc = f4v{0.f, 0.f, 0.f, 0.f};
-  for (int pb = pbeg; pb < pend; pb += 16) {
-    s4v av, bv;
+  for (int pb = pbeg; pb < pend; pb += 32) {
+    s8v av, bv;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int pk = pb + 4 * kq + e;
+    for (int e = 0; e < 8; ++e) {
+      const int pk = pb + 8 * kq + e;
       short va = 0, vb = 0;
       if (pk < pend) {
         va = (short)a.dy[(size_t)pk * a.co + cot * 16 + row];
@@ -186,7 +204,7 @@ __global__ __launch_bounds__(256) void k_wgrad(WgArgs a) {
       av[e] = va;
       bv[e] = vb;
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    acc = mfma32(av, bv, acc);
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r)
