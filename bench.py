@@ -245,6 +245,23 @@ def profiled_run(s, psi, form, steps, warmup, predictor):
     return kern
 
 
+def copy_ceiling(nbytes=1 << 30, reps=10):
+    """GB/s (bytes read + written per second) of the library's streaming copy of nbytes."""
+    import torch
+
+    from salmon2_amd import system
+    from salmon2_amd.hip import Handle
+
+    h = Handle(system.config("tiny"), stream=torch.cuda.current_stream())
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    ms = h.stream_copy(dst, src, reps=reps)
+    h.close()
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (ms / 1e3) / 1e9
+
+
 def run_batch(steps, warmup):
     """Batched steps of the seeded uneven systems (eager: the graph capture of the batch is
     switched off) and the same systems advanced one by one."""
@@ -401,6 +418,7 @@ def main():
     avg_ms = (k1["avg_ms"] * k1["launches"] + km["avg_ms"] * km["launches"]) / max(nl, 1)
     achieved = alg_bytes / (avg_ms / 1e3) / 1e9
     kname = "k_stencilz<d2,double,d2>" if s.nx <= 32 else "k_stencil<d2,double,d2,2,-1,256>"
+    ceil = copy_ceiling()
     out = {
         "metric": METRIC,
         "value": form_out[head]["steps_per_s"],
@@ -440,6 +458,10 @@ def main():
             "avg_launch_ms": avg_ms,
             "launches": nl,
             "timing": f"per-launch events over a separate {args.prof_steps}-step pass of {BASE_FORM}",
+            "copy_ceiling_GBps": ceil,
+            "frac_of_copy_ceiling": achieved / ceil,
+            "copy_ceiling_note": "streaming copy kernel of the library (dwordx4 non-temporal loads and stores), "
+                                 "1 GiB, bytes read + written per second",
         },
         "kernels_ms": {k: round(v["avg_ms"], 5) for k, v in kern.items()},
     }

@@ -147,6 +147,15 @@ int salmon_hip_taylor_reuse(salmon_hip_t h, const double *zc, int n_hamil, void 
 int salmon_hip_set_skip(salmon_hip_t h, const int *flag_dev);
 
 /* dst_dev[i] = src_dev[i] for i < n when *flag_dev != 0 (device arrays; on the handle's stream). */
+
+/*
+ * Streaming copy of `bytes` (a multiple of 16) from src_dev to dst_dev with dwordx4
+ * non-temporal loads and stores, one warm-up launch then `reps` timed launches on the
+ * handle's stream; *avg_ms = device time per launch (no counterpart in the reference: the
+ * HBM ceiling the memory-bound kernels are measured against).  Synchronises.
+ */
+int salmon_hip_stream_copy(salmon_hip_t h, void *dst_dev, const void *src_dev, size_t bytes, int reps,
+                           double *avg_ms);
 int salmon_hip_copy_if(salmon_hip_t h, const int *flag_dev, double *dst_dev, const double *src_dev, size_t n);
 
 /*

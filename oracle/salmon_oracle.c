@@ -92,36 +92,60 @@ static inline void axis_vw(const double *l, const double *nb, const cpx p[4], co
   *w = ww;
 }
 
-static inline void gather_axis(const double *t, int ix, int iy, int iz, int nx, int ny, int nz,
-                               int ly, int axis, cpx p[4], cpx m[4]) {
+/* idx/idy/idz(is-4:ie+4) of initialization.f90:574-587, 0-based: tab[j + 4] = mod(j, n) for
+   j = -4 .. n+3 (the reference reads its neighbours through these tables) */
+static int *wrap_table(int n) {
+  int *t = (int *)malloc(sizeof(int) * (size_t)(n + 8));
+  if (t)
+    for (int j = -4; j < n + 4; ++j) t[j + 4] = wrap(j, n);
+  return t;
+}
+
+/* The neighbours of point (ix, iy, iz) along one axis through the tables: p[n-1] = psi(r + n e),
+   m[n-1] = psi(r - n e); rowy[k] = offset of row (iy + k - 4, iz), rowz[k] = of (iy, iz + k - 4). */
+static inline void gather_x(const double *t, const int *tx, int ix, size_t row, cpx p[4], cpx m[4]) {
   for (int n = 1; n <= 4; ++n) {
-    int xp = ix, yp = iy, zp = iz, xm = ix, ym = iy, zm = iz;
-    if (axis == 0) { xp = wrap(ix + n, nx); xm = wrap(ix - n, nx); }
-    if (axis == 1) { yp = wrap(iy + n, ny); ym = wrap(iy - n, ny); }
-    if (axis == 2) { zp = wrap(iz + n, nz); zm = wrap(iz - n, nz); }
-    p[n - 1] = ld(t, (size_t)xp + (size_t)nx * ((size_t)yp + (size_t)ly * zp));
-    m[n - 1] = ld(t, (size_t)xm + (size_t)nx * ((size_t)ym + (size_t)ly * zm));
+    p[n - 1] = ld(t, row + (size_t)tx[ix + n + 4]);
+    m[n - 1] = ld(t, row + (size_t)tx[ix - n + 4]);
+  }
+}
+static inline void gather_r(const double *t, int ix, const size_t rows[9], cpx p[4], cpx m[4]) {
+  for (int n = 1; n <= 4; ++n) {
+    p[n - 1] = ld(t, rows[4 + n] + (size_t)ix);
+    m[n - 1] = ld(t, rows[4 - n] + (size_t)ix);
+  }
+}
+
+static void row_offsets(int iy, int iz, int nx, int ly, const int *ty, const int *tz, size_t rowy[9],
+                        size_t rowz[9]) {
+  for (int k = 0; k < 9; ++k) {
+    rowy[k] = (size_t)nx * ((size_t)ty[iy + k] + (size_t)ly * iz);
+    rowz[k] = (size_t)nx * ((size_t)iy + (size_t)ly * tz[iz + k]);
   }
 }
 
 /* zstencil_core/typical/seq.f90:17-127 (one orbital, periodic, no halo) */
 void orc_zstencil_seq(int nx, int ny, int nz, int ly, const double *tpsi, double *htpsi,
                       const double *V, double lap0, const double lapt[12], const double nabt[12]) {
+  int *tx = wrap_table(nx), *ty = wrap_table(ny), *tz = wrap_table(nz);
   for (int iz = 0; iz < nz; ++iz)
-    for (int iy = 0; iy < ny; ++iy)
+    for (int iy = 0; iy < ny; ++iy) {
+      size_t rowy[9], rowz[9];
+      row_offsets(iy, iz, nx, ly, ty, tz, rowy, rowz);
+      const size_t row = rowy[4];
       for (int ix = 0; ix < nx; ++ix) {
         cpx p[4], m[4], v, w, v2, w2;
-        gather_axis(tpsi, ix, iy, iz, nx, ny, nz, ly, 0, p, m);
+        gather_x(tpsi, tx, ix, row, p, m);
         axis_vw(lapt + 0, nabt + 0, p, m, &v, &w);
-        gather_axis(tpsi, ix, iy, iz, nx, ny, nz, ly, 1, p, m);
+        gather_r(tpsi, ix, rowy, p, m);
         axis_vw(lapt + 4, nabt + 4, p, m, &v2, &w2);
         v = radd(v2, v);
         w = radd(w2, w);
-        gather_axis(tpsi, ix, iy, iz, nx, ny, nz, ly, 2, p, m);
+        gather_r(tpsi, ix, rowz, p, m);
         axis_vw(lapt + 8, nabt + 8, p, m, &v2, &w2);
         v = radd(v2, v);
         w = radd(w2, w);
-        size_t i = (size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ly * iz);
+        size_t i = row + (size_t)ix;
         cpx t = ld(tpsi, i);
         double vl = V[(size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * iz)];
         /* htpsi = V*t + lap0*t - 0.5d0*v - zI*w ;  zI*w = (-w.im, w.re) */
@@ -130,22 +154,30 @@ void orc_zstencil_seq(int nx, int ny, int nz, int ly, const double *tpsi, double
         htpsi[C2(i)] = hre;
         htpsi[C2(i) + 1] = him;
       }
+    }
+  free(tx);
+  free(ty);
+  free(tz);
 }
 
 /* zstencil_core/typical/omp.f90:22-164: x+y first, z added in a second pass */
 void orc_zstencil_omp(int nx, int ny, int nz, int ly, const double *tpsi, double *htpsi,
                       const double *V, double lap0, const double lapt[12], const double nabt[12]) {
+  int *tx = wrap_table(nx), *ty = wrap_table(ny), *tz = wrap_table(nz);
   for (int iz = 0; iz < nz; ++iz)
     for (int iy = 0; iy < ny; ++iy) {
+      size_t rowy[9], rowz[9];
+      row_offsets(iy, iz, nx, ly, ty, tz, rowy, rowz);
+      const size_t row = rowy[4];
       for (int ix = 0; ix < nx; ++ix) {
         cpx p[4], m[4], v, w, v2, w2;
-        gather_axis(tpsi, ix, iy, iz, nx, ny, nz, ly, 0, p, m);
+        gather_x(tpsi, tx, ix, row, p, m);
         axis_vw(lapt + 0, nabt + 0, p, m, &v, &w);
-        gather_axis(tpsi, ix, iy, iz, nx, ny, nz, ly, 1, p, m);
+        gather_r(tpsi, ix, rowy, p, m);
         axis_vw(lapt + 4, nabt + 4, p, m, &v2, &w2);
         v = radd(v2, v);
         w = radd(w2, w);
-        size_t i = (size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ly * iz);
+        size_t i = row + (size_t)ix;
         cpx t = ld(tpsi, i);
         double vl = V[(size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * iz)];
         htpsi[C2(i)] = ((vl * t.re + lap0 * t.re) - 0.5 * v.re) - (-w.im);
@@ -153,14 +185,17 @@ void orc_zstencil_omp(int nx, int ny, int nz, int ly, const double *tpsi, double
       }
       for (int ix = 0; ix < nx; ++ix) {
         cpx p[4], m[4], v, w;
-        gather_axis(tpsi, ix, iy, iz, nx, ny, nz, ly, 2, p, m);
+        gather_r(tpsi, ix, rowz, p, m);
         axis_vw(lapt + 8, nabt + 8, p, m, &v, &w);
-        size_t i = (size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ly * iz);
+        size_t i = row + (size_t)ix;
         /* htpsi = htpsi - 0.5d0*v - zI*w */
         htpsi[C2(i)] = (htpsi[C2(i)] - 0.5 * v.re) - (-w.im);
         htpsi[C2(i) + 1] = (htpsi[C2(i) + 1] - 0.5 * v.im) - w.re;
       }
     }
+  free(tx);
+  free(ty);
+  free(tz);
 }
 
 static size_t orb_size(const orc_system *s) { return (size_t)s->nx * s->ly * s->nz; }

@@ -1966,6 +1966,22 @@ __global__ __launch_bounds__(256) void k_update_vlocal(VlocArgs a) {
   if (a.vout) a.vout[i] = v;
 }
 
+// Streaming copy of n 16-byte elements (HBM ceiling of the box): dwordx4 loads and stores,
+// non-temporal, four elements per thread in flight.
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_stream_copy(u4v *__restrict__ dst, const u4v *__restrict__ src, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    u4v v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = __builtin_nontemporal_load(src + i + k * stride);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(v[k], dst + i + k * stride);
+  }
+  for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
 // dst = src where the device flag is set (the accepted branch of a device-side gate)
 __global__ __launch_bounds__(256) void k_copy_if(const int *flag, double *dst, const double *src, size_t n) {
   if (!*flag) return;
@@ -2312,6 +2328,33 @@ int salmon_hip_set_vlocal_device(salmon_hip_t c, const double *vlocal_dev) {
   HIPCHK(hipMemcpyAsync(c->d_V, vlocal_dev, sizeof(double) * c->ngrid * c->nspin, hipMemcpyDeviceToDevice,
                         c->stream));
   c->have_V = true;
+  return 0;
+}
+
+int salmon_hip_stream_copy(salmon_hip_t c, void *dst_dev, const void *src_dev, size_t bytes, int reps,
+                           double *avg_ms) {
+  if (int rc = check(c)) return rc;
+  if (!dst_dev || !src_dev || bytes % 16 != 0 || bytes == 0 || reps < 1)
+    return fail(SALMON_HIP_EINVAL, "stream_copy: null buffer, %zu bytes (a positive multiple of 16) or reps %d",
+                bytes, reps);
+  const size_t n = bytes / 16;
+  const unsigned grid = (unsigned)std::min<size_t>((n + 4 * 256 - 1) / (4 * 256), 1u << 20);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (u4v *)dst_dev, (const u4v *)src_dev, n);
+  int rc = 0;
+  if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(SALMON_HIP_EHIP, "event");
+  for (int r = 0; r < reps && !rc; ++r)
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (u4v *)dst_dev, (const u4v *)src_dev, n);
+  float ms = 0.f;
+  if (!rc && (hipGetLastError() != hipSuccess || hipEventRecord(e1, c->stream) != hipSuccess ||
+              hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = fail(SALMON_HIP_EHIP, "stream_copy timing");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (avg_ms) *avg_ms = (double)ms / reps;
   return 0;
 }
 

@@ -32,6 +32,7 @@ EXPORTS = (
     "salmon_hip_taylor_reuse",
     "salmon_hip_set_skip",
     "salmon_hip_copy_if",
+    "salmon_hip_stream_copy",
     "salmon_hip_set_precision",
     "salmon_hip_autotune",
     "salmon_hip_set_launch_shape",
@@ -100,6 +101,7 @@ def load():
     L.salmon_hip_taylor_reuse.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_set_skip.argtypes = [C.c_void_p, C.c_void_p]
     L.salmon_hip_copy_if.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    L.salmon_hip_stream_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, _dp]
     L.salmon_hip_set_precision.argtypes = [C.c_void_p, C.c_int]
     L.salmon_hip_autotune.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
     L.salmon_hip_set_launch_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -240,6 +242,15 @@ class Handle:
         assert src.numel() == n and dst.dtype == src.dtype
         _chk(load().salmon_hip_copy_if(self.h, C.c_void_p(_ptr(flag)), C.c_void_p(_ptr(dst)), C.c_void_p(_ptr(src)),
                                        n), "copy_if")
+
+    def stream_copy(self, dst, src, reps=10):
+        """Device ms per launch of the library's streaming copy of src into dst (torch tensors)."""
+        nb = src.numel() * src.element_size()
+        assert dst.numel() * dst.element_size() >= nb
+        ms = C.c_double()
+        _chk(load().salmon_hip_stream_copy(self.h, C.c_void_p(_ptr(dst)), C.c_void_p(_ptr(src)), nb, int(reps),
+                                           C.byref(ms)), "stream_copy")
+        return ms.value
 
     PREC = {"fp64": 0, "mixed": 1}
 
