@@ -178,11 +178,22 @@ class Oracle:
         lib().orc_dipole(self.ref, _p(r), _p(out))
         return out
 
-    def rt_run(self, nt=None, psi=None):
+    def _ac(self, nt, ac):
+        """The field table rows 0 .. nt+1: the given one (e.g. the rows of a continued run), else
+        the system's pulse from t = 0."""
+        s = self.s
+        if ac is not None:
+            ac = np.ascontiguousarray(ac, dtype=np.float64)
+            if ac.shape != (nt + 2, 3):
+                raise ValueError(f"field table of shape {ac.shape}, expected {(nt + 2, 3)}")
+            return ac
+        return np.ascontiguousarray(s.pulse.table(s.dt, nt) if s.pulse else np.zeros((nt + 2, 3)))
+
+    def rt_run(self, nt=None, psi=None, ac=None):
         """Plain RT loop (fixed V_local, middlepoint, 'tr'); returns final psi, J(t), dipole(t), rNe(t)."""
         s = self.s
         nt = s.nt if nt is None else nt
-        ac = np.ascontiguousarray(s.pulse.table(s.dt, nt) if s.pulse else np.zeros((nt + 2, 3)))
+        ac = self._ac(nt, ac)
         p = np.array(s.psi if psi is None else psi, dtype=np.complex128, copy=True)
         cur = np.zeros((nt, 3))
         dip = np.zeros((nt, 3))
@@ -194,12 +205,12 @@ class Oracle:
         return p, cur, dip, rne
 
 
-    def rt_run_scf(self, pc: bool, nt=None, psi=None):
+    def rt_run_scf(self, pc: bool, nt=None, psi=None, ac=None):
         """RT loop with V_local = Vpsl + Vh[rho] + Vxc[rho] rebuilt every step (system.vlocal is Vpsl),
         optionally with the predictor-corrector; returns final psi, J(t), dipole(t), rNe(t), final V_local."""
         s = self.s
         nt = s.nt if nt is None else nt
-        ac = np.ascontiguousarray(s.pulse.table(s.dt, nt) if s.pulse else np.zeros((nt + 2, 3)))
+        ac = self._ac(nt, ac)
         p = np.array(s.psi if psi is None else psi, dtype=np.complex128, copy=True)
         vpsl = np.ascontiguousarray(s.vlocal[0], dtype=np.float64)
         vloc = np.zeros_like(vpsl)

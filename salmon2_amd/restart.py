@@ -13,6 +13,9 @@ A restart directory (``data_for_restart/``, ``data_for_restart_rt/`` or a
                   (write_wavefunction 'single' :607-631 and the MPI subarray view of
                   distributed_rw_wavefunction :2044-2090 write the same layout)
   Vh_stock.bin    two records Vh_stock1/2(nx, ny, nz) float64 (RT only, write_Vh_stock :897-967)
+  rtdata.bin      RT with trans_longi /= 'tr' (write_rtdata :1925-1957): three sequential
+                  records, (Ac_tot(:,itt), Ac_tot(:,itt+1)), (Ac_ind(:,itt), Ac_ind(:,itt+1)),
+                  Ac_ind(:,0), float64
 
 or, with method_wf_distributor = 'slice', ``k_KKKKKK_ob_BBBBBB/wfn_ob_OOOOOO.dat`` files of
 the nspin blocks of one orbital (write_sliced :633-675).  Sequential records carry 4-byte
@@ -140,6 +143,23 @@ def read_vh_stock(directory, n):
     nx, ny, nz = n
     rec = read_records(os.path.join(directory, "Vh_stock.bin"))
     return tuple(np.frombuffer(r, dtype="<f8").reshape(nz, ny, nx).copy() for r in rec[:2])
+
+
+def write_rtdata(directory, ac_tot, ac_ind, ac_ind0):
+    """rtdata.bin: ac_tot = (Ac_tot(itt), Ac_tot(itt+1)), ac_ind likewise (2 x 3 each), ac_ind0 = Ac_ind(0)."""
+    with open(os.path.join(directory, "rtdata.bin"), "wb") as f:
+        write_record(f, np.ascontiguousarray(ac_tot, dtype="<f8").reshape(6).tobytes())
+        write_record(f, np.ascontiguousarray(ac_ind, dtype="<f8").reshape(6).tobytes())
+        write_record(f, np.ascontiguousarray(ac_ind0, dtype="<f8").reshape(3).tobytes())
+
+
+def read_rtdata(directory):
+    """(Ac_tot(itt), Ac_tot(itt+1)) (2, 3), (Ac_ind(itt), Ac_ind(itt+1)) (2, 3) and Ac_ind(0) (3,)."""
+    rec = read_records(os.path.join(directory, "rtdata.bin"))
+    if len(rec) < 3 or len(rec[0]) != 48 or len(rec[1]) != 48 or len(rec[2]) != 24:
+        raise ValueError(f"{directory}/rtdata.bin: records {[len(r) for r in rec]}, expected 48, 48, 24 bytes")
+    f = [np.frombuffer(r, dtype="<f8") for r in rec[:3]]
+    return f[0].reshape(2, 3).copy(), f[1].reshape(2, 3).copy(), f[2].copy()
 
 
 def write_checkpoint(directory, psi, ny, rocc, itt, nprocs=1, if_real_orbital=False, vh_stock=None,

@@ -271,19 +271,31 @@ class RTPropagator:
 
     def checkpoint(self, directory, sliced_block=0):
         """The current orbitals, occupations and step count in SALMON's restart layout
-        (checkpoint_rt, src/io/checkpoint_restart.f90:231-275; salmon2_amd.restart).  The
-        induced field of trans_longi = 'lo' and the learned predictor's history are not
-        part of it."""
+        (checkpoint_rt, src/io/checkpoint_restart.f90:231-275; salmon2_amd.restart), with
+        Vh_stock.bin (the Hartree potential of the current density, both records) for the
+        self-consistent step and rtdata.bin (the fields at itt and itt+1) for trans_longi =
+        'lo'.  The learned predictor's history is not part of it."""
         from . import restart
 
         torch.cuda.synchronize(self.dev)
+        vh = None
+        if self.scf:
+            v = torch.zeros_like(self.vpsl)
+            self.h.hartree(self.rho, v)
+            vh = (v.cpu().numpy(),) * 2
         restart.write_checkpoint(directory, self.psi_a.cpu().numpy(), self.s.ny, self.s.rocc, self.itt,
-                                 sliced_block=sliced_block)
+                                 sliced_block=sliced_block, vh_stock=vh)
+        if self.trans_longi == "lo":
+            t = self.itt
+            if t + 1 >= len(self.ac):
+                raise ValueError(f"checkpoint at step {t}: the field table ends at row {len(self.ac) - 1}")
+            restart.write_rtdata(directory, self.ac[t:t + 2], self.ac_ind[t:t + 2], self.ac_ind[0])
 
     @classmethod
     def from_restart(cls, sysm, directory, nt, **kw):
         """Continue from a restart directory (restart_rt, src/io/checkpoint_restart.f90:277-306):
-        orbitals and occupations from its files, the step count from info.bin; nt = the total
+        orbitals and occupations from its files, the step count from info.bin, and for
+        trans_longi = 'lo' the fields of rtdata.bin (read_rtdata :1959-1990); nt = the total
         number of steps of the run (outputs are indexed from step 1)."""
         import dataclasses
 
@@ -293,7 +305,13 @@ class RTPropagator:
                                  ly=sysm.ly)
         p = cls(dataclasses.replace(sysm, rocc=d["rocc"]), psi=d["psi"], **kw)
         p.setup_outputs(nt)
-        p.itt = d["iter"]
+        t = d["iter"]
+        p.itt = t
+        if p.trans_longi == "lo":
+            ac_tot, ac_ind, ac_ind0 = restart.read_rtdata(directory)
+            p.ac[t:t + 2] = ac_tot
+            p.ac_ind[t:t + 2] = ac_ind
+            p.ac_ind[0] = ac_ind0
         return p
 
     def write_rt_data(self, path, res=None):

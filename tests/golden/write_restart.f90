@@ -1,7 +1,8 @@
 ! Writes a small restart directory with the Fortran I/O statements SALMON uses for its
 ! restart data (src/io/checkpoint_restart.f90: info.bin and occupation.bin sequential
 ! unformatted, wfn.bin stream access in (ix,iy,iz,is,io,ik) order, Vh_stock.bin two
-! sequential records), with values given by closed formulas the test recomputes.
+! sequential records, rtdata.bin the three records of write_rtdata), with values given by
+! closed formulas the test recomputes.
 ! Usage: write_restart <directory> [real]
 program write_restart
   implicit none
@@ -9,7 +10,8 @@ program write_restart
   complex(8) :: zwf(nx, ny, nz, nspin, no, nk)
   real(8) :: rwf(nx, ny, nz, nspin, no, nk)
   real(8) :: rocc(no, nk, nspin), vh1(nx, ny, nz), vh2(nx, ny, nz)
-  integer :: ix, iy, iz, is, io, ik
+  real(8) :: ac_tot(3, 0:1236), ac_ind(3, 0:1236)
+  integer :: ix, iy, iz, is, io, ik, itt
   character(256) :: dir, mode
   logical :: if_real
 
@@ -66,5 +68,18 @@ program write_restart
   open(97, file=trim(dir)//'/Vh_stock.bin', form='unformatted')
   write(97) vh1
   write(97) vh2
+  close(97)
+  ! write_rtdata (checkpoint_restart.f90:1925-1957) at itt = 1234
+  do itt = 0, 1236
+    do ix = 1, 3
+      ac_tot(ix, itt) = 1d-3 * itt + 0.1d0 * ix
+      ac_ind(ix, itt) = -2d-4 * itt + 0.01d0 * ix * ix
+    end do
+  end do
+  itt = 1234
+  open(97, file=trim(dir)//'/rtdata.bin', form='unformatted')
+  write(97) ac_tot(:, itt), ac_tot(:, itt + 1)
+  write(97) ac_ind(:, itt), ac_ind(:, itt + 1)
+  write(97) ac_ind(:, 0)
   close(97)
 end program write_restart

@@ -68,13 +68,25 @@ def test_reads_fortran_written_restart(fortran_dirs, mode):
     assert np.array_equal(d["rocc"], rocc)
     v1, v2 = R.read_vh_stock(fortran_dirs[mode], N)
     assert np.array_equal(v1, vh[0]) and np.array_equal(v2, vh[1])
+    ac_tot, ac_ind, ac_ind0 = R.read_rtdata(fortran_dirs[mode])
+    ref_tot, ref_ind = _rtdata_closed_form()
+    assert np.array_equal(ac_tot, ref_tot[1234:1236]) and np.array_equal(ac_ind, ref_ind[1234:1236])
+    assert np.array_equal(ac_ind0, ref_ind[0])
+
+
+def _rtdata_closed_form():
+    t = np.arange(0, 1237, dtype=np.float64)[:, None]
+    d = np.arange(1, 4, dtype=np.float64)[None, :]
+    return 1e-3 * t + 0.1 * d, -2e-4 * t + 0.01 * d * d
 
 
 @pytest.mark.parametrize("mode", ["complex", "real"])
 def test_writer_reproduces_fortran_bytes(fortran_dirs, mode, tmp_path):
     psi, rocc, vh = _closed_form(mode == "real")
     R.write_checkpoint(str(tmp_path), psi, N[1], rocc, 1234, nprocs=4, if_real_orbital=(mode == "real"), vh_stock=vh)
-    for name in ("info.bin", "occupation.bin", "wfn.bin", "Vh_stock.bin"):
+    tot, ind = _rtdata_closed_form()
+    R.write_rtdata(str(tmp_path), tot[1234:1236], ind[1234:1236], ind[0])
+    for name in ("info.bin", "occupation.bin", "wfn.bin", "Vh_stock.bin", "rtdata.bin"):
         a = open(os.path.join(fortran_dirs[mode], name), "rb").read()
         b = open(os.path.join(tmp_path, name), "rb").read()
         assert a == b, name
