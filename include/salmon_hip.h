@@ -147,6 +147,7 @@ int salmon_hip_taylor_reuse(salmon_hip_t h, const double *zc, int n_hamil, void 
 int salmon_hip_set_skip(salmon_hip_t h, const int *flag_dev);
 
 /* dst_dev[i] = src_dev[i] for i < n when *flag_dev != 0 (device arrays; on the handle's stream). */
+int salmon_hip_copy_if(salmon_hip_t h, const int *flag_dev, double *dst_dev, const double *src_dev, size_t n);
 
 /*
  * Streaming copy of `bytes` (a multiple of 16) from src_dev to dst_dev with dwordx4
@@ -156,7 +157,6 @@ int salmon_hip_set_skip(salmon_hip_t h, const int *flag_dev);
  */
 int salmon_hip_stream_copy(salmon_hip_t h, void *dst_dev, const void *src_dev, size_t bytes, int reps,
                            double *avg_ms);
-int salmon_hip_copy_if(salmon_hip_t h, const int *flag_dev, double *dst_dev, const double *src_dev, size_t n);
 
 /*
  * Arithmetic of salmon_hip_taylor (no counterpart in the reference, which is
@@ -197,6 +197,20 @@ int salmon_hip_set_overlap(salmon_hip_t h, int on);
  * (calc_density, src/common/density_matrix.f90:26-236).
  */
 int salmon_hip_density(salmon_hip_t h, const void *psi, double *rho_dev);
+
+/*
+ * salmon_hip_density with no device allocation anywhere in the call (no counterpart in
+ * the reference; for callers that cannot allocate: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same kernels and summation order,
+ * so rho equals salmon_hip_density's bit for bit; the orbital-group partial sums go to a
+ * file-scope device array of 8 Mi doubles (64 MiB) per GPU, held until the process ends,
+ * instead of handle memory.  Returns SALMON_HIP_EINVAL when the partials
+ * (ceil(no*nk/32) groups x nx*ny*nz x nspin doubles, none for one group) exceed it.
+ * Every call on one GPU, from any handle, shares that array, whose contents are whatever
+ * the last call left (every partial a call reads it writes first): calls that overlap on
+ * one GPU (different streams or host threads) race.
+ */
+int salmon_hip_density_noalloc(salmon_hip_t h, const void *psi, double *rho_dev);
 
 /*
  * curr(3, nspin) into a DEVICE array (calc_current, src/common/density_matrix.f90:240-463,

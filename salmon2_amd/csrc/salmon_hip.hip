@@ -29,6 +29,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1565,7 +1566,7 @@ __global__ __launch_bounds__(kBlock) void k_vshift(TH *h, const double2 *psi, co
 
 // ---------------------------------------------------------------------------
 // Density: rho(p) = sum_{ik, io} (|psi|^2 rocc) wtk, orbitals in (ik, io) order,
-// in groups of kDensGroup summed in order, groups combined in order.
+// in groups of gsz (kDensGroup by default) summed in order, groups combined in order.
 // ---------------------------------------------------------------------------
 struct DensArgs {
   const double2 *psi;
@@ -1574,7 +1575,7 @@ struct DensArgs {
   double *part;         // (ngrid, nspin, ngrp)
   double *rho;          // (ngrid, nspin)
   size_t osz;
-  int nx, ny, nz, ly, nspin, no, nk, ngrp;
+  int nx, ny, nz, ly, nspin, no, nk, ngrp, gsz;
   const int *skip;      // device flag: the launch does nothing when it is set (null: never)
 };
 
@@ -1591,7 +1592,7 @@ __global__ __launch_bounds__(kBlock) void k_density_part(DensArgs a) {
   const int z = yz / a.ny;
   const size_t off = (size_t)x + (size_t)a.nx * ((size_t)y + (size_t)a.ly * z);
   const int M = a.no * a.nk;
-  const int m0 = grp * kDensGroup, m1 = min(M, m0 + kDensGroup);
+  const int m0 = grp * a.gsz, m1 = min(M, m0 + a.gsz);
   double acc = 0.0;
   for (int mm = m0; mm < m1; ++mm) {
     const int io = mm % a.no, ik = mm / a.no;
@@ -1605,6 +1606,13 @@ __global__ __launch_bounds__(kBlock) void k_density_part(DensArgs a) {
   else
     a.part[g + ngrid * ((size_t)ispin + (size_t)a.nspin * grp)] = acc;
 }
+
+// Partial sums of salmon_hip_density_noalloc: a file-scope device array (one copy per GPU,
+// held until the process ends) in place of the handle's partial buffer.  Host code never
+// passes its name to a kernel: the per-device address from hipGetSymbolAddress, resolved
+// once per device when a handle is created there, is what the launches use.
+constexpr size_t kNoAllocScratch = (size_t)8 << 20;  // doubles (64 MiB)
+static __device__ __align__(256) double scratch[kNoAllocScratch];
 
 __global__ __launch_bounds__(kBlock) void k_density_sum(DensArgs a) {
   if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
@@ -2047,6 +2055,7 @@ struct salmon_hip_ctx {
   int prec = SALMON_HIP_PREC_FP64;
   // observables scratch
   double *d_curpart = nullptr, *d_contrib = nullptr, *d_denspart = nullptr;
+  double *d_scratch = nullptr;  // this device's copy of `scratch` (salmon_hip_density_noalloc)
   int dens_ngrp = 1;
   // Hartree (salmon_hip_set_poisson): twiddle tables per axis, 4 pi/|G|^2 table, work
   bool have_cell = false;
@@ -2087,6 +2096,27 @@ static int check(salmon_hip_t h) {
 }
 
 // Error text from the other translation units of the library (the predictor).
+// Per-device address of `scratch`, resolved once per device with that device current.
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kMaxDevices] = {};
+
+static int resolve_scratch(double **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(SALMON_HIP_EHIP, "hipGetDevice failed or device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(SALMON_HIP_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_dev[dev] = (double *)ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
 void salmon_hip_internal_error(const char *msg) { g_err = msg; }
 
 extern "C" {
@@ -2148,6 +2178,7 @@ int salmon_hip_create(int nx, int ny, int nz, int ly, int nspin, int no, int nk,
   if ((rc = dalloc(&c->d_curpart, 4 * (size_t)c->nyt * c->norb))) return bail(rc);
   if ((rc = dalloc(&c->d_contrib, 3 * (size_t)no * nk))) return bail(rc);
   if ((rc = dalloc(&c->d_sink, 1024))) return bail(rc);  // one element per thread of the largest tile
+  if ((rc = resolve_scratch(&c->d_scratch))) return bail(rc);
   *out = c;
   return 0;
 }
@@ -2989,19 +3020,18 @@ int salmon_hip_set_launch_shape(salmon_hip_t c, int whole_plane, int z_chunks) {
   return 0;
 }
 
-int salmon_hip_density(salmon_hip_t c, const void *psi, double *rho_dev) {
-  if (int rc = check(c)) return rc;
-  if (!psi || !rho_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+static int density_impl(salmon_hip_t c, const void *psi, double *rho_dev, double *part) {
   DensArgs a;
   a.psi = (const double2 *)psi;
   a.rocc = c->d_rocc;
   a.wtk = c->d_wtk;
-  a.part = c->d_denspart;
+  a.part = part;
   a.rho = rho_dev;
   a.osz = c->osz;
   a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.ly = c->ly;
   a.nspin = c->nspin; a.no = c->no; a.nk = c->nk;
   a.ngrp = c->dens_ngrp;
+  a.gsz = kDensGroup;
   a.skip = c->d_skip;
   dim3 grid((unsigned)((c->ngrid + kBlock - 1) / kBlock), a.ngrp, c->nspin);
   hipLaunchKernelGGL(k_density_part, grid, dim3(kBlock), 0, c->stream, a);
@@ -3012,6 +3042,25 @@ int salmon_hip_density(salmon_hip_t c, const void *psi, double *rho_dev) {
     LAUNCHCHK("k_density_sum");
   }
   return 0;
+}
+
+int salmon_hip_density(salmon_hip_t c, const void *psi, double *rho_dev) {
+  if (int rc = check(c)) return rc;
+  if (!psi || !rho_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+  return density_impl(c, psi, rho_dev, c->d_denspart);
+}
+
+int salmon_hip_density_noalloc(salmon_hip_t c, const void *psi, double *rho_dev) {
+  if (int rc = check(c)) return rc;
+  if (!psi || !rho_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+  if (!c->d_scratch) return fail(SALMON_HIP_ESTATE, "device scratch not resolved");
+  const size_t need = c->dens_ngrp > 1 ? c->ngrid * (size_t)c->nspin * (size_t)c->dens_ngrp : 0;
+  if (need > kNoAllocScratch)
+    return fail(SALMON_HIP_EINVAL,
+                "density_noalloc: %zu partial sums (%d groups of %d orbitals x %zu points x %d spins) exceed the "
+                "device scratch of %zu doubles",
+                need, c->dens_ngrp, kDensGroup, c->ngrid, c->nspin, kNoAllocScratch);
+  return density_impl(c, psi, rho_dev, c->d_scratch);
 }
 
 static int current_impl(salmon_hip_t c, const double *vec_Ac, const double *actab, const int *itt, const void *psi,

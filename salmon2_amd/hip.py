@@ -47,6 +47,7 @@ EXPORTS = (
     "salmon_hip_batch_capture",
     "salmon_hip_batch_replay",
     "salmon_hip_density",
+    "salmon_hip_density_noalloc",
     "salmon_hip_current",
     "salmon_hip_dipole",
     "salmon_hip_set_poisson",
@@ -117,6 +118,7 @@ def load():
     L.salmon_hip_batch_capture.argtypes = [vp]
     L.salmon_hip_batch_replay.argtypes = [vp, C.c_int]
     L.salmon_hip_density.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.salmon_hip_density_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_current.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p]
     L.salmon_hip_dipole.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_set_poisson.argtypes = [C.c_void_p, _dp]
@@ -273,6 +275,11 @@ class Handle:
 
     def density(self, psi, rho):
         _chk(load().salmon_hip_density(self.h, C.c_void_p(_ptr(psi)), C.c_void_p(_ptr(rho))), "density")
+
+    def density_noalloc(self, psi, rho):
+        """density with its partial sums in the library's per-GPU device array (no allocation)."""
+        _chk(load().salmon_hip_density_noalloc(self.h, C.c_void_p(_ptr(psi)), C.c_void_p(_ptr(rho))),
+             "density_noalloc")
 
     def current(self, vec_ac, psi, curr):
         a = np.ascontiguousarray(vec_ac, dtype=np.float64)
