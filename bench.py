@@ -174,10 +174,10 @@ def load_predictor(config):
     return None, f"no committed predictor weights for {config}"
 
 
-def make_prop(s, psi, form, predictor):
+def make_prop(s, psi, form, predictor, record=False):
     from salmon2_amd.rt import RTPropagator
 
-    kw = dict(FORMS[form])
+    kw = dict(FORMS[form], record=record)
     tol = kw.pop("tol", None)
     if kw.pop("learned", False):
         kw["learned"] = predictor
@@ -190,11 +190,24 @@ def accuracy_run(s, psi, form, nt, predictor):
     import numpy as np
     import torch
 
-    prop = make_prop(s, psi, form, predictor)
+    plain_pc = FORMS[form]["pc"] and not FORMS[form].get("learned")
+    prop = make_prop(s, psi, form, predictor, record=plain_pc)
+    v0 = prop.vcur.clone() if plain_pc else None
     res = prop.run(nt)
     out = dict(curr=res["curr"], dip=res["dip"], rne=res["rne"])
+    if plain_pc:
+        # the quantity the learned step is gated on, for the predictor-corrector's own midpoint:
+        # max |V_mid - (V(t) + V(t+dt)) / 2| per step
+        vs = [v0] + [r[0] for r in prop.record]
+        r = np.array([float((m - 0.5 * (vs[i] + vs[i + 1])).abs().max()) for i, (_, m, _) in enumerate(prop.record)])
+        out["pc_residual"] = {q: float(np.quantile(r, v)) for q, v in
+                              (("min", 0.0), ("p10", 0.1), ("median", 0.5), ("p90", 0.9))}
+        prop.record = None
     if prop.learned is not None:
         out["accepted"], out["attempted"], out["resid_median"] = prop.predictor_stats()
+        r = prop.resid_log[:min(out["attempted"], prop.resid_log.numel())].cpu().numpy()
+        out["resid_quantiles"] = {q: float(np.quantile(r, v)) for q, v in (("min", 0.0), ("p10", 0.1), ("p90", 0.9))} \
+            if r.size else None
     assert np.all(np.isfinite(out["curr"])), f"non-finite current in {form}"
     prop.close()
     del prop
@@ -243,6 +256,37 @@ def profiled_run(s, psi, form, steps, warmup, predictor):
     del prop
     torch.cuda.empty_cache()
     return kern
+
+
+def fortran_hosted(s, psi, steps, warmup):
+    """The fixed-potential FP64 RT loop driven from Fortran through the drop-in wrappers with
+    SALMON's signatures (tests/fortran/test_dropin.f90 'rt': spsi_in, spsi_out and tpsi
+    resident as after `!$acc enter data`; rho and J copied to the host arrays every step as
+    calc_density / calc_current return them); steps/s over `steps` steps after `warmup`."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    from salmon2_amd import build
+
+    if not os.path.exists(build.DRIVER_EXE):
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "tests", "fortran"))
+    import dropin_io
+
+    nt = steps + warmup
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        host = s.psi if psi is None else psi.cpu().numpy()
+        dropin_io.write_input(fin, s, np.zeros(3), psi=host, nt=nt, nwarm=warmup)
+        del host
+        r = subprocess.run([build.DRIVER_EXE, "rt", fin, fout], capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError("Fortran driver failed: " + r.stdout + r.stderr)
+        el = float(np.fromfile(fout, dtype=np.float64, count=1)[0])
+    return {"steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps,
+            "host": "Fortran (flang) driver over the iso_c_binding drop-in wrappers, orbitals resident"}
 
 
 def copy_ceiling(nbytes=1 << 30, reps=10):
@@ -334,6 +378,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--batch-steps", type=int, default=100, help="batched-form steps (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fortran-steps", type=int, default=200,
+                    help="steps of the Fortran-hosted fixed-potential loop (0 = skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the run, copy 1 GiB device-to-device (HBM-traffic counter calibration)")
     args = ap.parse_args()
@@ -386,9 +432,11 @@ def main():
         if base_sps:
             o["speedup_vs_" + BASE_FORM] = sps / base_sps
         r = acc[f]
+        if "pc_residual" in r:
+            o["pc_self_consistency_residual"] = r["pc_residual"]
         if "accepted" in r:
             o["predictor"] = {"accepted_steps": r["accepted"], "attempted_steps": r["attempted"],
-                              "median_residual": r["resid_median"],
+                              "median_residual": r["resid_median"], "residual_quantiles": r["resid_quantiles"],
                               "tol": FORMS[f].get("tol", predictor.tol), "weights": pred_note}
         if f != refs[f]:
             b = acc[refs[f]]
@@ -467,6 +515,12 @@ def main():
     }
     acc = None
     torch.cuda.empty_cache()
+    if args.fortran_steps > 0 and "fixedv-fp64" in form_out:
+        fh = fortran_hosted(s, psi, args.fortran_steps, 10)
+        if fh is not None:
+            fh["vs_python_hosted_fixedv-fp64"] = fh["steps_per_s"] / form_out["fixedv-fp64"]["steps_per_s"]
+            out["fortran_hosted_fixedv-fp64"] = fh
+            log(f"[bench] fortran-hosted fixedv-fp64: {fh['steps_per_s']:.2f} steps/s")
     if args.batch_steps > 0 and args.config == "c2":
         out["batch"] = run_batch(args.batch_steps, 4)
     if args.calib_copy:
