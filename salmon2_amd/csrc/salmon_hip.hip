@@ -1305,10 +1305,17 @@ __device__ __forceinline__ void stencil_tilez(const StencilArgs &a, int bid, dou
   }
   CH q[9];   // own values of planes p-8 .. p (q[8] = p) after the push at plane p
   CH hq[4];  // centre + x/y part of planes p-3 .. p; plane p's sits in hq[(p - z0) & 3]
+  // With the whole column in one chunk the planes past its end are its own first four planes
+  // (periodic z), whose own values are kept from the first pass: the tail needs only those
+  // (no x/y part is formed there), so it fetches nothing from HBM (its slot DMAs re-read the
+  // last plane, an L2 hit, to keep the per-plane memory-op count fixed).
+  const bool wrapkeep = a.nzt == 1;
+  CH keep[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     q[i] = cvt2<CH>(make_double2(0.0, 0.0));
     hq[i] = q[i];
+    keep[i] = q[i];
     q[5 + i] = cvt2<CH>(in[colOff + zs * wrapi(z0 - 4 + i, nz)]);
   }
   q[4] = q[0];
@@ -1323,12 +1330,14 @@ __device__ __forceinline__ void stencil_tilez(const StencilArgs &a, int bid, dou
     constexpr int P = decltype(ph)::v;
     wait_vm<2 * kOps>();
     __syncthreads();
-    dma_tile(min(p + 3, z1 + 3), (P + 3) & 3);
+    dma_tile(p + 3 < z1 ? p + 3 : (wrapkeep ? z1 - 1 : min(p + 3, z1 + 3)), (P + 3) & 3);
     dma_v(min(p + 3, z1 - 1), (P + 3) & 3);
     const TI *T = tileR + SLOT * P;
 #pragma unroll
     for (int i = 0; i < 8; ++i) q[i] = q[i + 1];
-    q[8] = cvt2<CH>(T[own]);
+    // in the tail of a whole-column chunk p = z1 + P wraps to plane P (z0 = 0, TZ % 4 == 0)
+    q[8] = (wrapkeep && p >= z1) ? keep[P] : cvt2<CH>(T[own]);
+    if (p < z0 + 4) keep[P] = q[8];
     const CH hp = hq[P];  // plane p - 4
     if (p < z1) {
       CH v, w;
