@@ -382,6 +382,9 @@ def main():
                     help="steps of the Fortran-hosted fixed-potential loop (0 = skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the run, copy 1 GiB device-to-device (HBM-traffic counter calibration)")
+    ap.add_argument("--trace", action="store_true",
+                    help="for profilers: only --warmup + --steps steps of each form (no accuracy runs, roofline, "
+                         "batch or CPU baseline) and --calib-copy; prints one line per form")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -404,6 +407,17 @@ def main():
         if predictor is None:
             log(f"[bench] {pred_note}: learned forms skipped")
             forms = [f for f in forms if not FORMS[f].get("learned")]
+    if args.trace:
+        for f in forms:
+            el = timed_run(s, psi, f, args.steps, args.warmup, predictor)
+            print(f"[trace] {args.config} {f}: {args.steps} steps, {1e3 * el / args.steps:.3f} ms/step", flush=True)
+        if args.calib_copy:
+            src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            print("[trace] calibration: 1 GiB device-to-device copy", flush=True)
+        return
     refs = {f: ("fixedv-fp64" if not FORMS[f]["scf"] else BASE_FORM) for f in forms}
     for f in list(forms):
         if refs[f] not in forms:

@@ -17,7 +17,7 @@
 //                     deterministic per-point gather over a point->(atom, j) inverse map
 //   k_density_*       calc_density (density_matrix.f90:150-230)
 //   k_current_*       stencil_current + calc_current (density_matrix.f90:240-530)
-//   k_dipole          calc_dip (src/rt/dip.f90:114-225)
+//   k_dipole_part/_sum calc_dip (src/rt/dip.f90:114-225)
 //   k_dft_axis        Hartree potential (poisson_ft, src/poisson/poisson_periodic.f90:23-242)
 //   k_xc_pz           PZ81 LDA (src/xc/builtin_pz.f90:23-83)
 //   k_update_vlocal   update_vlocal (src/common/hamiltonian.f90:813-843) with the
@@ -26,6 +26,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -125,9 +126,7 @@ __device__ __forceinline__ void kvec_field(const KvecArgs &a, double &ax, double
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_update_kvector(KvecArgs a) {
-  const int ilma = blockIdx.x;
-  const int ik = blockIdx.y;
+__device__ __forceinline__ void update_kvector_body(const KvecArgs &a, int ilma, int ik) {
   double ax, ay, az;
   kvec_field(a, ax, ay, az);
   const double kx = a.veck[3 * ik + 0] + ax;
@@ -163,6 +162,8 @@ __global__ __launch_bounds__(kBlock) void k_update_kvector(KvecArgs a) {
     a.zent[(size_t)(ilma - a.atom_lma[ia].x) + (size_t)a.cmax * ((size_t)e + (size_t)a.nent * ik)] = z;
   }
 }
+
+__global__ __launch_bounds__(kBlock) void k_update_kvector(KvecArgs a) { update_kvector_body(a, blockIdx.x, blockIdx.y); }
 
 // Per-k stencil terms only (hpsi reads system%vec_Ac at every call,
 // src/common/hamiltonian.f90:146-155; the phase table is left untouched).
@@ -215,18 +216,20 @@ struct ProjArgs {
   const int *skip;        // device flag: the launch does nothing when it is set (null: never)
 };
 
+// XCD-aware order of a launch of nblk blocks (block b runs on XCD b % 8): the blocks of one XCD
+// take a contiguous range of the work items, so the cache lines that neighbouring items share
+// are read through one L2
+__device__ __forceinline__ int xcd_order(int b, int nblk) {
+  const int nfull = (nblk / 8) * 8;
+  return b < nfull ? (b % 8) * (nblk / 8) + b / 8 : b;
+}
+
 template <int NW, typename TP>
-__global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
-  if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
+__device__ __forceinline__ void proj_body(const ProjArgs &a, int bid) {
   constexpr int MAXT = (NW * kMaxLmaPerAtom + 7) / 8;
   __shared__ double2 sp[kProjJC][kProjOrb];
   __shared__ double2 zsh[kProjJC][kMaxLmaPerAtom];
   __shared__ double rsh[kProjJC][3];
-  // XCD-aware order: the blocks of one XCD take a contiguous range of (group, atom, k), so the
-  // cache lines that neighbouring spheres share are read through one L2
-  int bid = blockIdx.x;
-  const int nfull = ((int)gridDim.x / 8) * 8;
-  if (bid < nfull) bid = (bid % 8) * ((int)gridDim.x / 8) + bid / 8;
   const int grp = bid % a.ngrp;
   const int t = bid / a.ngrp;
   const int ia = t % a.natom;
@@ -364,6 +367,12 @@ __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
   }
 }
 
+template <int NW, typename TP>
+__global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
+  if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
+  proj_body<NW, TP>(a, xcd_order(blockIdx.x, gridDim.x));
+}
+
 // ---------------------------------------------------------------------------
 // KB term of one H application fused with the projection of its result: for a Taylor term
 // h = H h_prev the pass adds s * sum_lma c_prev(lma) zekr(j, lma) at the sphere points that one
@@ -387,19 +396,15 @@ struct ProjScatArgs {
 };
 
 template <typename TP, int LM>
-__global__ __launch_bounds__(kBlock) void k_proj_scatter(ProjScatArgs A) {
+__device__ __forceinline__ void proj_scatter_body(const ProjScatArgs &A, int bid) {
   static_assert(LM <= kMaxLmaPerAtom, "LM bounds the projectors of one atom");
   const ProjArgs &a = A.p;
-  if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
   constexpr int MAXT = (kMaxLmaPerAtom + 7) / 8;
   // rows padded by one element: the per-point accesses of the completion step (lanes on points)
   // then fall on different banks
   __shared__ double2 sp[kProjJC][kProjOrb + 1];
   __shared__ double2 zsh[kProjJC][kMaxLmaPerAtom + 1];
   extern __shared__ double2 csh[];  // cmax x kProjOrb
-  int bid = blockIdx.x;
-  const int nfull = ((int)gridDim.x / 8) * 8;
-  if (bid < nfull) bid = (bid % 8) * ((int)gridDim.x / 8) + bid / 8;
   const int grp = bid % a.ngrp;
   const int t = bid / a.ngrp;
   const int ia = t % a.natom;
@@ -539,6 +544,12 @@ __global__ __launch_bounds__(kBlock) void k_proj_scatter(ProjScatArgs A) {
       a.out[((size_t)ik * a.nlma + ilma) * a.nb + ob] = make_double2(re * sc, im * sc);
     }
   }
+}
+
+template <typename TP, int LM>
+__global__ __launch_bounds__(kBlock) void k_proj_scatter(ProjScatArgs A) {
+  if (A.p.skip && *A.p.skip) return;  // a device-side gate switched this launch off
+  proj_scatter_body<TP, LM>(A, xcd_order(blockIdx.x, gridDim.x));
 }
 
 constexpr int kMaxComb = 5;   // Taylor order up to 6
@@ -1477,11 +1488,10 @@ struct ScatterArgs {
 };
 
 template <typename TH, int G, int CM>
-__global__ __launch_bounds__(kBlock) void k_nl_scatter(ScatterArgs a) {
-  if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  const int ik = blockIdx.y / a.ngrp;
-  const int b0 = (blockIdx.y % a.ngrp) * G;
+__device__ __forceinline__ void nl_scatter_body(const ScatterArgs &a, int bx, int by) {
+  const int i = bx * kBlock + threadIdx.x;
+  const int ik = by / a.ngrp;
+  const int b0 = (by % a.ngrp) * G;
   if (i >= a.ntp) return;
   const int4 inf = a.tp_info[i];
   const int nob = min(G, a.nb - b0);
@@ -1548,6 +1558,12 @@ __global__ __launch_bounds__(kBlock) void k_nl_scatter(ScatterArgs a) {
   }
 }
 
+template <typename TH, int G, int CM>
+__global__ __launch_bounds__(kBlock) void k_nl_scatter(ScatterArgs a) {
+  if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
+  nl_scatter_body<TH, G, CM>(a, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // h += dv * psi over the orbital blocks (dv per point and spin; padding rows untouched):
 // turns H_old psi into H_new psi when only V_local changed (salmon_hip_taylor_reuse).
@@ -1588,12 +1604,9 @@ struct DensArgs {
   const int *skip;      // device flag: the launch does nothing when it is set (null: never)
 };
 
-__global__ __launch_bounds__(kBlock) void k_density_part(DensArgs a) {
-  if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
+__device__ __forceinline__ void density_part_body(const DensArgs &a, int bx, int grp, int ispin) {
   const size_t ngrid = (size_t)a.nx * a.ny * a.nz;
-  const size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x;
-  const int grp = blockIdx.y;
-  const int ispin = blockIdx.z;
+  const size_t g = (size_t)bx * kBlock + threadIdx.x;
   if (g >= ngrid) return;
   const int x = g % a.nx;
   const int yz = g / a.nx;
@@ -1616,6 +1629,11 @@ __global__ __launch_bounds__(kBlock) void k_density_part(DensArgs a) {
     a.part[g + ngrid * ((size_t)ispin + (size_t)a.nspin * grp)] = acc;
 }
 
+__global__ __launch_bounds__(kBlock) void k_density_part(DensArgs a) {
+  if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
+  density_part_body(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 // Partial sums of salmon_hip_density_noalloc: a file-scope device array (one copy per GPU,
 // held until the process ends) in place of the handle's partial buffer.  Host code never
 // passes its name to a kernel: the per-device address from hipGetSymbolAddress, resolved
@@ -1623,14 +1641,18 @@ __global__ __launch_bounds__(kBlock) void k_density_part(DensArgs a) {
 constexpr size_t kNoAllocScratch = (size_t)8 << 20;  // doubles (64 MiB)
 static __device__ __align__(256) double scratch[kNoAllocScratch];
 
-__global__ __launch_bounds__(kBlock) void k_density_sum(DensArgs a) {
-  if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
+__device__ __forceinline__ void density_sum_body(const DensArgs &a, int bx) {
   const size_t ngrid = (size_t)a.nx * a.ny * a.nz;
-  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  const size_t i = (size_t)bx * kBlock + threadIdx.x;
   if (i >= ngrid * a.nspin) return;
   double acc = 0.0;
   for (int grp = 0; grp < a.ngrp; ++grp) acc = acc + a.part[i + ngrid * a.nspin * (size_t)grp];
   a.rho[i] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_density_sum(DensArgs a) {
+  if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
+  density_sum_body(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1661,11 +1683,11 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return r;
 }
 
-__global__ __launch_bounds__(kBlock) void k_current_local(CurArgs a) {
+__device__ __forceinline__ void current_local_body(const CurArgs &a, int bid) {
   extern __shared__ double2 lds[];
   __shared__ double sh[kBlock];
-  const int ytile = blockIdx.x % a.nyt;
-  const int b = blockIdx.x / a.nyt;
+  const int ytile = bid % a.nyt;
+  const int b = bid / a.nyt;
   const int nx = a.nx, ny = a.ny, nz = a.nz, TY = a.TY;
   const int y0 = ytile * TY;
   const int tid = threadIdx.x;
@@ -1728,6 +1750,8 @@ __global__ __launch_bounds__(kBlock) void k_current_local(CurArgs a) {
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_current_local(CurArgs a) { current_local_body(a, blockIdx.x); }
+
 struct CurFinArgs {
   const double *part;      // (4, nslab, norb)
   const double2 *nl;       // (4, nlma, nb, nk) raw moments, or null
@@ -1749,9 +1773,9 @@ struct CurFinArgs {
 // ((j1 + j2) + jNL) rocc wtk (calc_current, density_matrix.f90:405-450), with
 // jNL = 2 sum_lma Im(conj(uVpsi_r) uVpsi) (calc_current_nonlocal :532-591)
 // reduced across the wave in a fixed tree.
-__global__ __launch_bounds__(kBlock) void k_current_orb(CurFinArgs a, int ispin) {
+__device__ __forceinline__ void current_orb_body(const CurFinArgs &a, int bx, int ispin) {
   const int M = a.no * a.nk;
-  const int mm = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int mm = bx * (kBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (mm >= M) return;
   const int io = mm % a.no, ik = mm / a.no;
@@ -1795,9 +1819,11 @@ __global__ __launch_bounds__(kBlock) void k_current_orb(CurFinArgs a, int ispin)
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_current_orb(CurFinArgs a, int ispin) { current_orb_body(a, blockIdx.x, ispin); }
+
 // Sum of the orbital contributions in (ik, io) order, divided by N_grid
 // (contributions staged through LDS in chunks so the serial sum reads LDS).
-__global__ __launch_bounds__(kBlock) void k_current_sum(CurFinArgs a, int ispin) {
+__device__ __forceinline__ void current_sum_body(const CurFinArgs &a, int ispin) {
   __shared__ double sh[3 * 1024];
   const int M = a.no * a.nk;
   double w4 = 0.0;
@@ -1812,43 +1838,78 @@ __global__ __launch_bounds__(kBlock) void k_current_sum(CurFinArgs a, int ispin)
   if (threadIdx.x < 3) a.curr[3 * ispin + threadIdx.x] = w4 * a.inv_ngrid;
 }
 
+__global__ __launch_bounds__(kBlock) void k_current_sum(CurFinArgs a, int ispin) { current_sum_body(a, ispin); }
+
 // ---------------------------------------------------------------------------
 // Dipole: out[0:3] = -sum(pos_d rho) Hgs_d Hvol, out[3] = sum(rho) Hvol
 // pos = i (odd n) or i - 1/2 (even n), i the 1-based grid index.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_dipole(const double *rho, double *out, int nx, int ny, int nz,
-                                                 int nspin, double hx, double hy, double hz, double hvol) {
-  __shared__ double sh[4][1024];
-  const size_t ngrid = (size_t)nx * ny * nz;
+// Dipole moment and electron count (calc_dip / subdip, src/rt/dip.f90:114-225): per block a
+// strided share of the points (32-bit indices), tree-reduced in LDS into part(4, block); the
+// final kernel sums the block partials in block order.  Positions are the grid indices with
+// the reference's half-cell shift on even extents.
+constexpr int kDipBlocks = 128;
+
+struct DipArgs {
+  const double *rho;
+  double *part, *out;    // part(4, kDipBlocks); out: dipole (3) and electron count
+  int nx, ny, nz, nspin;
+  double hx, hy, hz, hvol;
+};
+
+__device__ __forceinline__ void dipole_part_body(const DipArgs &a, int bx) {
+  const double *rho = a.rho;
+  double *part = a.part;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nspin = a.nspin;
+  __shared__ double sh[4][256];
+  const int ngrid = nx * ny * nz;
+  const double sx = (nx % 2 == 1) ? 1.0 : 0.5, sy = (ny % 2 == 1) ? 1.0 : 0.5, sz = (nz % 2 == 1) ? 1.0 : 0.5;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int is = 0; is < nspin; ++is)
-    for (size_t g = threadIdx.x; g < ngrid; g += 1024) {
+    for (int g = bx * 256 + threadIdx.x; g < ngrid; g += 256 * kDipBlocks) {
       const int x = g % nx;
-      const int y = (g / nx) % ny;
-      const int z = g / ((size_t)nx * ny);
-      const double r = rho[g + ngrid * is];
-      const double px = (nx % 2 == 1) ? (double)(x + 1) : ((double)(x + 1) - 0.5);
-      const double py = (ny % 2 == 1) ? (double)(y + 1) : ((double)(y + 1) - 0.5);
-      const double pz = (nz % 2 == 1) ? (double)(z + 1) : ((double)(z + 1) - 0.5);
-      acc[0] += px * r;
-      acc[1] += py * r;
-      acc[2] += pz * r;
+      const int yz = g / nx;
+      const int y = yz % ny;
+      const int z = yz / ny;
+      const double r = rho[(size_t)g + (size_t)ngrid * is];
+      acc[0] += ((double)x + sx) * r;
+      acc[1] += ((double)y + sy) * r;
+      acc[2] += ((double)z + sz) * r;
       acc[3] += r;
     }
   for (int d = 0; d < 4; ++d) sh[d][threadIdx.x] = acc[d];
   __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int d = 0; d < 4; ++d) sh[d][threadIdx.x] = sh[d][threadIdx.x] + sh[d][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) part[4 * bx + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_dipole_part(DipArgs a) { dipole_part_body(a, blockIdx.x); }
+
+// the final sum, on the first kDipBlocks threads of the block
+__device__ __forceinline__ void dipole_sum_body(const DipArgs &a) {
+  __shared__ double sh[4][kDipBlocks];
+  const bool on = threadIdx.x < kDipBlocks;
+  for (int d = 0; d < 4; ++d)
+    if (on) sh[d][threadIdx.x] = a.part[4 * threadIdx.x + d];
+  __syncthreads();
+  for (int s = kDipBlocks / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s)
       for (int d = 0; d < 4; ++d) sh[d][threadIdx.x] = sh[d][threadIdx.x] + sh[d][threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = -sh[0][0] * hx * hvol;
-    out[1] = -sh[1][0] * hy * hvol;
-    out[2] = -sh[2][0] * hz * hvol;
-    out[3] = sh[3][0] * hvol;
+    a.out[0] = -sh[0][0] * a.hx * a.hvol;
+    a.out[1] = -sh[1][0] * a.hy * a.hvol;
+    a.out[2] = -sh[2][0] * a.hz * a.hvol;
+    a.out[3] = sh[3][0] * a.hvol;
   }
 }
+
+__global__ __launch_bounds__(256) void k_dipole_sum(DipArgs a) { dipole_sum_body(a); }
 
 // ---------------------------------------------------------------------------
 // Hartree potential: poisson_ft (src/poisson/poisson_periodic.f90:23-242) as six
@@ -2065,6 +2126,7 @@ struct salmon_hip_ctx {
   // observables scratch
   double *d_curpart = nullptr, *d_contrib = nullptr, *d_denspart = nullptr;
   double *d_scratch = nullptr;  // this device's copy of `scratch` (salmon_hip_density_noalloc)
+  double *d_dippart = nullptr;  // dipole block partials (4 x kDipBlocks)
   int dens_ngrp = 1;
   // Hartree (salmon_hip_set_poisson): twiddle tables per axis, 4 pi/|G|^2 table, work
   bool have_cell = false;
@@ -2104,7 +2166,6 @@ static int check(salmon_hip_t h) {
   return 0;
 }
 
-// Error text from the other translation units of the library (the predictor).
 // Per-device address of `scratch`, resolved once per device with that device current.
 constexpr int kMaxDevices = 64;
 static std::mutex g_scratch_mu;
@@ -2126,6 +2187,7 @@ static int resolve_scratch(double **out) {
   return 0;
 }
 
+// Error text from the other translation units of the library (the predictor).
 void salmon_hip_internal_error(const char *msg) { g_err = msg; }
 
 extern "C" {
@@ -2140,6 +2202,7 @@ int salmon_hip_create(int nx, int ny, int nz, int ly, int nspin, int no, int nk,
   if (nx < 4 || ny < 4 || nz < 4) return fail(SALMON_HIP_EINVAL, "grid %dx%dx%d below Nd=4", nx, ny, nz);
   if (nx > kBlock) return fail(SALMON_HIP_EUNSUPPORTED, "nx=%d above %d", nx, kBlock);
   if (ly < ny) return fail(SALMON_HIP_EINVAL, "ly=%d < ny=%d", ly, ny);
+  if ((size_t)nx * ly * nz > (size_t)INT_MAX) return fail(SALMON_HIP_EUNSUPPORTED, "grid of %dx%dx%d above 2^31 points", nx, ly, nz);
   if (nspin != 1 && nspin != 2) return fail(SALMON_HIP_EINVAL, "nspin=%d", nspin);
   if (no < 1 || nk < 1) return fail(SALMON_HIP_EINVAL, "no=%d nk=%d", no, nk);
   if (!coef_lap || !coef_nab || !vec_k || !rocc || !wtk || !hgs) return fail(SALMON_HIP_EINVAL, "null input");
@@ -2187,6 +2250,7 @@ int salmon_hip_create(int nx, int ny, int nz, int ly, int nspin, int no, int nk,
   if ((rc = dalloc(&c->d_curpart, 4 * (size_t)c->nyt * c->norb))) return bail(rc);
   if ((rc = dalloc(&c->d_contrib, 3 * (size_t)no * nk))) return bail(rc);
   if ((rc = dalloc(&c->d_sink, 1024))) return bail(rc);  // one element per thread of the largest tile
+  if ((rc = dalloc(&c->d_dippart, 4 * (size_t)kDipBlocks))) return bail(rc);
   if ((rc = resolve_scratch(&c->d_scratch))) return bail(rc);
   *out = c;
   return 0;
@@ -2202,7 +2266,7 @@ int salmon_hip_destroy(salmon_hip_t c) {
   dfree(c->d_rinv); dfree(c->d_zekr); dfree(c->d_zent); dfree(c->d_c); dfree(c->d_curnl);
   dfree(c->d_c2); dfree(c->d_jexcl); dfree(c->d_tp_shared);
   dfree(c->d_tp_info); dfree(c->d_sink);
-  dfree(c->d_curpart); dfree(c->d_contrib); dfree(c->d_denspart);
+  dfree(c->d_curpart); dfree(c->d_contrib); dfree(c->d_denspart); dfree(c->d_dippart);
   for (int d = 0; d < 3; ++d) dfree(c->d_tw[d]);
   dfree(c->d_gcoef); dfree(c->d_vh); dfree(c->d_fft[0]); dfree(c->d_fft[1]);
   for (double2 *&p : c->tscr) dfree(p);
@@ -2414,8 +2478,7 @@ int salmon_hip_copy_if(salmon_hip_t c, const int *flag_dev, double *dst_dev, con
   return 0;
 }
 
-static int update_kvector_impl(salmon_hip_t c, const double *vec_Ac, const double *actab, const int *itt,
-                               int tmax = 0) {
+static KvecArgs kvec_args(salmon_hip_t c, const double *vec_Ac, const double *actab, const int *itt, int tmax) {
   KvecArgs a;
   a.veck = c->d_veck;
   a.rxyz = c->d_rxyz;
@@ -2440,6 +2503,12 @@ static int update_kvector_impl(salmon_hip_t c, const double *vec_Ac, const doubl
   a.nlma = c->nlma;
   a.nps = c->nps;
   a.nk = c->nk;
+  return a;
+}
+
+static int update_kvector_impl(salmon_hip_t c, const double *vec_Ac, const double *actab, const int *itt,
+                               int tmax = 0) {
+  const KvecArgs a = kvec_args(c, vec_Ac, actab, itt, tmax);
   dim3 grid(std::max(1, c->nlma), c->nk);
   hipLaunchKernelGGL(k_update_kvector, grid, dim3(kBlock), 0, c->stream, a);
   LAUNCHCHK("k_update_kvector");
@@ -2474,7 +2543,8 @@ int salmon_hip_set_vec_ac(salmon_hip_t c, const double *vec_Ac) {
   return 0;
 }
 
-static int launch_proj(salmon_hip_t c, const void *psi, int nw, bool f32 = false) {
+// projection of psi into out (c(nb, nlma, nk) scaled by rinv, or the current moments)
+static ProjArgs proj_args(salmon_hip_t c, const void *psi, double2 *out) {
   ProjArgs a;
   a.psi = psi;
   a.zekr = c->d_zekr;
@@ -2483,7 +2553,7 @@ static int launch_proj(salmon_hip_t c, const void *psi, int nw, bool f32 = false
   a.mps = c->d_mps;
   a.atom_lma = c->d_atom_lma;
   a.rinv = c->d_rinv;
-  a.out = nw == 1 ? c->d_c : c->d_curnl;
+  a.out = out;
   a.osz = c->osz;
   a.nb = c->nb;
   a.nk = c->nk;
@@ -2492,6 +2562,11 @@ static int launch_proj(salmon_hip_t c, const void *psi, int nw, bool f32 = false
   a.nlma = c->nlma;
   a.ngrp = (c->nb + kProjOrb - 1) / kProjOrb;
   a.skip = c->d_skip;
+  return a;
+}
+
+static int launch_proj(salmon_hip_t c, const void *psi, int nw, bool f32 = false) {
+  const ProjArgs a = proj_args(c, psi, nw == 1 ? c->d_c : c->d_curnl);
   dim3 grid((unsigned)(a.ngrp * c->natom * c->nk));
   hipEvent_t e0, e1;
   prof_events(c, nw == 1 ? SALMON_HIP_KERNEL_PROJ : SALMON_HIP_KERNEL_PROJ_CURRENT, &e0, &e1);
@@ -2654,31 +2729,33 @@ extern "C" {
 
 // shared_only: the touched points with several (atom, point) pairs (before k_proj_scatter);
 // e0/e1 override the profiling events (nullptr: this launch's own)
-static int launch_scatter(salmon_hip_t c, void *hout, double2 s, bool f32 = false, bool shared_only = false,
-                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, bool own_events = true) {
-  if ((shared_only ? c->ntp_shared : c->ntp) == 0) return 0;
+// KB scatter hout += s * sum c zekr at the touched points (or only those several pairs cover)
+static ScatterArgs scatter_args(salmon_hip_t c, void *hout, double2 s, const double2 *cbuf, bool shared_only, bool f32) {
   ScatterArgs a;
   a.hout = hout;
-  a.c = c->d_c;
+  a.c = cbuf;
   a.zent = c->d_zent;
-  a.tp_info = c->d_tp_info;
+  a.tp_info = shared_only ? c->d_tp_shared : c->d_tp_info;
   a.pt_lma = c->d_pt_lma;
   a.s = s;
   a.osz = c->osz;
-  a.ntp = c->ntp;
+  a.ntp = shared_only ? c->ntp_shared : c->ntp;
   a.nb = c->nb;
   a.nlma = c->nlma;
   a.nent = c->nent;
   a.cmax = c->cmax;
   a.skip = c->d_skip;
-  hipEvent_t e0 = ev0, e1 = ev1;
-  if (own_events) prof_events(c, SALMON_HIP_KERNEL_NL_SCATTER, &e0, &e1);
-  if (shared_only) {
-    a.tp_info = c->d_tp_shared;
-    a.ntp = c->ntp_shared;
-  }
   const int G = f32 ? kScatG32 : kScatG;
   a.ngrp = (c->nb + G - 1) / G;
+  return a;
+}
+
+static int launch_scatter(salmon_hip_t c, void *hout, double2 s, bool f32 = false, bool shared_only = false,
+                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, bool own_events = true) {
+  if ((shared_only ? c->ntp_shared : c->ntp) == 0) return 0;
+  const ScatterArgs a = scatter_args(c, hout, s, c->d_c, shared_only, f32);
+  hipEvent_t e0 = ev0, e1 = ev1;
+  if (own_events) prof_events(c, SALMON_HIP_KERNEL_NL_SCATTER, &e0, &e1);
   dim3 grid((unsigned)((a.ntp + kBlock - 1) / kBlock), (unsigned)(c->nk * a.ngrp));
   if (f32) {
     if (c->cmax <= kScatCM32)
@@ -2695,6 +2772,17 @@ static int launch_scatter(salmon_hip_t c, void *hout, double2 s, bool f32 = fals
   return 0;
 }
 
+// the fused KB pass of hout (cin = c of the previous term, cout = c of hout)
+static ProjScatArgs projscat_args(salmon_hip_t c, void *hout, const double2 *cin, double2 *cout) {
+  ProjScatArgs A;
+  A.p = proj_args(c, hout, cout);
+  A.cin = cin;
+  A.jexcl = c->d_jexcl;
+  A.s = make_double2(1.0, 0.0);
+  A.cmax = c->cmax;
+  return A;
+}
+
 // The KB term of hout = H in (c of `in` current in d_c) fused with the projection of hout: the
 // points with several pairs by k_nl_scatter, then k_proj_scatter; afterwards d_c holds the c of
 // hout.  Profiled as one SALMON_HIP_KERNEL_KB_FUSED interval.
@@ -2704,28 +2792,8 @@ static int launch_kb_fused(salmon_hip_t c, void *hout, bool f32) {
   const bool shared = c->ntp_shared > 0;
   if (shared)
     if (int rc = launch_scatter(c, hout, make_double2(1.0, 0.0), f32, true, e0, nullptr, false)) return rc;
-  ProjScatArgs A;
-  ProjArgs &a = A.p;
-  a.psi = hout;
-  a.zekr = c->d_zekr;
-  a.rxyz = c->d_rxyz;
-  a.jidx = c->d_jidx;
-  a.mps = c->d_mps;
-  a.atom_lma = c->d_atom_lma;
-  a.rinv = c->d_rinv;
-  a.out = c->d_c2;
-  a.osz = c->osz;
-  a.nb = c->nb;
-  a.nk = c->nk;
-  a.natom = c->natom;
-  a.nps = c->nps;
-  a.nlma = c->nlma;
-  a.ngrp = (c->nb + kProjOrb - 1) / kProjOrb;
-  a.skip = c->d_skip;
-  A.cin = c->d_c;
-  A.jexcl = c->d_jexcl;
-  A.s = make_double2(1.0, 0.0);
-  A.cmax = c->cmax;
+  const ProjScatArgs A = projscat_args(c, hout, c->d_c, c->d_c2);
+  const ProjArgs &a = A.p;
   const size_t shmem = sizeof(double2) * (size_t)c->cmax * kProjOrb;
   dim3 grid((unsigned)(a.ngrp * c->natom * c->nk));
   hipEvent_t s0 = shared ? nullptr : e0;
@@ -3029,7 +3097,7 @@ int salmon_hip_set_launch_shape(salmon_hip_t c, int whole_plane, int z_chunks) {
   return 0;
 }
 
-static int density_impl(salmon_hip_t c, const void *psi, double *rho_dev, double *part) {
+static DensArgs dens_args(salmon_hip_t c, const void *psi, double *rho_dev, double *part) {
   DensArgs a;
   a.psi = (const double2 *)psi;
   a.rocc = c->d_rocc;
@@ -3042,6 +3110,11 @@ static int density_impl(salmon_hip_t c, const void *psi, double *rho_dev, double
   a.ngrp = c->dens_ngrp;
   a.gsz = kDensGroup;
   a.skip = c->d_skip;
+  return a;
+}
+
+static int density_impl(salmon_hip_t c, const void *psi, double *rho_dev, double *part) {
+  const DensArgs a = dens_args(c, psi, rho_dev, part);
   dim3 grid((unsigned)((c->ngrid + kBlock - 1) / kBlock), a.ngrp, c->nspin);
   hipLaunchKernelGGL(k_density_part, grid, dim3(kBlock), 0, c->stream, a);
   LAUNCHCHK("k_density_part");
@@ -3072,8 +3145,7 @@ int salmon_hip_density_noalloc(salmon_hip_t c, const void *psi, double *rho_dev)
   return density_impl(c, psi, rho_dev, c->d_scratch);
 }
 
-static int current_impl(salmon_hip_t c, const double *vec_Ac, const double *actab, const int *itt, const void *psi,
-                        double *curr_dev, int tmax = 0) {
+static CurArgs cur_args(salmon_hip_t c, const void *psi) {
   CurArgs a;
   a.psi = (const double2 *)psi;
   a.part = c->d_curpart;
@@ -3082,13 +3154,13 @@ static int current_impl(salmon_hip_t c, const double *vec_Ac, const double *acta
   a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.ly = c->ly;
   a.TY = c->TY;
   a.nyt = c->nyt;
-  const size_t shm = sizeof(double2) * 2 * (size_t)(c->TY + 4) * c->nx;
-  // the projector moments run on the side stream, overlapping the local part
-  if (c->natom > 0)
-    if (int rc = launch_proj(c, psi, 4)) return rc;
-  hipLaunchKernelGGL(k_current_local, dim3((unsigned)(c->norb * c->nyt)), dim3(kBlock), shm, c->stream, a);
-  LAUNCHCHK("k_current_local");
-  if (int rc = join_proj(c)) return rc;
+  return a;
+}
+
+static size_t cur_shmem(salmon_hip_t c) { return sizeof(double2) * 2 * (size_t)(c->TY + 4) * c->nx; }
+
+static CurFinArgs curfin_args(salmon_hip_t c, const double *vec_Ac, const double *actab, const int *itt,
+                              double *curr_dev, int tmax) {
   CurFinArgs f;
   f.part = c->d_curpart;
   f.nl = c->natom > 0 ? c->d_curnl : nullptr;
@@ -3110,6 +3182,19 @@ static int current_impl(salmon_hip_t c, const double *vec_Ac, const double *acta
   f.no = c->no;
   f.nk = c->nk;
   f.nlma = c->nlma;
+  return f;
+}
+
+static int current_impl(salmon_hip_t c, const double *vec_Ac, const double *actab, const int *itt, const void *psi,
+                        double *curr_dev, int tmax = 0) {
+  const CurArgs a = cur_args(c, psi);
+  // the projector moments run on the side stream, overlapping the local part
+  if (c->natom > 0)
+    if (int rc = launch_proj(c, psi, 4)) return rc;
+  hipLaunchKernelGGL(k_current_local, dim3((unsigned)(c->norb * c->nyt)), dim3(kBlock), cur_shmem(c), c->stream, a);
+  LAUNCHCHK("k_current_local");
+  if (int rc = join_proj(c)) return rc;
+  const CurFinArgs f = curfin_args(c, vec_Ac, actab, itt, curr_dev, tmax);
   const int M = c->no * c->nk;
   for (int is = 0; is < c->nspin; ++is) {
     hipLaunchKernelGGL(k_current_orb, dim3((unsigned)((M + 3) / 4)), dim3(kBlock), 0, c->stream, f, is);
@@ -3127,12 +3212,24 @@ int salmon_hip_current(salmon_hip_t c, const double *vec_Ac, const void *psi, do
   return current_impl(c, vec_Ac, nullptr, nullptr, psi, curr_dev);
 }
 
+static DipArgs dip_args(salmon_hip_t c, const double *rho_dev, double *out_dev) {
+  DipArgs a;
+  a.rho = rho_dev;
+  a.part = c->d_dippart;
+  a.out = out_dev;
+  a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.nspin = c->nspin;
+  a.hx = c->hgs[0]; a.hy = c->hgs[1]; a.hz = c->hgs[2]; a.hvol = c->hvol;
+  return a;
+}
+
 int salmon_hip_dipole(salmon_hip_t c, const double *rho_dev, double *out_dev) {
   if (int rc = check(c)) return rc;
   if (!rho_dev || !out_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
-  hipLaunchKernelGGL(k_dipole, dim3(1), dim3(1024), 0, c->stream, rho_dev, out_dev, c->nx, c->ny, c->nz,
-                     c->nspin, c->hgs[0], c->hgs[1], c->hgs[2], c->hvol);
-  LAUNCHCHK("k_dipole");
+  const DipArgs a = dip_args(c, rho_dev, out_dev);
+  hipLaunchKernelGGL(k_dipole_part, dim3(kDipBlocks), dim3(256), 0, c->stream, a);
+  LAUNCHCHK("k_dipole_part");
+  hipLaunchKernelGGL(k_dipole_sum, dim3(1), dim3(256), 0, c->stream, a);
+  LAUNCHCHK("k_dipole_sum");
   return 0;
 }
 
@@ -3319,6 +3416,112 @@ int launch_packed(hipStream_t st, int hc, int nc, const StencilArgs *sys, const 
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Packed launches of the batched step: one launch runs a kernel body for every system of the
+// batch.  The systems' argument records sit in a device array and system s owns the blocks
+// [off[s], off[s+1]) of the launch; a block finds its system by binary search over off (block-
+// uniform, scalar loads), copies the record and runs the body with its local block index.  The
+// ranges of the bodies that order their blocks for the XCDs (projections, fused KB pass) are
+// padded to multiples of 8 blocks, so a local block index keeps its XCD (block b runs on XCD
+// b % 8); padding blocks return.  The per-system kernels and these run the same bodies on the
+// same records, so a packed step equals the per-system step bit for bit.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int pack_find(const int *off, int nsys, int b) {
+  int lo = 0, hi = nsys;  // off[lo] <= b < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void kp_update_kvector(const KvecArgs *args, const int *off, int nsys) {
+  const int s = pack_find(off, nsys, blockIdx.x);
+  const KvecArgs a = args[s];
+  const int lb = (int)blockIdx.x - off[s], gx = max(1, a.nlma);
+  update_kvector_body(a, lb % gx, lb / gx);
+}
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void kp_proj(const ProjArgs *args, const int *off, int nsys) {
+  const int s = pack_find(off, nsys, blockIdx.x);
+  const ProjArgs a = args[s];
+  const int item = xcd_order((int)blockIdx.x - off[s], off[s + 1] - off[s]);
+  if (item >= a.ngrp * a.natom * a.nk) return;
+  proj_body<NW, double2>(a, item);
+}
+
+template <int LM>
+__global__ __launch_bounds__(kBlock) void kp_proj_scatter(const ProjScatArgs *args, const int *off, int nsys) {
+  const int s = pack_find(off, nsys, blockIdx.x);
+  const ProjScatArgs A = args[s];
+  const int item = xcd_order((int)blockIdx.x - off[s], off[s + 1] - off[s]);
+  if (item >= A.p.ngrp * A.p.natom * A.p.nk) return;
+  proj_scatter_body<double2, LM>(A, item);
+}
+
+template <int CM>
+__global__ __launch_bounds__(kBlock) void kp_nl_scatter(const ScatterArgs *args, const int *off, int nsys) {
+  const int s = pack_find(off, nsys, blockIdx.x);
+  const ScatterArgs a = args[s];
+  const int lb = (int)blockIdx.x - off[s], gx = (a.ntp + kBlock - 1) / kBlock;
+  nl_scatter_body<double2, kScatG, CM>(a, lb % gx, lb / gx);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_density_part(const DensArgs *args, const int *off, int nsys) {
+  const int s = pack_find(off, nsys, blockIdx.x);
+  const DensArgs a = args[s];
+  const int lb = (int)blockIdx.x - off[s];
+  const int gx = (a.nx * a.ny * a.nz + kBlock - 1) / kBlock;
+  density_part_body(a, lb % gx, (lb / gx) % a.ngrp, lb / (gx * a.ngrp));
+}
+
+__global__ __launch_bounds__(kBlock) void kp_density_sum(const DensArgs *args, const int *off, int nsys) {
+  const int s = pack_find(off, nsys, blockIdx.x);
+  const DensArgs a = args[s];
+  density_sum_body(a, (int)blockIdx.x - off[s]);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_current_local(const CurArgs *args, const int *off, int nsys) {
+  const int s = pack_find(off, nsys, blockIdx.x);
+  const CurArgs a = args[s];
+  current_local_body(a, (int)blockIdx.x - off[s]);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_current_orb(const CurFinArgs *args, const int *off, int nsys) {
+  const int s = pack_find(off, nsys, blockIdx.x);
+  const CurFinArgs a = args[s];
+  const int lb = (int)blockIdx.x - off[s], gx = (a.no * a.nk + 3) / 4;
+  current_orb_body(a, lb % gx, lb / gx);
+}
+
+__global__ __launch_bounds__(kBlock) void kp_current_sum(const CurFinArgs *args, const int *off, int nsys) {
+  const int s = pack_find(off, nsys, blockIdx.x);
+  const CurFinArgs a = args[s];
+  current_sum_body(a, (int)blockIdx.x - off[s]);
+}
+
+__global__ __launch_bounds__(256) void kp_dipole_part(const DipArgs *args, const int *off, int nsys) {
+  const int s = pack_find(off, nsys, blockIdx.x);
+  const DipArgs a = args[s];
+  dipole_part_body(a, (int)blockIdx.x - off[s]);
+}
+
+__global__ __launch_bounds__(256) void kp_dipole_sum(const DipArgs *args, const int *off, int nsys) {
+  const int s = pack_find(off, nsys, blockIdx.x);
+  const DipArgs a = args[s];
+  dipole_sum_body(a);
+}
+
+// one packed launch: device records, block offsets, grid, dynamic LDS
+struct Packed {
+  void *args = nullptr;
+  int *off = nullptr;
+  int nblk = 0;
+  size_t shmem = 0;
+};
+
 struct salmon_hip_batch {
   std::vector<salmon_hip_t> h;
   hipStream_t stream = nullptr;
@@ -3358,6 +3561,13 @@ struct salmon_hip_batch {
   std::vector<hipStream_t> lanes;
   std::vector<hipEvent_t> lane_ev;
   hipEvent_t fork_ev = nullptr;
+  // packed launches of the per-system phases ([parity] = which orbital buffer is the input;
+  // [term] for the fused KB passes); used when every system with projectors runs the fused
+  // KB pass, else the step forks per-system launches onto the lanes
+  bool packed = false;
+  int lm = 4, cm = 0;                     // template choices common to the systems
+  Packed p_kvec, p_proj[2], p_kbsh[2][kMaxComb], p_kbf[2][kMaxComb], p_last[2];
+  Packed p_dpart[2], p_dsum[2], p_proj4[2], p_curl[2], p_curo, p_curs, p_dipp, p_dips;
 };
 
 namespace {
@@ -3500,6 +3710,168 @@ int batch_step(salmon_hip_batch *b, int p) {
   return 0;
 }
 
+template <typename A>
+int pack_make(salmon_hip_batch *b, Packed &P, const std::vector<A> &recs, const std::vector<int> &cnt, bool pad8,
+              size_t shmem = 0) {
+  const int n = (int)recs.size();
+  std::vector<int> off(n + 1, 0);
+  for (int i = 0; i < n; ++i) off[i + 1] = off[i] + (pad8 ? (cnt[i] + 7) / 8 * 8 : cnt[i]);
+  A *da = nullptr;
+  int *doff = nullptr;
+  if (int rc = balloc(b, &da, (size_t)n)) return rc;
+  if (int rc = balloc(b, &doff, (size_t)n + 1)) return rc;
+  HIPCHK(hipMemcpy(da, recs.data(), sizeof(A) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(doff, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  P.args = da;
+  P.off = doff;
+  P.nblk = off[n];
+  P.shmem = shmem;
+  return 0;
+}
+
+// Records of the packed launches for both buffer parities.  The two c buffers of each handle
+// alternate over the Taylor terms exactly as in launch_kb_fused: the first projection writes X,
+// the fused pass of term n reads (n odd ? X : Y) and writes the other, the last scatter reads
+// what the pass of term N-1 wrote.
+int batch_packed_tables(salmon_hip_batch *b) {
+  b->packed = false;
+  const int ns = (int)b->h.size(), N = b->n_hamil;
+  int cmax = 0;
+  for (int i = 0; i < ns; ++i) {
+    salmon_hip_t c = b->h[i];
+    if (c->natom > 0 && c->nlma > 0 && !kb_fused_on(c)) return 0;  // per-system launches on the lanes
+    cmax = std::max(cmax, c->natom > 0 ? c->cmax : 0);
+  }
+  if (cmax > kMaxLmaPerAtom) return 0;
+  b->lm = cmax <= 4 ? 4 : cmax <= 9 ? 9 : 16;
+  b->cm = cmax <= kScatCM ? kScatCM : 0;
+  auto kb = [&](salmon_hip_t c) { return kb_fused_on(c); };
+  auto scat_blocks = [&](salmon_hip_t c, int ntp) {
+    return ntp > 0 ? ((ntp + kBlock - 1) / kBlock) * c->nk * ((c->nb + kScatG - 1) / kScatG) : 0;
+  };
+  auto proj_blocks = [&](salmon_hip_t c) { return ((c->nb + kProjOrb - 1) / kProjOrb) * c->natom * c->nk; };
+  std::vector<int> cnt(ns);
+  {
+    std::vector<KvecArgs> r(ns);
+    for (int i = 0; i < ns; ++i) {
+      r[i] = kvec_args(b->h[i], nullptr, b->sys[i].actab, b->d_itt, b->sys[i].nt + 1);
+      cnt[i] = std::max(1, b->h[i]->nlma) * b->h[i]->nk;
+    }
+    if (int rc = pack_make(b, b->p_kvec, r, cnt, false)) return rc;
+  }
+  size_t kb_shm = 0, cur_shm = 0;
+  for (int i = 0; i < ns; ++i) {
+    kb_shm = std::max(kb_shm, sizeof(double2) * (size_t)b->h[i]->cmax * kProjOrb);
+    cur_shm = std::max(cur_shm, cur_shmem(b->h[i]));
+  }
+  for (int p = 0; p < 2; ++p) {
+    std::vector<ProjArgs> pr(ns), p4(ns);
+    std::vector<ScatterArgs> last(ns);
+    std::vector<DensArgs> de(ns);
+    std::vector<CurArgs> cl(ns);
+    std::vector<int> c4(ns), cl_n(ns), cd(ns), cs(ns), cls(ns);
+    for (int i = 0; i < ns; ++i) {
+      salmon_hip_t c = b->h[i];
+      void *hh[kMaxComb + 1], *out;
+      batch_buffers(b, i, p, hh, &out);
+      pr[i] = proj_args(c, hh[0], c->d_c);
+      cnt[i] = kb(c) ? proj_blocks(c) : 0;
+      const double2 *cl_buf = ((N - 1) % 2 == 1) ? c->d_c2 : c->d_c;
+      last[i] = scatter_args(c, out, make_double2(b->zc[2 * (N - 1)], b->zc[2 * (N - 1) + 1]), cl_buf, false, false);
+      cls[i] = kb(c) ? scat_blocks(c, c->ntp) : 0;
+      de[i] = dens_args(c, out, b->sys[i].rho, c->d_denspart);
+      const int gx = (int)((c->ngrid + kBlock - 1) / kBlock);
+      cd[i] = gx * c->dens_ngrp * c->nspin;
+      cs[i] = c->dens_ngrp > 1 ? (int)((c->ngrid * c->nspin + kBlock - 1) / kBlock) : 0;
+      p4[i] = proj_args(c, out, c->d_curnl);
+      c4[i] = c->natom > 0 ? proj_blocks(c) : 0;
+      cl[i] = cur_args(c, out);
+      cl_n[i] = c->norb * c->nyt;
+    }
+    if (int rc = pack_make(b, b->p_proj[p], pr, cnt, true)) return rc;
+    if (int rc = pack_make(b, b->p_last[p], last, cls, false)) return rc;
+    if (int rc = pack_make(b, b->p_dpart[p], de, cd, false)) return rc;
+    if (int rc = pack_make(b, b->p_dsum[p], de, cs, false)) return rc;
+    if (int rc = pack_make(b, b->p_proj4[p], p4, c4, true)) return rc;
+    if (int rc = pack_make(b, b->p_curl[p], cl, cl_n, false, cur_shm)) return rc;
+    for (int n = 1; n < N; ++n) {
+      std::vector<ScatterArgs> sh(ns);
+      std::vector<ProjScatArgs> fu(ns);
+      std::vector<int> shn(ns), fun(ns);
+      for (int i = 0; i < ns; ++i) {
+        salmon_hip_t c = b->h[i];
+        void *hh[kMaxComb + 1], *out;
+        batch_buffers(b, i, p, hh, &out);
+        const double2 *cin = (n % 2 == 1) ? c->d_c : c->d_c2;
+        double2 *cout = (n % 2 == 1) ? c->d_c2 : c->d_c;
+        sh[i] = scatter_args(c, hh[n], make_double2(1.0, 0.0), cin, true, false);
+        shn[i] = kb(c) ? scat_blocks(c, c->ntp_shared) : 0;
+        fu[i] = projscat_args(c, hh[n], cin, cout);
+        fun[i] = kb(c) ? proj_blocks(c) : 0;
+      }
+      if (int rc = pack_make(b, b->p_kbsh[p][n - 1], sh, shn, false)) return rc;
+      if (int rc = pack_make(b, b->p_kbf[p][n - 1], fu, fun, true, kb_shm)) return rc;
+    }
+  }
+  std::vector<CurFinArgs> cf(ns);
+  std::vector<DipArgs> dp(ns);
+  std::vector<int> co(ns), csum(ns), dpn(ns, kDipBlocks), dsn(ns, 1);
+  for (int i = 0; i < ns; ++i) {
+    salmon_hip_t c = b->h[i];
+    cf[i] = curfin_args(c, nullptr, b->sys[i].actab, b->d_itt, b->sys[i].curr_s, b->sys[i].nt + 1);
+    co[i] = ((c->no * c->nk + 3) / 4) * c->nspin;
+    csum[i] = c->nspin;
+    dp[i] = dip_args(c, b->sys[i].rho, b->sys[i].dip_s);
+  }
+  if (int rc = pack_make(b, b->p_curo, cf, co, false)) return rc;
+  if (int rc = pack_make(b, b->p_curs, cf, csum, false)) return rc;
+  if (int rc = pack_make(b, b->p_dipp, dp, dpn, false)) return rc;
+  if (int rc = pack_make(b, b->p_dips, dp, dsn, false)) return rc;
+  b->packed = true;
+  return 0;
+}
+
+#define PACKED_LAUNCH(kern, P, A)                                                                      \
+  do {                                                                                                 \
+    if ((P).nblk > 0) {                                                                                \
+      hipLaunchKernelGGL(kern, dim3((unsigned)(P).nblk), dim3(kBlock), (P).shmem, b->stream,          \
+                         (const A *)(P).args, (const int *)(P).off, ns);                               \
+      LAUNCHCHK(#kern);                                                                                \
+    }                                                                                                  \
+  } while (0)
+
+// One step of every system with one launch per phase (the packed stencil per halo class and term).
+int batch_step_packed(salmon_hip_batch *b, int p) {
+  const int N = b->n_hamil;
+  const int ns = (int)b->h.size();
+  PACKED_LAUNCH(kp_update_kvector, b->p_kvec, KvecArgs);
+  PACKED_LAUNCH(kp_proj<1>, b->p_proj[p], ProjArgs);
+  for (int n = 1; n <= N; ++n) {
+    if (int rc = batch_stencil(b, p, n)) return rc;
+    const Packed &S = n < N ? b->p_kbsh[p][n - 1] : b->p_last[p];
+    if (b->cm == kScatCM) PACKED_LAUNCH(kp_nl_scatter<kScatCM>, S, ScatterArgs);
+    else PACKED_LAUNCH(kp_nl_scatter<0>, S, ScatterArgs);
+    if (n < N) {
+      const Packed &F = b->p_kbf[p][n - 1];
+      if (b->lm == 4) PACKED_LAUNCH(kp_proj_scatter<4>, F, ProjScatArgs);
+      else if (b->lm == 9) PACKED_LAUNCH(kp_proj_scatter<9>, F, ProjScatArgs);
+      else PACKED_LAUNCH(kp_proj_scatter<16>, F, ProjScatArgs);
+    }
+  }
+  PACKED_LAUNCH(kp_density_part, b->p_dpart[p], DensArgs);
+  PACKED_LAUNCH(kp_density_sum, b->p_dsum[p], DensArgs);
+  PACKED_LAUNCH(kp_proj<4>, b->p_proj4[p], ProjArgs);
+  PACKED_LAUNCH(kp_current_local, b->p_curl[p], CurArgs);
+  PACKED_LAUNCH(kp_current_orb, b->p_curo, CurFinArgs);
+  PACKED_LAUNCH(kp_current_sum, b->p_curs, CurFinArgs);
+  PACKED_LAUNCH(kp_dipole_part, b->p_dipp, DipArgs);
+  PACKED_LAUNCH(kp_dipole_sum, b->p_dips, DipArgs);
+  hipLaunchKernelGGL(k_record, dim3(1), dim3(std::max<size_t>(64, (ns + 63) / 64 * 64)), 0, b->stream, b->d_csrc,
+                     b->d_cdst, b->d_cw, b->d_dsrc, b->d_ddst, b->d_nt, ns, b->d_itt);
+  LAUNCHCHK("k_record");
+  return 0;
+}
+
 int bcheck(salmon_hip_batch_t b) {
   if (!b) return fail(SALMON_HIP_EINVAL, "null batch");
   return 0;
@@ -3618,6 +3990,7 @@ int salmon_hip_batch_prepare(salmon_hip_batch_t b, const double *zc, int n_hamil
     g->mem.push_back(i);
   }
   if (int rc = batch_tables(b)) return rc;
+  if (int rc = batch_packed_tables(b)) return rc;
   std::vector<const double *> cs(ns), ds(ns);
   std::vector<double *> cd(ns), dd(ns);
   std::vector<int> cw(ns), nt(ns);
@@ -3655,7 +4028,7 @@ int salmon_hip_batch_step(salmon_hip_batch_t b) {
   if (!b->prepared) return fail(SALMON_HIP_ESTATE, "batch_prepare not called");
   if (b->itt > b->nt_min)
     return fail(SALMON_HIP_EINVAL, "step %d past the field tables (%d steps)", b->itt, b->nt_min);
-  if (int rc = batch_step(b, b->parity)) return rc;
+  if (int rc = b->packed ? batch_step_packed(b, b->parity) : batch_step(b, b->parity)) return rc;
   b->parity ^= 1;
   b->itt += 1;
   return 0;
