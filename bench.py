@@ -361,8 +361,9 @@ def run_batch(steps, warmup):
         "one_by_one_steps_per_s": seq_steps / tseq,
         "speedup_vs_one_by_one": (steps / (t1 - t0)) / (seq_steps / tseq),
         "autotune_choice": choice,
-        "form": "fixed-potential FP64 step; one persistent stencil kernel per halo class and Taylor term, "
-                "per-system kernels on side streams; eager launches (graph capture switched off)",
+        "form": "fixed-potential FP64 step; one persistent stencil kernel per halo class and Taylor term, one "
+                "packed launch per phase for all systems (phase table, projections, fused KB passes, scatter, "
+                "density, current, dipole); eager launches (graph capture switched off)",
     }
 
 
@@ -384,7 +385,8 @@ def main():
                     help="after the run, copy 1 GiB device-to-device (HBM-traffic counter calibration)")
     ap.add_argument("--trace", action="store_true",
                     help="for profilers: only --warmup + --steps steps of each form (no accuracy runs, roofline, "
-                         "batch or CPU baseline) and --calib-copy; prints one line per form")
+                         "batch or CPU baseline) and --calib-copy; prints one line per form; with --forms batch, "
+                         "--batch-steps steps of the batched form")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -401,6 +403,8 @@ def main():
     s = system.config(args.config, host_orbitals=not big)
     psi = system.device_orbitals(s, "cuda") if big else None
     forms = [f for f in (args.forms.split(",") if args.forms else DEFAULT_FORMS.get(args.config, list(FORMS))) if f]
+    if args.trace and args.forms == "batch":
+        forms = []
     predictor, pred_note = (None, None)
     if any(FORMS[f].get("learned") for f in forms):
         predictor, pred_note = load_predictor(args.config)
@@ -408,6 +412,10 @@ def main():
             log(f"[bench] {pred_note}: learned forms skipped")
             forms = [f for f in forms if not FORMS[f].get("learned")]
     if args.trace:
+        if args.batch_steps > 0 and args.config == "c2" and args.forms == "batch":
+            r = run_batch(args.batch_steps, 2)
+            print(f"[trace] batch: {r['batched_steps_per_s']:.2f} batched steps/s", flush=True)
+            return
         for f in forms:
             el = timed_run(s, psi, f, args.steps, args.warmup, predictor)
             print(f"[trace] {args.config} {f}: {args.steps} steps, {1e3 * el / args.steps:.3f} ms/step", flush=True)

@@ -275,7 +275,11 @@ int salmon_hip_update_potential(salmon_hip_t h, const double *rho_dev, const dou
  * (update_kvector, taylor, calc_density, calc_current, calc_dip of
  * src/rt/time_evolution_step.f90) for all of them together: the stencil passes of
  * all systems of one halo class run as ONE persistent kernel over a combined tile
- * list, A(t) is read from per-system DEVICE tables (3, 0:nt+1) at a device step
+ * list, every other phase (phase table, projections, fused KB passes, scatter,
+ * density, current, dipole) runs as ONE launch for all systems (per-system argument
+ * records and block offsets on the device; when a system with projectors cannot run
+ * the fused KB pass, those phases fall back to per-system launches forked onto four
+ * side streams), A(t) is read from per-system DEVICE tables (3, 0:nt+1) at a device step
  * counter, and the per-step current curr(3*nspin) and dipole/electron count (4) are
  * written into per-system DEVICE series at row itt-1, so that a captured step can be
  * replayed.  FP64 only.  All handles are moved onto `stream` (NULL: the first
@@ -283,8 +287,8 @@ int salmon_hip_update_potential(salmon_hip_t h, const double *rho_dev, const dou
  * its handles must not overlap.
  *   batch_set_system: buffers of system i (psi_a holds the initial orbitals; the two
  *     orbital buffers alternate by step), Ac_tot table, series of nt rows.
- *   batch_prepare: Taylor coefficients; builds the tile lists and descriptor tables,
- *     sets the step counter to 1.
+ *   batch_prepare: Taylor coefficients; builds the tile lists, descriptor tables and
+ *     the packed launches' records, sets the step counter to 1.
  *   batch_autotune: times the packed stencil for each candidate (z-chunks per orbital:
  *     the systems' rule, 1 or 2; persistent blocks: 256, 512, 1024, 2048 or one per
  *     tile) and keeps the fastest; *choice = 100*z + blocks/256 (99 = one per tile).

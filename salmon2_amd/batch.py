@@ -2,8 +2,9 @@
 
 Each system has its own Handle (grid, orbitals, projectors, fixed V_local) and its
 own pulse; the batch runs the fixed-potential RT step for all of them with one
-persistent stencil kernel per halo class and per Taylor term, the per-system
-kernels on side streams, and A(t) tables and the step counter on the device.
+persistent stencil kernel per halo class and per Taylor term, one packed launch
+per phase for all systems (phase table, projections, fused KB passes, scatter,
+density, current, dipole), and A(t) tables and the step counter on the device.
 Launches are eager: the library's graph capture of the batched step is switched
 off (salmon_hip_batch_capture reports it).  torch provides the device buffers and
 the stream.
