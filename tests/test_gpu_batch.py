@@ -120,6 +120,25 @@ def test_batch_24_seeded_systems(torch_dev, oracle_lib):
     torch.cuda.empty_cache()
 
 
+def test_batch_rejects_steps_past_the_field_tables(torch_dev):
+    """The device field tables end at nt: a step past them is refused on the host (no launch)."""
+    from salmon2_amd.batch import BatchRunner
+    from salmon2_amd.hip import SalmonHipError
+
+    nt = 3
+    br = BatchRunner([system.config("tiny", nt=nt), system.config("c1", nt=nt, no=8)], nt)
+    br.run(nt)
+    with pytest.raises(SalmonHipError, match="past the field tables"):
+        br.run(1)
+    with pytest.raises(SalmonHipError, match="outside"):
+        br.batch.set_time(nt + 2)
+    br.reset()
+    br.run(nt)
+    got = br.results(nt)
+    br.close()
+    assert all(np.all(np.isfinite(r["curr"])) for r in got)
+
+
 def test_batch_capture_reports_switched_off(torch_dev):
     from salmon2_amd.batch import BatchRunner
     from salmon2_amd.hip import SalmonHipError
