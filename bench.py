@@ -487,7 +487,8 @@ def main():
     nl = k1["launches"] + km["launches"]
     avg_ms = (k1["avg_ms"] * k1["launches"] + km["avg_ms"] * km["launches"]) / max(nl, 1)
     achieved = alg_bytes / (avg_ms / 1e3) / 1e9
-    kname = "k_stencilz<d2,double,d2>" if s.nx <= 32 else "k_stencil<d2,double,d2,2,-1,256>"
+    ty = min(s.ny, 256 // s.nx)
+    kname = f"k_stencilz<d2,double,d2,{2 if (ty + 8) * s.nx <= 512 else 3}>"
     ceil = copy_ceiling()
     out = {
         "metric": METRIC,

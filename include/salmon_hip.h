@@ -221,6 +221,21 @@ int salmon_hip_density_noalloc(salmon_hip_t h, const void *psi, double *rho_dev)
 int salmon_hip_current(salmon_hip_t h, const double *vec_Ac, const void *psi, double *curr_dev);
 
 /*
+ * salmon_hip_current, and with the same gather of psi the projection of psi under the phase
+ * table of the field vec_Ac_next, which is built here ahead of time (no counterpart in the
+ * reference: in time_evolution_step the current of step itt (:243-246) and the phase table
+ * and first projection of step itt+1 (:126, taylor -> hpsi -> zpseudo) read the same
+ * orbitals twice).  The next salmon_hip_update_kvector whose field equals vec_Ac_next bit
+ * for bit takes that table instead of building it, and the next salmon_hip_taylor /
+ * salmon_hip_hpsi of psi takes that projection instead of forming it; any other
+ * update_kvector, projection or fused KB pass on the handle discards them.  psi must not
+ * change in between.  Results equal the unfused sequence bit for bit.  Not for handles of
+ * a batch (SALMON_HIP_ESTATE).
+ */
+int salmon_hip_current_next(salmon_hip_t h, const double *vec_Ac, const void *psi, double *curr_dev,
+                            const double *vec_Ac_next);
+
+/*
  * Electronic dipole and electron count from rho (DEVICE arrays): out[0:3] =
  * -sum(pos_d * rho) * Hgs_d * Hvol, out[3] = sum(rho) * Hvol (calc_dip and
  * subdip, src/rt/dip.f90:23-112,114-225).

@@ -210,10 +210,14 @@ struct ProjArgs {
   const int *mps;         // (natom)
   const int2 *atom_lma;   // (natom): first ilma, count
   const double *rinv;     // (nlma)
-  double2 *out;           // NW=1: c(nb, nlma, nk) scaled by rinv; NW=4: (4, nlma, nb, nk) raw
+  double2 *out;           // NW=1: c(nb, nlma, nk) scaled by rinv; NW=4, 5: (4, nlma, nb, nk) raw
   size_t osz;
   int nb, nk, natom, nps, nlma, ngrp;
   const int *skip;        // device flag: the launch does nothing when it is set (null: never)
+  // NW=5: weight 4 is the plain projection under a second phase table (the next step's), written
+  // as c(nb, nlma, nk) scaled by rinv
+  const double2 *zekr2;
+  double2 *out2;
 };
 
 // XCD-aware order of a launch of nblk blocks (block b runs on XCD b % 8): the blocks of one XCD
@@ -224,11 +228,14 @@ __device__ __forceinline__ int xcd_order(int b, int nblk) {
   return b < nfull ? (b % 8) * (nblk / 8) + b / 8 : b;
 }
 
-template <int NW, typename TP>
+// LMX bounds the projectors of one atom (the tile and staging sizes)
+template <int NW, typename TP, int LMX>
 __device__ __forceinline__ void proj_body(const ProjArgs &a, int bid) {
-  constexpr int MAXT = (NW * kMaxLmaPerAtom + 7) / 8;
+  static_assert(LMX <= kMaxLmaPerAtom, "LMX bounds the projectors of one atom");
+  constexpr int MAXT = (NW * LMX + 7) / 8;
   __shared__ double2 sp[kProjJC][kProjOrb];
-  __shared__ double2 zsh[kProjJC][kMaxLmaPerAtom];
+  __shared__ double2 zsh[kProjJC][LMX];
+  __shared__ double2 zsh2[NW == 5 ? kProjJC : 1][LMX];
   __shared__ double rsh[kProjJC][3];
   const int grp = bid % a.ngrp;
   const int t = bid / a.ngrp;
@@ -254,13 +261,14 @@ __device__ __forceinline__ void proj_body(const ProjArgs &a, int bid) {
   // psi at kProjJC sphere points for the block's 64 orbitals: chunk j0 + kProjJC is loaded into
   // registers while the MFMAs consume chunk j0 from LDS
   constexpr int kPer = (kProjJC * kProjOrb) / kBlock;
-  constexpr int kZPer = (kProjJC * kMaxLmaPerAtom + kBlock - 1) / kBlock;
+  constexpr int kZPer = (kProjJC * LMX + kBlock - 1) / kBlock;
   TP pre[kPer];
-  double2 zpre[kZPer];
+  double2 zpre[kZPer], zpre2[NW == 5 ? kZPer : 1];
   double rpre = 0.0;
   const TP *psi = (const TP *)a.psi;
   const int *jx = a.jidx + (size_t)a.nps * ia;
   const double2 *zk = a.zekr + (size_t)a.nps * ((size_t)al.x + (size_t)a.nlma * ik);
+  const double2 *zk2 = NW == 5 ? a.zekr2 + (size_t)a.nps * ((size_t)al.x + (size_t)a.nlma * ik) : nullptr;
   // every thread loads the same sphere point of each chunk (kBlock is a multiple of kProjJC);
   // its grid index is fetched one chunk ahead, so the psi gathers wait on one load, not two
   static_assert(kBlock % kProjJC == 0, "one sphere point per thread and chunk");
@@ -275,6 +283,8 @@ __device__ __forceinline__ void proj_body(const ProjArgs &a, int bid) {
       const int jj = e % kProjJC, l = e / kProjJC;
       const int j = j0 + jj;
       zpre[r] = (j < m && l < al.y) ? zk[(size_t)j + (size_t)a.nps * l] : make_double2(0.0, 0.0);
+      if constexpr (NW == 5)
+        zpre2[r] = (j < m && l < al.y) ? zk2[(size_t)j + (size_t)a.nps * l] : make_double2(0.0, 0.0);
     }
     if (NW > 1 && threadIdx.x < 3 * kProjJC) {
       const int j = j0 + threadIdx.x / 3;
@@ -304,7 +314,9 @@ __device__ __forceinline__ void proj_body(const ProjArgs &a, int bid) {
 #pragma unroll
     for (int r = 0; r < kZPer; ++r) {
       const int e = threadIdx.x + kBlock * r;
-      if (e < kProjJC * kMaxLmaPerAtom) zsh[e % kProjJC][e / kProjJC] = zpre[r];
+      if (e < kProjJC * LMX) zsh[e % kProjJC][e / kProjJC] = zpre[r];
+      if constexpr (NW == 5)
+        if (e < kProjJC * LMX) zsh2[e % kProjJC][e / kProjJC] = zpre2[r];
     }
     if (NW > 1 && threadIdx.x < 3 * kProjJC) rsh[threadIdx.x / 3][threadIdx.x % 3] = rpre;
     __syncthreads();
@@ -331,8 +343,8 @@ __device__ __forceinline__ void proj_body(const ProjArgs &a, int bid) {
           if (jv && lw < nlw) {
             const int w = lw % NW;
             const int l = lw / NW;
-            const double2 z = zsh[jp][l];
-            const double r = (w == 0) ? 1.0 : rw[w - 1];
+            const double2 z = (NW == 5 && w == 4) ? zsh2[jp][l] : zsh[jp][l];
+            const double r = (w == 0 || w == 4) ? 1.0 : rw[w - 1];
             zr = z.x * r;
             zi = z.y * r;
             aval = part == 0 ? (comp ? zi : zr) : (comp ? zr : -zi);
@@ -360,6 +372,9 @@ __device__ __forceinline__ void proj_body(const ProjArgs &a, int bid) {
       if (NW == 1) {
         const double s = a.rinv[ilma];
         a.out[((size_t)ik * a.nlma + ilma) * a.nb + ob] = make_double2(re * s, im * s);
+      } else if (NW == 5 && w == 4) {
+        const double s = a.rinv[ilma];
+        a.out2[((size_t)ik * a.nlma + ilma) * a.nb + ob] = make_double2(re * s, im * s);
       } else {
         a.out[((kbase + ob) * a.nlma + ilma) * 4 + w] = make_double2(re, im);
       }
@@ -367,10 +382,10 @@ __device__ __forceinline__ void proj_body(const ProjArgs &a, int bid) {
   }
 }
 
-template <int NW, typename TP>
+template <int NW, typename TP, int LMX>
 __global__ __launch_bounds__(kBlock) void k_proj(ProjArgs a) {
   if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
-  proj_body<NW, TP>(a, xcd_order(blockIdx.x, gridDim.x));
+  proj_body<NW, TP, LMX>(a, xcd_order(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -1070,14 +1085,14 @@ __device__ __forceinline__ void stencil_plane(const StencilArgs &a, int bid, dou
 // ---------------------------------------------------------------------------
 template <int P> struct Ph { static constexpr int v = P; };
 
-template <typename TI, typename R, typename TO>
+template <typename TI, typename R, typename TO, int NCH>
 __device__ __forceinline__ void stencil_tile8(const StencilArgs &a, int bid, double2 *lds) {
   using CH = typename Cx<R>::T;
   constexpr int BS = kTile;
   constexpr int WI = DmaW<TI>::W;
   constexpr unsigned EB = sizeof(TI);
-  constexpr int SLOT = 2 * BS;                // elements per slot: (TY + 8) * nx <= 512
-  constexpr int kOps = 2 * WI + 2 + 1;
+  constexpr int SLOT = NCH * BS;              // elements per slot: (TY + 8) * nx <= NCH * 256
+  constexpr int kOps = NCH * WI + 2 + 1;
   TI *tileR = (TI *)lds;                      // 8 x SLOT
   double *vR = (double *)(tileR + 8 * SLOT);  // 4 x BS
 
@@ -1110,9 +1125,9 @@ __device__ __forceinline__ void stencil_tile8(const StencilArgs &a, int bid, dou
     const int ec = e < nslot ? e : 0;
     return (size_t)(ec % nx) + (size_t)nx * wrapi(y0 - 4 + ec / nx, ny);
   };
-  size_t toff[2][2];
+  size_t toff[NCH][2];
 #pragma unroll
-  for (int k = 0; k < 2; ++k)
+  for (int k = 0; k < NCH; ++k)
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk)
       toff[k][kk] = sofs(64 * (wave + 4 * k) + (WI == 1 ? lane : 32 * kk + (lane >> 1)));
@@ -1129,7 +1144,7 @@ __device__ __forceinline__ void stencil_tile8(const StencilArgs &a, int bid, dou
   auto dma_tile = [&](int plane, int slot) {
     const TI *src = in + zs * wrapi(plane, nz);
 #pragma unroll
-    for (int k = 0; k < 2; ++k) dma_chunk<TI>(src, toff[k], tileA + EB * (unsigned)(SLOT * slot + 4 * 64 * k), lane);
+    for (int k = 0; k < NCH; ++k) dma_chunk<TI>(src, toff[k], tileA + EB * (unsigned)(SLOT * slot + 4 * 64 * k), lane);
   };
   auto dma_v = [&](int plane, int slot) {
     const float *vb = (const float *)Vb + 2 * gplane * wrapi(plane, nz);
@@ -1232,14 +1247,14 @@ __device__ __forceinline__ void stencil_tile8(const StencilArgs &a, int bid, dou
 // a regrouping of stencil_tile's sum.  Needs TZ % 4 == 0; the z loop runs TZ + 4 planes,
 // unrolled by 4 so that slot and queue indices are compile-time.
 // ---------------------------------------------------------------------------
-template <typename TI, typename R, typename TO>
+template <typename TI, typename R, typename TO, int NCH>
 __device__ __forceinline__ void stencil_tilez(const StencilArgs &a, int bid, double2 *lds) {
   using CH = typename Cx<R>::T;
   constexpr int BS = kTile;
   constexpr int WI = DmaW<TI>::W;
   constexpr unsigned EB = sizeof(TI);
-  constexpr int SLOT = 2 * BS;                // elements per slot: (TY + 8) * nx <= 512
-  constexpr int kOps = 2 * WI + 2 + 1;
+  constexpr int SLOT = NCH * BS;              // elements per slot: (TY + 8) * nx <= NCH * 256
+  constexpr int kOps = NCH * WI + 2 + 1;
   TI *tileR = (TI *)lds;                      // 4 x SLOT
   double *vR = (double *)(tileR + 4 * SLOT);  // 4 x BS
 
@@ -1271,9 +1286,9 @@ __device__ __forceinline__ void stencil_tilez(const StencilArgs &a, int bid, dou
     const int ec = e < nslot ? e : 0;
     return (size_t)(ec % nx) + (size_t)nx * wrapi(y0 - 4 + ec / nx, ny);
   };
-  size_t toff[2][2];
+  size_t toff[NCH][2];
 #pragma unroll
-  for (int k = 0; k < 2; ++k)
+  for (int k = 0; k < NCH; ++k)
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk)
       toff[k][kk] = sofs(64 * (wave + 4 * k) + (WI == 1 ? lane : 32 * kk + (lane >> 1)));
@@ -1290,7 +1305,7 @@ __device__ __forceinline__ void stencil_tilez(const StencilArgs &a, int bid, dou
   auto dma_tile = [&](int plane, int slot) {
     const TI *src = in + zs * wrapi(plane, nz);
 #pragma unroll
-    for (int k = 0; k < 2; ++k) dma_chunk<TI>(src, toff[k], tileA + EB * (unsigned)(SLOT * slot + 4 * 64 * k), lane);
+    for (int k = 0; k < NCH; ++k) dma_chunk<TI>(src, toff[k], tileA + EB * (unsigned)(SLOT * slot + 4 * 64 * k), lane);
   };
   auto dma_v = [&](int plane, int slot) {
     const float *vb = (const float *)Vb + 2 * gplane * wrapi(plane, nz);
@@ -1398,24 +1413,24 @@ __device__ __forceinline__ void stencil_tilez(const StencilArgs &a, int bid, dou
   wait_vm<0>();
 }
 
-template <typename TI, typename R, typename TO>
+template <typename TI, typename R, typename TO, int NCH>
 __global__ __launch_bounds__(kTile) void k_stencilz(StencilArgs a) {
   extern __shared__ double2 lds[];
   if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
   int bid = blockIdx.x;
   const int nfull = (a.nblk / 8) * 8;
   if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;  // XCD-aware tile order
-  stencil_tilez<TI, R, TO>(a, bid, lds);
+  stencil_tilez<TI, R, TO, NCH>(a, bid, lds);
 }
 
-template <typename TI, typename R, typename TO>
+template <typename TI, typename R, typename TO, int NCH>
 __global__ __launch_bounds__(kTile) void k_stencil8(StencilArgs a) {
   extern __shared__ double2 lds[];
   if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
   int bid = blockIdx.x;
   const int nfull = (a.nblk / 8) * 8;
   if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;  // XCD-aware tile order
-  stencil_tile8<TI, R, TO>(a, bid, lds);
+  stencil_tile8<TI, R, TO, NCH>(a, bid, lds);
 }
 
 template <typename TI, typename R, typename TO, int HC, int NC, int BS>
@@ -2118,6 +2133,17 @@ struct salmon_hip_ctx {
   int2 *d_atom_lma = nullptr;
   double *d_rxyz = nullptr, *d_uv = nullptr, *d_rinv = nullptr;
   double2 *d_zekr = nullptr, *d_zent = nullptr, *d_c = nullptr, *d_curnl = nullptr;
+  // salmon_hip_current_next: the next step's phase table built ahead into d_*_n (staged, for the
+  // field staged_A) and the projection of that step's orbitals pre_psi under it left in d_c;
+  // pre_state 1 = made, table not yet promoted; 2 = promoted by update_kvector(staged_A), d_c
+  // holds c of pre_psi for its first Taylor term; 0 = none
+  double2 *d_zekr_n = nullptr, *d_zent_n = nullptr;
+  double *d_kpar_n = nullptr;
+  bool staged = false;
+  double staged_A[3] = {0.0, 0.0, 0.0};
+  const void *pre_psi = nullptr;
+  int pre_state = 0;
+  bool in_batch = false;  // a batch's packed records hold this handle's table pointers
   double2 *d_c2 = nullptr;  // the fused pass writes the next term's c here; the two are then swapped
   // Taylor scratch: H^2 psi .. H^(N-2) psi (allocated on first use); FP32 chain h_1 .. h_{N-1}
   std::vector<double2 *> tscr;
@@ -2260,10 +2286,13 @@ int salmon_hip_destroy(salmon_hip_t c) {
   if (!c) return 0;
   if (c->own) (void)hipStreamSynchronize(c->own);
   if (c->stream && c->stream != c->own) (void)hipStreamSynchronize(c->stream);
-  dfree(c->d_veck); dfree(c->d_rocc); dfree(c->d_wtk); dfree(c->d_V); dfree(c->d_kpar);
+  dfree(c->d_veck); dfree(c->d_rocc); dfree(c->d_wtk); dfree(c->d_V); dfree(c->d_kpar); dfree(c->d_kpar_n);
   dfree(c->d_mps); dfree(c->d_jidx); dfree(c->d_lma_atom); dfree(c->d_pt_off);
   dfree(c->d_atom_lma); dfree(c->d_pt_lma); dfree(c->d_entof); dfree(c->d_rxyz); dfree(c->d_uv);
   dfree(c->d_rinv); dfree(c->d_zekr); dfree(c->d_zent); dfree(c->d_c); dfree(c->d_curnl);
+  dfree(c->d_zekr_n); dfree(c->d_zent_n);
+  c->staged = false;
+  c->pre_state = 0;
   dfree(c->d_c2); dfree(c->d_jexcl); dfree(c->d_tp_shared);
   dfree(c->d_tp_info); dfree(c->d_sink);
   dfree(c->d_curpart); dfree(c->d_contrib); dfree(c->d_denspart); dfree(c->d_dippart);
@@ -2297,6 +2326,9 @@ int salmon_hip_set_nonlocal(salmon_hip_t c, int natom, int nps, const int *mps, 
   dfree(c->d_mps); dfree(c->d_jidx); dfree(c->d_lma_atom); dfree(c->d_pt_off);
   dfree(c->d_atom_lma); dfree(c->d_pt_lma); dfree(c->d_entof); dfree(c->d_rxyz); dfree(c->d_uv);
   dfree(c->d_rinv); dfree(c->d_zekr); dfree(c->d_zent); dfree(c->d_c); dfree(c->d_curnl);
+  dfree(c->d_zekr_n); dfree(c->d_zent_n);
+  c->staged = false;
+  c->pre_state = 0;
   dfree(c->d_c2); dfree(c->d_jexcl); dfree(c->d_tp_shared);
   dfree(c->d_tp_info);
   c->natom = c->nps = c->nlma = c->nent = c->cmax = c->ntp = c->ntp_shared = 0;
@@ -2508,6 +2540,18 @@ static KvecArgs kvec_args(salmon_hip_t c, const double *vec_Ac, const double *ac
 
 static int update_kvector_impl(salmon_hip_t c, const double *vec_Ac, const double *actab, const int *itt,
                                int tmax = 0) {
+  if (!actab && vec_Ac && c->staged && memcmp(vec_Ac, c->staged_A, sizeof(c->staged_A)) == 0) {
+    // the table salmon_hip_current_next built ahead for this field becomes the active one
+    std::swap(c->d_zekr, c->d_zekr_n);
+    std::swap(c->d_zent, c->d_zent_n);
+    std::swap(c->d_kpar, c->d_kpar_n);
+    c->staged = false;
+    if (c->pre_state == 1) c->pre_state = 2;
+    c->have_k = true;
+    return 0;
+  }
+  c->staged = false;
+  c->pre_state = 0;
   const KvecArgs a = kvec_args(c, vec_Ac, actab, itt, tmax);
   dim3 grid(std::max(1, c->nlma), c->nk);
   hipLaunchKernelGGL(k_update_kvector, grid, dim3(kBlock), 0, c->stream, a);
@@ -2562,11 +2606,24 @@ static ProjArgs proj_args(salmon_hip_t c, const void *psi, double2 *out) {
   a.nlma = c->nlma;
   a.ngrp = (c->nb + kProjOrb - 1) / kProjOrb;
   a.skip = c->d_skip;
+  a.zekr2 = nullptr;
+  a.out2 = nullptr;
   return a;
 }
 
+// nw = 1: c of psi into d_c (skipped when salmon_hip_current_next left it there); 4: the current
+// moments into d_curnl; 5: the moments and c of psi under the staged table into d_c
 static int launch_proj(salmon_hip_t c, const void *psi, int nw, bool f32 = false) {
-  const ProjArgs a = proj_args(c, psi, nw == 1 ? c->d_c : c->d_curnl);
+  if (nw == 1) {
+    const bool pre = !f32 && c->pre_state == 2 && psi == c->pre_psi;
+    c->pre_state = 0;
+    if (pre) return 0;
+  }
+  ProjArgs a = proj_args(c, psi, nw == 1 ? c->d_c : c->d_curnl);
+  if (nw == 5) {
+    a.zekr2 = c->d_zekr_n;
+    a.out2 = c->d_c;
+  }
   dim3 grid((unsigned)(a.ngrp * c->natom * c->nk));
   hipEvent_t e0, e1;
   prof_events(c, nw == 1 ? SALMON_HIP_KERNEL_PROJ : SALMON_HIP_KERNEL_PROJ_CURRENT, &e0, &e1);
@@ -2578,12 +2635,18 @@ static int launch_proj(salmon_hip_t c, const void *psi, int nw, bool f32 = false
     HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     st = c->side;
   }
-  if (nw == 1 && f32)
-    hipExtLaunchKernelGGL((k_proj<1, float2>), grid, dim3(kBlock), 0, st, e0, e1, 0, a);
-  else if (nw == 1)
-    hipExtLaunchKernelGGL((k_proj<1, double2>), grid, dim3(kBlock), 0, st, e0, e1, 0, a);
-  else
-    hipExtLaunchKernelGGL((k_proj<4, double2>), grid, dim3(kBlock), 0, st, e0, e1, 0, a);
+  const int lmx = c->cmax <= 4 ? 4 : c->cmax <= 9 ? 9 : 16;
+#define PROJ_LAUNCH(NWv, TPv)                                                                             \
+  do {                                                                                                   \
+    if (lmx == 4) hipExtLaunchKernelGGL((k_proj<NWv, TPv, 4>), grid, dim3(kBlock), 0, st, e0, e1, 0, a);   \
+    else if (lmx == 9) hipExtLaunchKernelGGL((k_proj<NWv, TPv, 9>), grid, dim3(kBlock), 0, st, e0, e1, 0, a); \
+    else hipExtLaunchKernelGGL((k_proj<NWv, TPv, 16>), grid, dim3(kBlock), 0, st, e0, e1, 0, a);           \
+  } while (0)
+  if (nw == 1 && f32) PROJ_LAUNCH(1, float2);
+  else if (nw == 1) PROJ_LAUNCH(1, double2);
+  else if (nw == 5) PROJ_LAUNCH(5, double2);
+  else PROJ_LAUNCH(4, double2);
+#undef PROJ_LAUNCH
   LAUNCHCHK("k_proj");
   if (c->overlap) {
     HIPCHK(hipEventRecord(c->ev_join, c->side));
@@ -2660,44 +2723,59 @@ static int launch_stencil_wp(salmon_hip_ctx *c, int stage, dim3 grid, hipEvent_t
 }
 
 // y tiles with the merged halo and compile-time ring slots (stencil_tile8): the middle terms of
-// the mixed chain, rows of at most 32 points, (TY + 8) * nx <= 512, z chunks of 8k planes.
+// the mixed chain, rows of at most 64 points, (TY + 8) * nx <= 768 (slots of two or three
+// 256-element chunks), z chunks of 8k planes.
 static bool tile8_fits(const salmon_hip_ctx *c, int stage, int nc, const StencilArgs &a) {
   (void)nc;
-  return stage == kStageMixedMiddle && a.nx <= 32 && (a.TY + 8) * a.nx <= 2 * kTile && a.TY * a.nx <= kTile &&
+  return stage == kStageMixedMiddle && a.nx <= 64 && (a.TY + 8) * a.nx <= 3 * kTile && a.TY * a.nx <= kTile &&
          a.TZ % 8 == 0 && a.nz % 8 == 0 && c->tile8;
 }
 
 // y tiles with the delayed z part (stencil_tilez): the FP64 terms without the Taylor sum and the
-// mixed first term (FP64 input), rows of at most 32 points, z chunks of 4k planes.
+// mixed first term (FP64 input), z chunks of 4k planes; a plane's tile and halo rows fill a slot
+// of two 256-element chunks (rows of up to 32 points) or three (up to 64 points: 48-point rows
+// take 5 tile rows, 13 x 48 = 624 elements).
 static bool tilez_fits(const salmon_hip_ctx *c, int stage, int nc, const StencilArgs &a) {
-  return ((stage == kStagePlain && nc < 0) || stage == kStageMixedFirst) && a.nx <= 32 &&
-         (a.TY + 8) * a.nx <= 2 * kTile && a.TY * a.nx <= kTile && a.TZ % 4 == 0 && a.nz % 4 == 0 && c->tilez;
+  return ((stage == kStagePlain && nc < 0) || stage == kStageMixedFirst) && a.nx <= 64 &&
+         (a.TY + 8) * a.nx <= 3 * kTile && a.TY * a.nx <= kTile && a.TZ % 4 == 0 && a.nz % 4 == 0 && c->tilez;
+}
+
+template <typename TI, typename R, typename TO, int NCH>
+static int launch_tilez_n(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
+  const size_t shmem = sizeof(TI) * 4 * NCH * kTile + sizeof(double) * 4 * kTile;
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencilz<TI, R, TO, NCH>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)shmem));
+    attr_set = true;
+  }
+  hipExtLaunchKernelGGL((k_stencilz<TI, R, TO, NCH>), grid, dim3(kTile), shmem, c->stream, e0, e1, 0, a);
+  return 0;
 }
 
 template <typename TI, typename R, typename TO>
 static int launch_tilez_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
-  const size_t shmem = sizeof(TI) * 4 * 2 * kTile + sizeof(double) * 4 * kTile;
+  if ((a.TY + 8) * a.nx <= 2 * kTile) return launch_tilez_n<TI, R, TO, 2>(c, grid, e0, e1, a);
+  return launch_tilez_n<TI, R, TO, 3>(c, grid, e0, e1, a);
+}
+
+template <typename TI, typename R, typename TO, int NCH>
+static int launch_tile8_n(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
+  const size_t shmem = sizeof(TI) * 8 * NCH * kTile + sizeof(double) * 4 * kTile;
   static bool attr_set = false;
   if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencilz<TI, R, TO>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencil8<TI, R, TO, NCH>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)shmem));
     attr_set = true;
   }
-  hipExtLaunchKernelGGL((k_stencilz<TI, R, TO>), grid, dim3(kTile), shmem, c->stream, e0, e1, 0, a);
+  hipExtLaunchKernelGGL((k_stencil8<TI, R, TO, NCH>), grid, dim3(kTile), shmem, c->stream, e0, e1, 0, a);
   return 0;
 }
 
 template <typename TI, typename R, typename TO>
 static int launch_tile8_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
-  const size_t shmem = sizeof(TI) * 8 * 2 * kTile + sizeof(double) * 4 * kTile;
-  static bool attr_set = false;
-  if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencil8<TI, R, TO>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)shmem));
-    attr_set = true;
-  }
-  hipExtLaunchKernelGGL((k_stencil8<TI, R, TO>), grid, dim3(kTile), shmem, c->stream, e0, e1, 0, a);
-  return 0;
+  if ((a.TY + 8) * a.nx <= 2 * kTile) return launch_tile8_n<TI, R, TO, 2>(c, grid, e0, e1, a);
+  return launch_tile8_n<TI, R, TO, 3>(c, grid, e0, e1, a);
 }
 
 template <int HC>
@@ -2815,6 +2893,7 @@ static int launch_kb_fused(salmon_hip_t c, void *hout, bool f32) {
   }
   LAUNCHCHK("k_proj_scatter");
   std::swap(c->d_c, c->d_c2);
+  c->pre_state = 0;
   return 0;
 }
 
@@ -3186,11 +3265,11 @@ static CurFinArgs curfin_args(salmon_hip_t c, const double *vec_Ac, const double
 }
 
 static int current_impl(salmon_hip_t c, const double *vec_Ac, const double *actab, const int *itt, const void *psi,
-                        double *curr_dev, int tmax = 0) {
+                        double *curr_dev, int tmax = 0, bool next = false) {
   const CurArgs a = cur_args(c, psi);
   // the projector moments run on the side stream, overlapping the local part
   if (c->natom > 0)
-    if (int rc = launch_proj(c, psi, 4)) return rc;
+    if (int rc = launch_proj(c, psi, next ? 5 : 4)) return rc;
   hipLaunchKernelGGL(k_current_local, dim3((unsigned)(c->norb * c->nyt)), dim3(kBlock), cur_shmem(c), c->stream, a);
   LAUNCHCHK("k_current_local");
   if (int rc = join_proj(c)) return rc;
@@ -3220,6 +3299,34 @@ static DipArgs dip_args(salmon_hip_t c, const double *rho_dev, double *out_dev) 
   a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.nspin = c->nspin;
   a.hx = c->hgs[0]; a.hy = c->hgs[1]; a.hz = c->hgs[2]; a.hvol = c->hvol;
   return a;
+}
+
+int salmon_hip_current_next(salmon_hip_t c, const double *vec_Ac, const void *psi, double *curr_dev,
+                            const double *vec_Ac_next) {
+  if (int rc = check(c)) return rc;
+  if (!vec_Ac || !psi || !curr_dev || !vec_Ac_next) return fail(SALMON_HIP_EINVAL, "null pointer");
+  if (c->natom > 0 && !c->have_k) return fail(SALMON_HIP_ESTATE, "update_kvector not called");
+  if (c->in_batch) return fail(SALMON_HIP_ESTATE, "current_next on a handle of a batch");
+  if (c->natom == 0 || c->nlma == 0) return current_impl(c, vec_Ac, nullptr, nullptr, psi, curr_dev);
+  if (!c->d_zekr_n) {
+    int rc = dalloc(&c->d_zekr_n, (size_t)c->nps * c->nlma * c->nk);
+    if (!rc) rc = dalloc(&c->d_zent_n, (size_t)c->cmax * std::max(c->nent, 1) * c->nk);
+    if (!rc && !c->d_kpar_n) rc = dalloc(&c->d_kpar_n, 16 * (size_t)c->nk);
+    if (rc) return rc;
+  }
+  // the next step's phase table and stencil terms into the staging buffers
+  KvecArgs ka = kvec_args(c, vec_Ac_next, nullptr, nullptr, 0);
+  ka.zekr = c->d_zekr_n;
+  ka.zent = c->d_zent_n;
+  ka.kpar = c->d_kpar_n;
+  hipLaunchKernelGGL(k_update_kvector, dim3(std::max(1, c->nlma), c->nk), dim3(kBlock), 0, c->stream, ka);
+  LAUNCHCHK("k_update_kvector");
+  c->staged = true;
+  memcpy(c->staged_A, vec_Ac_next, sizeof(c->staged_A));
+  if (int rc = current_impl(c, vec_Ac, nullptr, nullptr, psi, curr_dev, 0, true)) return rc;
+  c->pre_state = 1;
+  c->pre_psi = psi;
+  return 0;
 }
 
 int salmon_hip_dipole(salmon_hip_t c, const double *rho_dev, double *out_dev) {
@@ -3443,13 +3550,13 @@ __global__ __launch_bounds__(kBlock) void kp_update_kvector(const KvecArgs *args
   update_kvector_body(a, lb % gx, lb / gx);
 }
 
-template <int NW>
+template <int NW, int LMX>
 __global__ __launch_bounds__(kBlock) void kp_proj(const ProjArgs *args, const int *off, int nsys) {
   const int s = pack_find(off, nsys, blockIdx.x);
   const ProjArgs a = args[s];
   const int item = xcd_order((int)blockIdx.x - off[s], off[s + 1] - off[s]);
   if (item >= a.ngrp * a.natom * a.nk) return;
-  proj_body<NW, double2>(a, item);
+  proj_body<NW, double2, LMX>(a, item);
 }
 
 template <int LM>
@@ -3845,7 +3952,9 @@ int batch_step_packed(salmon_hip_batch *b, int p) {
   const int N = b->n_hamil;
   const int ns = (int)b->h.size();
   PACKED_LAUNCH(kp_update_kvector, b->p_kvec, KvecArgs);
-  PACKED_LAUNCH(kp_proj<1>, b->p_proj[p], ProjArgs);
+  if (b->lm == 4) PACKED_LAUNCH((kp_proj<1, 4>), b->p_proj[p], ProjArgs);
+  else if (b->lm == 9) PACKED_LAUNCH((kp_proj<1, 9>), b->p_proj[p], ProjArgs);
+  else PACKED_LAUNCH((kp_proj<1, 16>), b->p_proj[p], ProjArgs);
   for (int n = 1; n <= N; ++n) {
     if (int rc = batch_stencil(b, p, n)) return rc;
     const Packed &S = n < N ? b->p_kbsh[p][n - 1] : b->p_last[p];
@@ -3860,7 +3969,9 @@ int batch_step_packed(salmon_hip_batch *b, int p) {
   }
   PACKED_LAUNCH(kp_density_part, b->p_dpart[p], DensArgs);
   PACKED_LAUNCH(kp_density_sum, b->p_dsum[p], DensArgs);
-  PACKED_LAUNCH(kp_proj<4>, b->p_proj4[p], ProjArgs);
+  if (b->lm == 4) PACKED_LAUNCH((kp_proj<4, 4>), b->p_proj4[p], ProjArgs);
+  else if (b->lm == 9) PACKED_LAUNCH((kp_proj<4, 9>), b->p_proj4[p], ProjArgs);
+  else PACKED_LAUNCH((kp_proj<4, 16>), b->p_proj4[p], ProjArgs);
   PACKED_LAUNCH(kp_current_local, b->p_curl[p], CurArgs);
   PACKED_LAUNCH(kp_current_orb, b->p_curo, CurFinArgs);
   PACKED_LAUNCH(kp_current_sum, b->p_curs, CurFinArgs);
@@ -3887,6 +3998,11 @@ int salmon_hip_batch_create(const salmon_hip_t *handles, int n, void *stream, sa
   salmon_hip_batch *b = new salmon_hip_batch();
   for (int i = 0; i < n; ++i)
     if (!handles[i]) { delete b; return fail(SALMON_HIP_EINVAL, "handle %d is null", i); }
+  for (int i = 0; i < n; ++i) {
+    handles[i]->in_batch = true;
+    handles[i]->staged = false;
+    handles[i]->pre_state = 0;
+  }
   // the null stream cannot be captured into a graph: NULL selects the first handle's own stream
   b->stream = stream ? (hipStream_t)stream : handles[0]->own;
   for (int i = 0; i < n; ++i) {
@@ -3937,7 +4053,10 @@ int salmon_hip_batch_destroy(salmon_hip_batch_t b) {
   for (hipStream_t st : b->lanes) (void)hipStreamDestroy(st);
   for (hipEvent_t ev : b->lane_ev) (void)hipEventDestroy(ev);
   if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
-  for (salmon_hip_t c : b->h) c->stream = b->stream;
+  for (salmon_hip_t c : b->h) {
+    c->stream = b->stream;
+    c->in_batch = false;
+  }
   delete b;
   return 0;
 }

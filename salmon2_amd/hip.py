@@ -49,6 +49,7 @@ EXPORTS = (
     "salmon_hip_density",
     "salmon_hip_density_noalloc",
     "salmon_hip_current",
+    "salmon_hip_current_next",
     "salmon_hip_dipole",
     "salmon_hip_set_poisson",
     "salmon_hip_hartree",
@@ -120,6 +121,7 @@ def load():
     L.salmon_hip_density.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_density_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_current.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p]
+    L.salmon_hip_current_next.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p, _dp]
     L.salmon_hip_dipole.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_set_poisson.argtypes = [C.c_void_p, _dp]
     L.salmon_hip_hartree.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -284,6 +286,14 @@ class Handle:
     def current(self, vec_ac, psi, curr):
         a = np.ascontiguousarray(vec_ac, dtype=np.float64)
         _chk(load().salmon_hip_current(self.h, _p(a), C.c_void_p(_ptr(psi)), C.c_void_p(_ptr(curr))), "current")
+
+    def current_next(self, vec_ac, psi, curr, vec_ac_next):
+        """current of psi, and the phase table of vec_ac_next with psi's projection under it, built
+        ahead for the next update_kvector(vec_ac_next) and taylor/hpsi of psi."""
+        a = np.ascontiguousarray(vec_ac, dtype=np.float64)
+        an = np.ascontiguousarray(vec_ac_next, dtype=np.float64)
+        _chk(load().salmon_hip_current_next(self.h, _p(a), C.c_void_p(_ptr(psi)), C.c_void_p(_ptr(curr)), _p(an)),
+             "current_next")
 
     def dipole(self, rho, out):
         _chk(load().salmon_hip_dipole(self.h, C.c_void_p(_ptr(rho)), C.c_void_p(_ptr(out))), "dipole")

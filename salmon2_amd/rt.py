@@ -56,7 +56,7 @@ from .hip import Handle
 
 class RTPropagator:
     def __init__(self, sysm, device="cuda:0", psi=None, scf=False, pc=False, precision="fp64", trans_longi="tr",
-                 learned=None, record=False):
+                 learned=None, record=False, fuse_current=True):
         self.s = sysm
         self.dev = torch.device(device)
         torch.cuda.set_device(self.dev)
@@ -111,6 +111,9 @@ class RTPropagator:
             self.stats = torch.zeros(2, dtype=torch.int32, device=self.dev)  # accepted, attempted
             self.resid_log = torch.zeros(0, dtype=torch.float64, device=self.dev)
         self.record = [] if record else None
+        # the current's projection of the new orbitals also forms the next step's first projection
+        # (salmon_hip_current_next) when the next field is known in advance ('tr')
+        self.fuse_current = bool(fuse_current) and sysm.natom > 0
         self._vpred_rec = None
         # the current of a predictor-corrector step runs on a second stream while the density and
         # the potential update run on the first (independent reads of the new orbitals); joined at
@@ -170,7 +173,7 @@ class RTPropagator:
             self.record.append((self.vcur.clone(), self.vmid.clone(), self._vpred_rec))
             self._vpred_rec = None
         if not self.cur_async:
-            self.h.current(self.ac[itt], self.psi_b, self.curr_t[itt - 1])
+            self._current(itt)
         self.h.dipole(self.rho, self.dip_t[itt - 1])
         if self.cur_async:
             self.stream.wait_event(self.ev_cur)
@@ -187,11 +190,20 @@ class RTPropagator:
         self.cur_stream.wait_event(self.ev_psi)
         self.h.set_stream(self.cur_stream)
         try:
-            self.h.current(self.ac[itt], self.psi_b, self.curr_t[itt - 1])
+            self._current(itt)
         finally:
             self.h.set_stream(self.stream)
         self.ev_cur.record(self.cur_stream)
         self.cur_async = True
+
+    def _current(self, itt):
+        """J of the step's new orbitals (psi_b) with Ac_tot(itt); with a field fixed in advance the
+        same gather also projects psi_b under the next step's midpoint field."""
+        if self.fuse_current and self.trans_longi == "tr" and itt + 1 < len(self.ac):
+            a_next = 0.5 * (self.ac[itt + 1] + self.ac[itt])
+            self.h.current_next(self.ac[itt], self.psi_b, self.curr_t[itt - 1], a_next)
+        else:
+            self.h.current(self.ac[itt], self.psi_b, self.curr_t[itt - 1])
 
     def _corrector(self, v_pred):
         """The corrector Taylor step with V_mid (the handle's current potential) after a
