@@ -414,11 +414,11 @@ template <typename TP, int LM>
 __device__ __forceinline__ void proj_scatter_body(const ProjScatArgs &A, int bid) {
   static_assert(LM <= kMaxLmaPerAtom, "LM bounds the projectors of one atom");
   const ProjArgs &a = A.p;
-  constexpr int MAXT = (kMaxLmaPerAtom + 7) / 8;
+  constexpr int MAXT = (LM + 7) / 8;
   // rows padded by one element: the per-point accesses of the completion step (lanes on points)
   // then fall on different banks
   __shared__ double2 sp[kProjJC][kProjOrb + 1];
-  __shared__ double2 zsh[kProjJC][kMaxLmaPerAtom + 1];
+  __shared__ double2 zsh[kProjJC][LM + 1];
   extern __shared__ double2 csh[];  // cmax x kProjOrb
   const int grp = bid % a.ngrp;
   const int t = bid / a.ngrp;
@@ -447,7 +447,7 @@ __device__ __forceinline__ void proj_scatter_body(const ProjScatArgs &A, int bid
   const int oc = lane & 15;
 
   constexpr int kPer = (kProjJC * kProjOrb) / kBlock;
-  constexpr int kZPer = (kProjJC * kMaxLmaPerAtom + kBlock - 1) / kBlock;
+  constexpr int kZPer = (kProjJC * LM + kBlock - 1) / kBlock;
   TP pre[kPer];
   double2 zpre[kZPer];
   TP *h = (TP *)a.psi;
@@ -517,7 +517,7 @@ __device__ __forceinline__ void proj_scatter_body(const ProjScatArgs &A, int bid
 #pragma unroll
     for (int r = 0; r < kZPer; ++r) {
       const int e = threadIdx.x + kBlock * r;
-      if (e < kProjJC * kMaxLmaPerAtom) zsh[e % kProjJC][e / kProjJC] = zpre[r];
+      if (e < kProjJC * LM) zsh[e % kProjJC][e / kProjJC] = zpre[r];
     }
     __syncthreads();
     if (j0 + kProjJC < m) load_chunk(j0 + kProjJC);
