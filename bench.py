@@ -289,6 +289,62 @@ def fortran_hosted(s, psi, steps, warmup):
             "host": "Fortran (flang) driver over the iso_c_binding drop-in wrappers, orbitals resident"}
 
 
+def predictor_diagnostic(s, nt, every):
+    """How close predictors of the predictor-corrector's midpoint potential come to it: at every
+    `every`-th step of a pc-fp64 run, max|V_pred - V_mid(PC)| for linear extrapolation of the last
+    two potentials, the trained network, and the exact density-Taylor predictors of order 1 and 2
+    (rho(t+dt) ~ rho + dt rho' + dt^2/2 rho'', rho' = 2 sum w Im(psi* H psi),
+    rho'' = 2 sum w (|H psi|^2 - Re(psi* H^2 psi)), then (V(t) + V[rho_pred]) / 2)."""
+    import numpy as np
+    import torch
+
+    from salmon2_amd import learned
+    from salmon2_amd.rt import RTPropagator
+
+    p = RTPropagator(s, scf=True, pc=True, precision="fp64", record=True, fuse_current=False)
+    p.setup_outputs(nt)
+    P = learned.Predictor.load(learned.weights_path("c2"), stream=torch.cuda.current_stream())
+    ny = s.ny
+    w = torch.from_numpy((s.rocc[:, :, 0] * s.wtk[None, :]).T.copy()).cuda()[:, :, None, None, None, None]
+    h1, h2 = torch.zeros_like(p.psi_a), torch.zeros_like(p.psi_a)
+    rho0, vtmp = torch.zeros_like(p.rho), torch.zeros_like(p.vpsl)
+    hist, rows = [p.vcur.clone()], []
+    for it in range(nt):
+        itt = p.itt + 1
+        sample = it >= 4 and it % every == 0
+        if sample:
+            p.h.update_kvector(0.5 * (p.ac[itt] + p.ac[itt - 1]))
+            psi = p.psi_a
+            p.h.hpsi(psi, h1)
+            p.h.hpsi(h1, h2)
+            p.h.density(psi, rho0)
+            ps, a1, a2 = psi[..., :ny, :], h1[..., :ny, :], h2[..., :ny, :]
+            r1 = (2 * w * (ps.conj() * a1).imag).sum(dim=(0, 1)).reshape(rho0.shape)
+            r2 = (2 * w * (a1.abs() ** 2 - (ps.conj() * a2).real)).sum(dim=(0, 1)).reshape(rho0.shape)
+            preds = {}
+            for k, rr in (("density_taylor1", rho0 + s.dt * r1),
+                          ("density_taylor2", rho0 + s.dt * r1 + 0.5 * s.dt ** 2 * r2)):
+                p.h.update_potential(rr.contiguous(), p.vpsl, vmix=p.vcur, vout=vtmp)
+                preds[k] = vtmp.clone()
+            p.h.set_vlocal_device(p.vcur)
+            preds["linear"] = hist[-1] + 0.5 * (hist[-1] - hist[-2])
+            vp = torch.zeros_like(p.vpsl)
+            P.predict([hist[-1], hist[-2], hist[-3], hist[-4]], p.vpsl, vp)
+            preds["network"] = vp
+        p.step()
+        vnew, vmid, _ = p.record[-1]
+        hist = (hist + [vnew.clone()])[-6:]
+        p.record = p.record[-2:]
+        if sample:
+            rows.append({k: float((v - vmid).abs().max()) for k, v in preds.items()})
+            log(f"[bench] diagnostic step {it + 1}: " + " ".join(f"{k} {v:.2e}" for k, v in rows[-1].items()))
+    p.close()
+    return {"metric": "max |V_pred - V_mid| of the predictor-corrector, median over sampled steps",
+            "steps": nt, "samples": len(rows),
+            "median": {k: float(np.median([r[k] for r in rows])) for k in rows[0]},
+            "per_step": rows}
+
+
 def copy_ceiling(nbytes=1 << 30, reps=10):
     """GB/s (bytes read + written per second) of the library's streaming copy of nbytes."""
     import torch
@@ -383,6 +439,8 @@ def main():
                     help="steps of the Fortran-hosted fixed-potential loop (0 = skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the run, copy 1 GiB device-to-device (HBM-traffic counter calibration)")
+    ap.add_argument("--predictor-diagnostic", type=int, default=0, metavar="NT",
+                    help="only the midpoint-predictor diagnostic over NT pc-fp64 steps (every 50th sampled)")
     ap.add_argument("--trace", action="store_true",
                     help="for profilers: only --warmup + --steps steps of each form (no accuracy runs, roofline, "
                          "batch or CPU baseline) and --calib-copy; prints one line per form; with --forms batch, "
@@ -401,6 +459,9 @@ def main():
 
     big = args.config == "c4"
     s = system.config(args.config, host_orbitals=not big)
+    if args.predictor_diagnostic:
+        print(json.dumps(predictor_diagnostic(s, args.predictor_diagnostic, 50)), flush=True)
+        return
     psi = system.device_orbitals(s, "cuda") if big else None
     forms = [f for f in (args.forms.split(",") if args.forms else DEFAULT_FORMS.get(args.config, list(FORMS))) if f]
     if args.trace and args.forms == "batch":
