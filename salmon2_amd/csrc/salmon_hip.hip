@@ -32,6 +32,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "salmon_hip.h"
@@ -1630,13 +1631,30 @@ __device__ __forceinline__ void density_part_body(const DensArgs &a, int bx, int
   const size_t off = (size_t)x + (size_t)a.nx * ((size_t)y + (size_t)a.ly * z);
   const int M = a.no * a.nk;
   const int m0 = grp * a.gsz, m1 = min(M, m0 + a.gsz);
+  const double2 *pb = a.psi + off;
+  const size_t bstride = (size_t)a.nspin * a.osz;  // one orbital to the next
+  const double *occ = a.rocc + (size_t)a.no * a.nk * ispin;
+  int io = m0 % a.no, ik = m0 / a.no;
   double acc = 0.0;
-  for (int mm = m0; mm < m1; ++mm) {
-    const int io = mm % a.no, ik = mm / a.no;
-    const size_t b = (size_t)ispin + (size_t)a.nspin * mm;
-    const double2 p = a.psi[b * a.osz + off];
+  int mm = m0;
+  // U orbitals' loads in flight before the in-order sum consumes them
+  constexpr int U = 8;
+  for (; mm + U <= m1; mm += U) {
+    double2 p[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) p[u] = pb[((size_t)ispin * a.osz) + (size_t)(mm + u) * bstride];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double w2 = p[u].x * p[u].x + p[u].y * p[u].y;
+      acc = acc + (w2 * occ[io + (size_t)a.no * ik]) * a.wtk[ik];
+      if (++io == a.no) { io = 0; ++ik; }
+    }
+  }
+  for (; mm < m1; ++mm) {
+    const double2 p = pb[((size_t)ispin * a.osz) + (size_t)mm * bstride];
     const double w2 = p.x * p.x + p.y * p.y;
-    acc = acc + (w2 * a.rocc[io + (size_t)a.no * (ik + (size_t)a.nk * ispin)]) * a.wtk[ik];
+    acc = acc + (w2 * occ[io + (size_t)a.no * ik]) * a.wtk[ik];
+    if (++io == a.no) { io = 0; ++ik; }
   }
   if (a.ngrp == 1)
     a.rho[g + ngrid * ispin] = acc;
@@ -1660,8 +1678,18 @@ __device__ __forceinline__ void density_sum_body(const DensArgs &a, int bx) {
   const size_t ngrid = (size_t)a.nx * a.ny * a.nz;
   const size_t i = (size_t)bx * kBlock + threadIdx.x;
   if (i >= ngrid * a.nspin) return;
+  const size_t gs = ngrid * a.nspin;  // one group's partials to the next
   double acc = 0.0;
-  for (int grp = 0; grp < a.ngrp; ++grp) acc = acc + a.part[i + ngrid * a.nspin * (size_t)grp];
+  int grp = 0;
+  // 8 partials' loads in flight ahead of the in-order sum
+  for (; grp + 8 <= a.ngrp; grp += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = a.part[i + gs * (size_t)(grp + u)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = acc + v[u];
+  }
+  for (; grp < a.ngrp; ++grp) acc = acc + a.part[i + gs * (size_t)grp];
   a.rho[i] = acc;
 }
 
@@ -1685,6 +1713,13 @@ struct CurArgs {
   size_t osz;
   int nx, ny, nz, ly, TY, nyt;
 };
+
+// f(integral_constant<int, I>) for I = 0, 1, ... in order while it returns true: a loop body
+// whose iteration index is a compile-time constant (register-array slots stay registers).
+template <int... I, typename F>
+__device__ __forceinline__ void unroll_while(std::integer_sequence<int, I...>, F &&f) {
+  (void)(f(std::integral_constant<int, I>{}) && ...);
+}
 
 __device__ __forceinline__ double block_sum(double v, double *sh) {
   sh[threadIdx.x] = v;
@@ -1716,41 +1751,81 @@ __device__ __forceinline__ void current_local_body(const CurArgs &a, int bid) {
   const double2 *p = a.psi + (size_t)b * a.osz;
   const size_t col = (size_t)tx + (size_t)nx * yw;
   const int planeL = (TY + 4) * nx;
-  // halo: rows y0+TY .. y0+TY+3, 4*nx elements, loaded by threads e < 4*nx (looping)
-  double2 q[5];
+  // z neighbours: a register ring of R planes, plane z in slot z % R (the z loop is unrolled by R,
+  // so every slot index is a constant and no register moves wait on a load); plane z + R - 1 is
+  // loaded R - 5 iterations before its first use.  Halo: rows y0+TY .. y0+TY+3, 4*nx elements,
+  // element e = tid + k * kBlock (k < nh <= 4, nx <= 256), loaded one plane ahead into a
+  // two-deep register ring (R even: slot parity = plane parity).
+  constexpr int R = 8;
+  double2 q[R];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) q[i] = p[col + zs * i];
+  for (int i = 0; i < R - 1; ++i) q[i] = p[col + zs * wrapi(i, nz)];
+  const int nh = (4 * nx + kBlock - 1) / kBlock;
+  int hoff[4];     // halo element k's offset within a plane (0 where the lane has none: loaded, unused)
+  bool hown[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = tid + k * kBlock;
+    hown[k] = k < nh && e < 4 * nx;
+    const int ee = hown[k] ? e : 0;
+    hoff[k] = ee % nx + nx * wrapi(y0 + TY + ee / nx, ny);
+  }
+  // halo rows of the even (hA*) and odd (hB*) planes; named registers, not arrays
+  double2 hA0 = {}, hA1 = {}, hA2 = {}, hA3 = {}, hB0 = {}, hB1 = {}, hB2 = {}, hB3 = {};
+  auto halo_get = [&](size_t zoff, double2 &v0, double2 &v1, double2 &v2, double2 &v3)
+                      __attribute__((always_inline)) {
+    v0 = p[(size_t)hoff[0] + zoff];
+    if (nh > 1) v1 = p[(size_t)hoff[1] + zoff];
+    if (nh > 2) v2 = p[(size_t)hoff[2] + zoff];
+    if (nh > 3) v3 = p[(size_t)hoff[3] + zoff];
+  };
+  auto halo_put = [&](double2 *P, const double2 &v0, const double2 &v1, const double2 &v2, const double2 &v3)
+                      __attribute__((always_inline)) {
+    double2 *H = P + TY * nx + tid;
+    if (hown[0]) H[0] = v0;
+    if (hown[1]) H[kBlock] = v1;
+    if (hown[2]) H[2 * kBlock] = v2;
+    if (hown[3]) H[3 * kBlock] = v3;
+  };
+  halo_get(0, hA0, hA1, hA2, hA3);
   double r0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-  int buf = 0;
-  for (int z = 0; z < nz; ++z) {
-    q[4] = p[col + zs * wrapi(z + 4, nz)];
-    double2 *P = lds + buf * planeL;
-    if (active) P[ty * nx + tx] = q[0];
-    for (int e = tid; e < 4 * nx; e += kBlock) {
-      const int hr = e / nx, hx = e % nx;
-      P[(TY + hr) * nx + hx] = p[(size_t)hx + (size_t)nx * wrapi(y0 + TY + hr, ny) + zs * z];
-    }
-    __syncthreads();
-    if (own) {
-      const double2 c = q[0];
-      r0 += c.x * c.x + c.y * c.y;
-      const double cr = c.x, ci = -c.y;
-#pragma unroll
-      for (int n = 1; n <= 4; ++n) {
-        const double2 px = P[ty * nx + wrapi(tx + n, nx)];
-        const double2 py = P[(ty + n) * nx + tx];
-        const double2 pz = q[n];
-        const double ax = a.coef_nab[n - 1] * cr, bx = a.coef_nab[n - 1] * ci;
-        const double ay = a.coef_nab[3 + n] * cr, by = a.coef_nab[3 + n] * ci;
-        const double az = a.coef_nab[7 + n] * cr, bz = a.coef_nab[7 + n] * ci;
-        sx += ax * px.y + bx * px.x;
-        sy += ay * py.y + by * py.x;
-        sz += az * pz.y + bz * pz.x;
+  for (int z0 = 0; z0 < nz; z0 += R) {
+    // plane z = z0 + u for the constants u = 0 .. R-1, while z < nz
+    unroll_while(std::make_integer_sequence<int, R>{}, [&](auto uc) __attribute__((always_inline)) {
+      constexpr int u = decltype(uc)::value;
+      const int z = z0 + u;
+      if (z >= nz) return false;
+      double2 *P = lds + (u & 1) * planeL;
+      if (active) P[ty * nx + tx] = q[u];
+      const size_t zn = zs * wrapi(z + 1, nz);
+      if constexpr ((u & 1) == 0) {
+        halo_put(P, hA0, hA1, hA2, hA3);
+        halo_get(zn, hB0, hB1, hB2, hB3);
+      } else {
+        halo_put(P, hB0, hB1, hB2, hB3);
+        halo_get(zn, hA0, hA1, hA2, hA3);
       }
-    }
+      q[(u + R - 1) % R] = p[col + zs * wrapi(z + R - 1, nz)];
+      __syncthreads();
+      if (own) {
+        const double2 c = q[u];
+        r0 += c.x * c.x + c.y * c.y;
+        const double cr = c.x, ci = -c.y;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) q[i] = q[i + 1];
-    buf ^= 1;
+        for (int n = 1; n <= 4; ++n) {
+          const double2 px = P[ty * nx + wrapi(tx + n, nx)];
+          const double2 py = P[(ty + n) * nx + tx];
+          const double2 pz = q[(u + n) % R];
+          const double ax = a.coef_nab[n - 1] * cr, bx = a.coef_nab[n - 1] * ci;
+          const double ay = a.coef_nab[3 + n] * cr, by = a.coef_nab[3 + n] * ci;
+          const double az = a.coef_nab[7 + n] * cr, bz = a.coef_nab[7 + n] * ci;
+          sx += ax * px.y + bx * px.x;
+          sy += ay * py.y + by * py.x;
+          sz += az * pz.y + bz * pz.x;
+        }
+      }
+      return true;
+    });
   }
   r0 = block_sum(r0, sh);
   sx = block_sum(sx, sh);
@@ -1978,7 +2053,21 @@ __global__ __launch_bounds__(256) void k_dft_axis(DftArgs a) {
     if (cl >= ncl) continue;
     const double2 *w = a.tw + (size_t)k * n;
     double sr = 0.0, si = 0.0;
-    for (int j = 0; j < n; ++j) {
+    int j = 0;
+    // 8 twiddle loads in flight ahead of the in-order sum
+    for (; j + 8 <= n; j += 8) {
+      double2 e[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) e[u] = w[j + u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const double2 x = dft_tile[(j + u) * cols + cl];
+        const double ei = a.sg * e[u].y;
+        sr = sr + (e[u].x * x.x - ei * x.y);
+        si = si + (e[u].x * x.y + ei * x.x);
+      }
+    }
+    for (; j < n; ++j) {
       const double2 e = w[j];
       const double2 x = dft_tile[j * cols + cl];
       const double ei = a.sg * e.y;
