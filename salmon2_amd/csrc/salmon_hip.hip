@@ -1424,6 +1424,24 @@ __global__ __launch_bounds__(kTile) void k_stencilz(StencilArgs a) {
   stencil_tilez<TI, R, TO, NCH>(a, bid, lds);
 }
 
+// Persistent form over many systems (the batched step's terms without the Taylor sum): block b
+// takes tiles b, b + grid, ... of `tiles` = (system index, tile index).  stencil_tilez returns
+// with its DMAs drained (vmcnt 0); the barrier keeps the next tile's DMAs off slots that other
+// waves still read.
+template <typename TI, typename R, typename TO, int NCH>
+__global__ __launch_bounds__(kTile) void k_stencilz_packed(const StencilArgs *__restrict__ sys,
+                                                           const int2 *__restrict__ tiles, int ntiles) {
+  extern __shared__ double2 lds[];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int2 tr = tiles[t];
+    const int si = __builtin_amdgcn_readfirstlane(tr.x);
+    const int bid = __builtin_amdgcn_readfirstlane(tr.y);
+    const StencilArgs a = sys[si];
+    stencil_tilez<TI, R, TO, NCH>(a, bid, lds);
+    __syncthreads();
+  }
+}
+
 template <typename TI, typename R, typename TO, int NCH>
 __global__ __launch_bounds__(kTile) void k_stencil8(StencilArgs a) {
   extern __shared__ double2 lds[];
@@ -3610,6 +3628,27 @@ int launch_packed(hipStream_t st, int hc, int nc, const StencilArgs *sys, const 
   return launch_packed_hc<8>(st, nc, sys, tiles, ntiles, grid);
 }
 
+template <int NCH>
+int launch_packed_z_t(hipStream_t st, const StencilArgs *sys, const int2 *tiles, int ntiles, int grid) {
+  const size_t shmem = sizeof(double2) * 4 * NCH * kTile + sizeof(double) * 4 * kTile;
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencilz_packed<double2, double, double2, NCH>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_stencilz_packed<double2, double, double2, NCH>), dim3(grid), dim3(kTile), shmem, st, sys,
+                     tiles, ntiles);
+  LAUNCHCHK("k_stencilz_packed");
+  return 0;
+}
+
+// the delayed-z y tiles for the terms without the Taylor sum, slots of nch x 256 elements
+int launch_packed_z(hipStream_t st, int nch, const StencilArgs *sys, const int2 *tiles, int ntiles, int grid) {
+  return nch == 2 ? launch_packed_z_t<2>(st, sys, tiles, ntiles, grid)
+                  : launch_packed_z_t<3>(st, sys, tiles, ntiles, grid);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -3736,6 +3775,7 @@ struct salmon_hip_batch {
     int2 *d_tiles[2] = {nullptr, nullptr};  // middle terms, last term (different z-chunking)
     int ntiles[2] = {0, 0}, grid[2] = {0, 0};
     StencilArgs *d_args = nullptr;      // [parity][term][member]
+    int zch = 0;                        // middle terms on the delayed-z kernel, slots of zch x 256 (0: no)
   };
   std::vector<Group> groups;
   int n_hamil = 0;
@@ -3789,6 +3829,17 @@ void batch_buffers(salmon_hip_batch *b, int i, int p, void **hh, void **out) {
 // (re)build tile lists and descriptor tables for the current z_chunks / grid_mult
 int batch_tables(salmon_hip_batch *b) {
   for (auto &g : b->groups) {
+    // the terms without the Taylor sum run on the delayed-z tiles when every member's rows fit
+    // its slots (nx <= 64: zch = hc + 1 chunks hold the TY + 8 rows) and its z extent is a
+    // multiple of 4; z chunks are then rounded up to whole multiples of 4 planes
+    const int zch = g.hc + 1;
+    bool zok = g.hc <= 2;
+    for (int i : g.mem) {
+      const salmon_hip_t c = b->h[i];
+      zok = zok && c->tilez && c->nx <= 64 && c->nz % 4 == 0 && c->TY * c->nx <= kTile &&
+            (c->TY + 8) * c->nx <= zch * kTile;
+    }
+    g.zch = zok ? zch : 0;
     std::vector<int2> tiles[2];
     std::vector<StencilArgs> args;
     for (int p = 0; p < 2; ++p)
@@ -3804,6 +3855,11 @@ int batch_tables(salmon_hip_batch *b) {
           c->mid_nzt = b->z_chunks;
           StencilArgs a = stencil_args(c, hh[n - 1], fin ? out : hh[n], fin, hh, n - 1, b->zc.data(), &bs, true);
           c->mid_nzt = save;
+          if (g.zch && !fin) {
+            a.TZ = (a.TZ + 3) / 4 * 4;
+            a.nzt = (a.nz + a.TZ - 1) / a.TZ;
+            a.nblk = c->norb * a.nyt * a.nzt;
+          }
           args.push_back(a);
           if (p == 0 && (n == 1 || fin))
             for (int t = 0; t < a.nblk; ++t) tiles[fin ? 1 : 0].push_back(make_int2((int)m, t));
@@ -3827,7 +3883,9 @@ int batch_stencil(salmon_hip_batch *b, int p, int n) {
   for (auto &g : b->groups) {
     const StencilArgs *args = g.d_args + ((size_t)p * b->n_hamil + (n - 1)) * g.mem.size();
     const int k = nc >= 0 ? 1 : 0;
-    if (int rc = launch_packed(b->stream, g.hc, nc, args, g.d_tiles[k], g.ntiles[k], g.grid[k])) return rc;
+    const int rc = (nc < 0 && g.zch) ? launch_packed_z(b->stream, g.zch, args, g.d_tiles[0], g.ntiles[0], g.grid[0])
+                                     : launch_packed(b->stream, g.hc, nc, args, g.d_tiles[k], g.ntiles[k], g.grid[k]);
+    if (rc) return rc;
   }
   return 0;
 }
