@@ -106,6 +106,9 @@ class RTPropagator:
             self.ev_pred = torch.cuda.Event()
             self.pred_ready = False
             self.vnew = torch.zeros_like(self.vpsl)
+            # V_H and V_xc of the attempt's density, kept for the rejected branch's midpoint potential
+            self.vh_att = torch.zeros_like(self.vpsl)
+            self.vxc_att = torch.zeros_like(self.vpsl)
             self.resid = torch.zeros(1, dtype=torch.float64, device=self.dev)
             self.flag = torch.zeros(1, dtype=torch.int32, device=self.dev)  # 1: this step's attempt accepted
             self.stats = torch.zeros(2, dtype=torch.int32, device=self.dev)  # accepted, attempted
@@ -224,14 +227,18 @@ class RTPropagator:
         self.h.set_vlocal_device(self.vpred)
         self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
         self.h.density(self.psi_b, self.rho)
-        self.h.update_potential(self.rho, self.vpsl, vout=self.vnew)
+        # V[rho'] in its parts (the same kernels' arithmetic as update_potential), so that the
+        # rejected branch forms (V(t) + V[rho'])/2 without a second Hartree solve of rho'
+        self.h.hartree(self.rho, self.vh_att)
+        self.h.xc_pz(self.rho, self.vxc_att)
+        self.h.update_vlocal(self.vpsl, self.vh_att, self.vxc_att, vout=self.vnew)
         P.net.residual(self.vpred, self.vcur, self.vnew, self.resid)
         P.net.gate(self.resid, getattr(P, "tol_run", P.tol), self.flag, self.stats, self.resid_log)
         # rejected: the attempt is the predictor, V_mid = (V(t) + V[rho'])/2, then the corrector,
         # the density and V(t+dt); accepted: these launches do nothing
         self.h.set_skip(self.flag)
         try:
-            self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)
+            self.h.update_vlocal(self.vpsl, self.vh_att, self.vxc_att, vmix=self.vcur, vout=self.vmid)
             self._corrector(self.vpred)
             self.h.density(self.psi_b, self.rho)
             self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
