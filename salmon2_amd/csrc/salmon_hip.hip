@@ -1093,7 +1093,11 @@ __device__ __forceinline__ void stencil_tile8(const StencilArgs &a, int bid, dou
   constexpr int WI = DmaW<TI>::W;
   constexpr unsigned EB = sizeof(TI);
   constexpr int SLOT = NCH * BS;              // elements per slot: (TY + 8) * nx <= NCH * 256
-  constexpr int kOps = NCH * WI + 2 + 1;
+  // FP32 elements in 2-chunk slots (rows of even length): one dwordx4 DMA per wave and plane moves
+  // the wave's 128 slot elements (lane l: elements 128 w + 2l and 2l + 1, one row, 16-byte
+  // aligned) and one more the V of its 64 points (lanes < 32: points 64 w + 2l, 2l + 1)
+  constexpr bool X4 = WI == 2 && NCH == 2;
+  constexpr int kOps = X4 ? 1 + 1 + 1 : NCH * WI + 2 + 1;
   TI *tileR = (TI *)lds;                      // 8 x SLOT
   double *vR = (double *)(tileR + 8 * SLOT);  // 4 x BS
 
@@ -1139,18 +1143,33 @@ __device__ __forceinline__ void stencil_tile8(const StencilArgs &a, int bid, dou
     const int ec = e < ntile ? e : 0;
     voff[kk] = 2 * ((size_t)(ec % nx) + (size_t)nx * wrapi(y0 + ec / nx, ny)) + (lane & 1);
   }
+  size_t toff4 = 0, voff4 = 0;
+  if (X4) {
+    toff4 = sofs(128 * wave + 2 * lane);
+    const int e = 64 * wave + 2 * (lane & 31);
+    const int ec = e < ntile ? e : 0;
+    voff4 = (size_t)(ec % nx) + (size_t)nx * wrapi(y0 + ec / nx, ny);
+  }
   const double *Vb = a.V + (size_t)ispin * gplane * nz;
-  const unsigned tileA = __builtin_amdgcn_readfirstlane(lds_addr(tileR) + EB * 64u * wave);
+  const unsigned tileA = __builtin_amdgcn_readfirstlane(lds_addr(tileR) + EB * (X4 ? 128u : 64u) * wave);
   const unsigned vA = __builtin_amdgcn_readfirstlane(lds_addr(vR) + 8u * 64u * wave);
   auto dma_tile = [&](int plane, int slot) {
     const TI *src = in + zs * wrapi(plane, nz);
+    if constexpr (X4) {
+      dma16(src + toff4, tileA + EB * (unsigned)(SLOT * slot));
+    } else {
 #pragma unroll
-    for (int k = 0; k < NCH; ++k) dma_chunk<TI>(src, toff[k], tileA + EB * (unsigned)(SLOT * slot + 4 * 64 * k), lane);
+      for (int k = 0; k < NCH; ++k) dma_chunk<TI>(src, toff[k], tileA + EB * (unsigned)(SLOT * slot + 4 * 64 * k), lane);
+    }
   };
   auto dma_v = [&](int plane, int slot) {
-    const float *vb = (const float *)Vb + 2 * gplane * wrapi(plane, nz);
-    dma4(vb + voff[0], vA + 8u * BS * slot);
-    dma4(vb + voff[1], vA + 8u * BS * slot + 256u);
+    if constexpr (X4) {
+      if (lane < 32) dma16(Vb + gplane * wrapi(plane, nz) + voff4, vA + 8u * BS * slot);
+    } else {
+      const float *vb = (const float *)Vb + 2 * gplane * wrapi(plane, nz);
+      dma4(vb + voff[0], vA + 8u * BS * slot);
+      dma4(vb + voff[1], vA + 8u * BS * slot + 256u);
+    }
   };
   // element indices in a slot: own point, x and y neighbours
   const int own = (ty + 4) * nx + tx;
@@ -2838,6 +2857,11 @@ static bool tile8_fits(const salmon_hip_ctx *c, int stage, int nc, const Stencil
          a.TZ % 8 == 0 && a.nz % 8 == 0 && c->tile8;
 }
 
+// 2-chunk slots take the paired dwordx4 DMAs: rows of even length, 16-byte aligned orbital blocks
+static bool tile8_pairs(const StencilArgs &a) {
+  return (a.TY + 8) * a.nx <= 2 * kTile && a.nx % 2 == 0 && a.osz % 2 == 0;
+}
+
 // y tiles with the delayed z part (stencil_tilez): the FP64 terms without the Taylor sum and the
 // mixed first term (FP64 input), z chunks of 4k planes; a plane's tile and halo rows fill a slot
 // of two 256-element chunks (rows of up to 32 points) or three (up to 64 points: 48-point rows
@@ -2881,7 +2905,7 @@ static int launch_tile8_n(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_
 
 template <typename TI, typename R, typename TO>
 static int launch_tile8_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
-  if ((a.TY + 8) * a.nx <= 2 * kTile) return launch_tile8_n<TI, R, TO, 2>(c, grid, e0, e1, a);
+  if (tile8_pairs(a)) return launch_tile8_n<TI, R, TO, 2>(c, grid, e0, e1, a);
   return launch_tile8_n<TI, R, TO, 3>(c, grid, e0, e1, a);
 }
 
