@@ -45,6 +45,10 @@ CASES = {
     "ragged_24x22x16": lambda: system.make_si_system(n=(24, 22, 16), no=6, nkgrid=(1, 1, 1), seed=19),
     "overlap_rc3.6": lambda: system.make_si_system(n=12, no=4, nkgrid=(1, 1, 2), rcut=3.6, seed=23),
     "selfimage_rc5.3": lambda: system.make_si_system(n=12, no=4, nkgrid=(1, 1, 2), rcut=5.3, seed=23),
+    # 72-point rows (separate-halo tiles, 4 halo chunks) and an odd row length (the middle terms'
+    # 3-chunk slots with paired dword DMAs instead of the 2-chunk paired dwordx4 form)
+    "wide_72x12x16": lambda: system.make_si_system(n=(72, 12, 16), no=3, nkgrid=(1, 1, 1), seed=29),
+    "odd_21x18x16": lambda: system.make_si_system(n=(21, 18, 16), no=4, nkgrid=(1, 1, 1), seed=37),
 }
 
 

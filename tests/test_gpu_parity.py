@@ -46,6 +46,13 @@ def _cases():
         "overlap_rc3.6": lambda: system.make_si_system(n=12, no=4, nkgrid=(1, 1, 2), rcut=3.6, seed=23),
         # radius 5.3 > a/2: every touched point covered 2-10 times, also by images of the same atom
         "selfimage_rc5.3": lambda: system.make_si_system(n=12, no=4, nkgrid=(1, 1, 2), rcut=5.3, seed=23),
+        # rows wider than 64 points: 3-row y tiles, halo ring of 4 chunks (8 x 72 halo points), two
+        # halo elements per lane in the local current
+        "wide_72x12x16": lambda: system.make_si_system(n=(72, 12, 16), no=3, nkgrid=(1, 1, 1), seed=29),
+        # 136-point rows: one-row tiles, halo ring of 8 chunks, every y neighbour from the halo
+        "wide_136x8x8": lambda: system.make_si_system(n=(136, 8, 8), no=2, nkgrid=(1, 1, 1), seed=31),
+        # odd row length
+        "odd_21x18x16": lambda: system.make_si_system(n=(21, 18, 16), no=4, nkgrid=(1, 1, 1), seed=37),
     }
 
 
