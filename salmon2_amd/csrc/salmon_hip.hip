@@ -1078,26 +1078,30 @@ __device__ __forceinline__ void stencil_plane(const StencilArgs &a, int bid, dou
 // slots and the z loop is unrolled by 8, so slot offsets become immediate offsets of the
 // LDS reads; z neighbours above come from the ring, those below from a 4-deep register
 // queue whose rotation is resolved by the unrolling (no moves).  Issue order per plane and
-// wave: 2 x W tile DMAs (plane z+7), 2 V dword DMAs (plane z+3), 1 store; the wait at the
-// top of a plane leaves the last two planes' operations in flight.  Needs TZ % 8 == 0.
+// wave: the tile DMAs of plane z+7 (NCH x W, or with PR ceil(NCH/2) paired dwordx4 ones), the V
+// DMAs of plane z+3 (2 dword, or with PR 1 dwordx4), 1 store; the wait at the top of a plane
+// leaves the last two planes' operations in flight.  Needs TZ % 8 == 0.
 // Used for the middle terms of the mixed chain (instruction-issue bound): the pair sums of
 // the x, y and z neighbours go into one fused multiply-add chain per component, a regrouping
 // of stencil_tile's sum that saves two instructions per component.
 // ---------------------------------------------------------------------------
 template <int P> struct Ph { static constexpr int v = P; };
 
-template <typename TI, typename R, typename TO, int NCH>
+template <typename TI, typename R, typename TO, int NCH, bool PR>
 __device__ __forceinline__ void stencil_tile8(const StencilArgs &a, int bid, double2 *lds) {
   using CH = typename Cx<R>::T;
   constexpr int BS = kTile;
   constexpr int WI = DmaW<TI>::W;
   constexpr unsigned EB = sizeof(TI);
   constexpr int SLOT = NCH * BS;              // elements per slot: (TY + 8) * nx <= NCH * 256
-  // FP32 elements in 2-chunk slots (rows of even length): one dwordx4 DMA per wave and plane moves
-  // the wave's 128 slot elements (lane l: elements 128 w + 2l and 2l + 1, one row, 16-byte
-  // aligned) and one more the V of its 64 points (lanes < 32: points 64 w + 2l, 2l + 1)
-  constexpr bool X4 = WI == 2 && NCH == 2;
-  constexpr int kOps = X4 ? 1 + 1 + 1 : NCH * WI + 2 + 1;
+  // PR (FP32 elements, rows of even length): the wave's 64 NCH slot elements move by paired dwordx4
+  // DMAs (lane l: elements 2l and 2l + 1 of a 128-element piece, one row, 16-byte aligned), one
+  // instruction for NCH = 2 and two for NCH = 3 (the second on lanes < 32), and the V of its 64
+  // points by one more (lanes < 32: points 64 w + 2l, 2l + 1)
+  constexpr bool X4 = PR;
+  static_assert(!PR || WI == 2, "paired DMAs move FP32 elements");
+  constexpr int NT4 = (NCH + 1) / 2;
+  constexpr int kOps = X4 ? NT4 + 1 + 1 : NCH * WI + 2 + 1;
   TI *tileR = (TI *)lds;                      // 8 x SLOT
   double *vR = (double *)(tileR + 8 * SLOT);  // 4 x BS
 
@@ -1143,20 +1147,24 @@ __device__ __forceinline__ void stencil_tile8(const StencilArgs &a, int bid, dou
     const int ec = e < ntile ? e : 0;
     voff[kk] = 2 * ((size_t)(ec % nx) + (size_t)nx * wrapi(y0 + ec / nx, ny)) + (lane & 1);
   }
-  size_t toff4 = 0, voff4 = 0;
+  size_t toff4[2] = {0, 0}, voff4 = 0;
   if (X4) {
-    toff4 = sofs(128 * wave + 2 * lane);
+#pragma unroll
+    for (int i = 0; i < NT4; ++i) toff4[i] = sofs(64 * NCH * wave + 128 * i + 2 * lane);
     const int e = 64 * wave + 2 * (lane & 31);
     const int ec = e < ntile ? e : 0;
     voff4 = (size_t)(ec % nx) + (size_t)nx * wrapi(y0 + ec / nx, ny);
   }
   const double *Vb = a.V + (size_t)ispin * gplane * nz;
-  const unsigned tileA = __builtin_amdgcn_readfirstlane(lds_addr(tileR) + EB * (X4 ? 128u : 64u) * wave);
+  const unsigned tileA = __builtin_amdgcn_readfirstlane(lds_addr(tileR) + EB * (X4 ? 64u * NCH : 64u) * wave);
   const unsigned vA = __builtin_amdgcn_readfirstlane(lds_addr(vR) + 8u * 64u * wave);
   auto dma_tile = [&](int plane, int slot) {
     const TI *src = in + zs * wrapi(plane, nz);
     if constexpr (X4) {
-      dma16(src + toff4, tileA + EB * (unsigned)(SLOT * slot));
+      dma16(src + toff4[0], tileA + EB * (unsigned)(SLOT * slot));
+      if constexpr (NT4 > 1) {
+        if (lane < 32) dma16(src + toff4[1], tileA + EB * (unsigned)(SLOT * slot + 128));
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < NCH; ++k) dma_chunk<TI>(src, toff[k], tileA + EB * (unsigned)(SLOT * slot + 4 * 64 * k), lane);
@@ -1461,14 +1469,14 @@ __global__ __launch_bounds__(kTile) void k_stencilz_packed(const StencilArgs *__
   }
 }
 
-template <typename TI, typename R, typename TO, int NCH>
+template <typename TI, typename R, typename TO, int NCH, bool PR>
 __global__ __launch_bounds__(kTile) void k_stencil8(StencilArgs a) {
   extern __shared__ double2 lds[];
   if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
   int bid = blockIdx.x;
   const int nfull = (a.nblk / 8) * 8;
   if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;  // XCD-aware tile order
-  stencil_tile8<TI, R, TO, NCH>(a, bid, lds);
+  stencil_tile8<TI, R, TO, NCH, PR>(a, bid, lds);
 }
 
 template <typename TI, typename R, typename TO, int HC, int NC, int BS>
@@ -2857,10 +2865,8 @@ static bool tile8_fits(const salmon_hip_ctx *c, int stage, int nc, const Stencil
          a.TZ % 8 == 0 && a.nz % 8 == 0 && c->tile8;
 }
 
-// 2-chunk slots take the paired dwordx4 DMAs: rows of even length, 16-byte aligned orbital blocks
-static bool tile8_pairs(const StencilArgs &a) {
-  return (a.TY + 8) * a.nx <= 2 * kTile && a.nx % 2 == 0 && a.osz % 2 == 0;
-}
+// the paired dwordx4 DMAs need rows of even length and 16-byte aligned orbital blocks
+static bool tile8_pairs(const StencilArgs &a) { return a.nx % 2 == 0 && a.osz % 2 == 0; }
 
 // y tiles with the delayed z part (stencil_tilez): the FP64 terms without the Taylor sum and the
 // mixed first term (FP64 input), z chunks of 4k planes; a plane's tile and halo rows fill a slot
@@ -2890,23 +2896,25 @@ static int launch_tilez_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_
   return launch_tilez_n<TI, R, TO, 3>(c, grid, e0, e1, a);
 }
 
-template <typename TI, typename R, typename TO, int NCH>
+template <typename TI, typename R, typename TO, int NCH, bool PR>
 static int launch_tile8_n(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
   const size_t shmem = sizeof(TI) * 8 * NCH * kTile + sizeof(double) * 4 * kTile;
   static bool attr_set = false;
   if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencil8<TI, R, TO, NCH>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)shmem));
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencil8<TI, R, TO, NCH, PR>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     attr_set = true;
   }
-  hipExtLaunchKernelGGL((k_stencil8<TI, R, TO, NCH>), grid, dim3(kTile), shmem, c->stream, e0, e1, 0, a);
+  hipExtLaunchKernelGGL((k_stencil8<TI, R, TO, NCH, PR>), grid, dim3(kTile), shmem, c->stream, e0, e1, 0, a);
   return 0;
 }
 
 template <typename TI, typename R, typename TO>
 static int launch_tile8_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
-  if (tile8_pairs(a)) return launch_tile8_n<TI, R, TO, 2>(c, grid, e0, e1, a);
-  return launch_tile8_n<TI, R, TO, 3>(c, grid, e0, e1, a);
+  const bool pr = tile8_pairs(a);
+  if ((a.TY + 8) * a.nx <= 2 * kTile)
+    return pr ? launch_tile8_n<TI, R, TO, 2, true>(c, grid, e0, e1, a) : launch_tile8_n<TI, R, TO, 2, false>(c, grid, e0, e1, a);
+  return pr ? launch_tile8_n<TI, R, TO, 3, true>(c, grid, e0, e1, a) : launch_tile8_n<TI, R, TO, 3, false>(c, grid, e0, e1, a);
 }
 
 template <int HC>
