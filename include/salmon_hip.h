@@ -290,7 +290,9 @@ int salmon_hip_update_potential(salmon_hip_t h, const double *rho_dev, const dou
  * (update_kvector, taylor, calc_density, calc_current, calc_dip of
  * src/rt/time_evolution_step.f90) for all of them together: the stencil passes of
  * all systems of one halo class run as ONE persistent kernel over a combined tile
- * list, every other phase (phase table, projections, fused KB passes, scatter,
+ * list (the terms without the Taylor sum on the single-system path's delayed-z tiles
+ * when every member's rows fit them, z chunks rounded up to multiples of 4 planes),
+ * every other phase (phase table, projections, fused KB passes, scatter,
  * density, current, dipole) runs as ONE launch for all systems (per-system argument
  * records and block offsets on the device; when a system with projectors cannot run
  * the fused KB pass, those phases fall back to per-system launches forked onto four
