@@ -1806,7 +1806,7 @@ __device__ __forceinline__ void current_local_body(const CurArgs &a, int bid) {
 #pragma unroll
   for (int i = 0; i < R - 1; ++i) q[i] = p[col + zs * wrapi(i, nz)];
   const int nh = (4 * nx + kBlock - 1) / kBlock;
-  int hoff[4];     // halo element k's offset within a plane (0 where the lane has none: loaded, unused)
+  int hoff[4];     // halo element k's offset within a plane (element 0's where the lane has none: loaded, unused)
   bool hown[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
