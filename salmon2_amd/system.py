@@ -395,6 +395,7 @@ def make_si_system(
     rc=1.5,
     rcut=2.5,
     host_orbitals=True,
+    nspin=1,
 ):
     """Seeded diamond-Si supercell of the kind the benchmark configurations name.
 
@@ -426,11 +427,18 @@ def make_si_system(
         uv = np.zeros((0, 0))
         ia_tbl = np.zeros(0, np.int32)
         rinv = np.zeros(0)
-    nspin = 1
+    if nspin not in (1, 2):
+        raise ValueError("nspin must be 1 or 2")
     ly = nn[1] + pad
     rng_state = rng.bit_generator.state
     psi = random_orbitals(rng, nn, ly, no, nk, nspin, hvol) if host_orbitals else None
-    rocc = np.full((nspin, nk, no), 2.0)
+    # spin-polarised (collinear): one electron per orbital and spin channel, and a spin-dependent
+    # V_local (the second channel's differs by a smooth exchange-like splitting along x)
+    rocc = np.full((nspin, nk, no), 2.0 / nspin)
+    if nspin == 2:
+        x = np.arange(nn[0]) * hgs[0]
+        split = 0.1 * np.cos(2.0 * np.pi * x / al[0])[None, None, :]
+        V = np.stack([V, V + split])
     pulse = Pulse(A0=A0, omega=omega, tw=nt * dt)
     return System(
         name=name,
@@ -450,7 +458,7 @@ def make_si_system(
         coef_lap=lap,
         coef_nab=nab,
         coef_lap0=lap0,
-        vlocal=V[None].copy(),
+        vlocal=V[None].copy() if nspin == 1 else V.copy(),
         rion=rion,
         natom=natom,
         nps=nps,

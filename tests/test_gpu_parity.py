@@ -53,6 +53,8 @@ def _cases():
         "wide_136x8x8": lambda: system.make_si_system(n=(136, 8, 8), no=2, nkgrid=(1, 1, 1), seed=31),
         # odd row length
         "odd_21x18x16": lambda: system.make_si_system(n=(21, 18, 16), no=4, nkgrid=(1, 1, 1), seed=37),
+        # spin-polarised (collinear): two channels, each with its own V_local
+        "spin2_16x14x12": lambda: system.make_si_system(n=(16, 14, 12), no=3, nkgrid=(1, 1, 2), seed=47, nspin=2),
     }
 
 
@@ -155,6 +157,28 @@ def test_observables(torch_dev, oracle_lib, case):
     assert e_cur <= SERIES_TOL
     assert e_dip <= SERIES_TOL
     h.close()
+
+
+def test_rt_spin2_fixed_potential(torch_dev, oracle_lib):
+    """Spin-polarised fixed-potential run (two channels with their own V_local): J(t) summed over
+    the channels, dipole(t) and N_e(t) of the total density, the final orbitals."""
+    s = system.make_si_system(n=(16, 14, 12), no=3, nkgrid=(1, 1, 2), seed=47, nspin=2, nt=30)
+    print("[gpu] rt spin2 30 steps", flush=True)
+    from salmon2_amd.rt import RTPropagator
+
+    rt = RTPropagator(s)
+    res = rt.run()
+    O = oracle_lib.Oracle(s)
+    psi_ref, cur_ref, dip_ref, rne_ref = O.rt_run()
+    e_cur = rel(res["curr"], cur_ref)
+    e_dip = rel(res["dip"], dip_ref)
+    e_rne = rel(res["rne"], rne_ref)
+    e_psi = rel(res["psi"].cpu().numpy()[..., : s.ny, :], psi_ref[..., : s.ny, :])
+    for k, v in (("current", e_cur), ("dipole", e_dip), ("rNe", e_rne), ("psi", e_psi)):
+        _report("spin2-30", k, v)
+    assert max(e_cur, e_dip, e_rne) <= SERIES_TOL
+    assert e_psi <= HPSI_TOL
+    rt.close()
 
 
 def test_rt_c1_full_run(torch_dev, oracle_lib):
