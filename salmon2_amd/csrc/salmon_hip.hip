@@ -1894,7 +1894,7 @@ struct CurFinArgs {
   const double *veck;      // (3, nk)
   const double *rocc;      // (no, nk, nspin)
   const double *wtk;       // (nk)
-  double *contrib;         // (3, no*nk) scratch
+  double *contrib;         // (3, no*nk, nspin) scratch
   double *curr;            // (3, nspin)
   double ax, ay, az;
   const double *actab;     // with actab: vec_Ac = actab(:, *itt) (Ac_tot at the device step counter)
@@ -1949,7 +1949,7 @@ __device__ __forceinline__ void current_orb_body(const CurFinArgs &a, int bx, in
     for (int d = 0; d < 3; ++d) {
       const double j1 = kA[d] * rt;
       const double j2 = s[d] * 2.0;
-      a.contrib[3 * (size_t)mm + d] = (((j1 + j2) + jw[d] * 2.0) * occ) * a.wtk[ik];
+      a.contrib[3 * ((size_t)mm + (size_t)M * ispin) + d] = (((j1 + j2) + jw[d] * 2.0) * occ) * a.wtk[ik];
     }
   }
 }
@@ -1961,10 +1961,11 @@ __global__ __launch_bounds__(kBlock) void k_current_orb(CurFinArgs a, int ispin)
 __device__ __forceinline__ void current_sum_body(const CurFinArgs &a, int ispin) {
   __shared__ double sh[3 * 1024];
   const int M = a.no * a.nk;
+  const double *cs = a.contrib + 3 * (size_t)M * ispin;  // this spin's contributions
   double w4 = 0.0;
   for (int m0 = 0; m0 < M; m0 += 1024) {
     const int n = min(1024, M - m0);
-    for (int i = threadIdx.x; i < 3 * n; i += kBlock) sh[i] = a.contrib[3 * (size_t)m0 + i];
+    for (int i = threadIdx.x; i < 3 * n; i += kBlock) sh[i] = cs[3 * (size_t)m0 + i];
     __syncthreads();
     if (threadIdx.x < 3)
       for (int mm = 0; mm < n; ++mm) w4 = w4 + sh[3 * mm + threadIdx.x];
@@ -2408,7 +2409,8 @@ int salmon_hip_create(int nx, int ny, int nz, int ly, int nspin, int no, int nk,
   c->dens_ngrp = (no * nk + kDensGroup - 1) / kDensGroup;
   if (c->dens_ngrp > 1 && (rc = dalloc(&c->d_denspart, c->ngrid * nspin * c->dens_ngrp))) return bail(rc);
   if ((rc = dalloc(&c->d_curpart, 4 * (size_t)c->nyt * c->norb))) return bail(rc);
-  if ((rc = dalloc(&c->d_contrib, 3 * (size_t)no * nk))) return bail(rc);
+  // per spin: the batched step forms both spins' orbital contributions in one launch
+  if ((rc = dalloc(&c->d_contrib, 3 * (size_t)no * nk * nspin))) return bail(rc);
   if ((rc = dalloc(&c->d_sink, 1024))) return bail(rc);  // one element per thread of the largest tile
   if ((rc = dalloc(&c->d_dippart, 4 * (size_t)kDipBlocks))) return bail(rc);
   if ((rc = resolve_scratch(&c->d_scratch))) return bail(rc);
