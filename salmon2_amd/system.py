@@ -295,18 +295,30 @@ def projector_spheres(n, hgs, al, rion, rps):
 
 
 def ylm_monomial(x, y, z, l, m):
-    """SALMON's Ylm for l <= 1 (salmon_math.f90:386-426)."""
+    """SALMON's Ylm for l <= 1 (salmon_math.f90:386-426); for l = 2 and 3 real solid-harmonic
+    polynomials without SALMON's normalisation (synthetic tables for the d- and f-channel paths:
+    the GPU and the oracle consume the same tables, so only their shape matters)."""
     if l == 0:
         return np.ones_like(x)
     if l == 1:
         return {-1: -y, 0: z, 1: -x}[m]
-    raise ValueError("only s and p channels are generated")
+    r2 = (x * x + y * y) + z * z
+    if l == 2:
+        return {-2: x * y, -1: y * z, 0: 3.0 * z * z - r2, 1: x * z, 2: x * x - y * y}[m]
+    if l == 3:
+        return {-3: y * (3.0 * x * x - y * y), -2: x * y * z, -1: y * (5.0 * z * z - r2),
+                0: z * (5.0 * z * z - 3.0 * r2), 1: x * (5.0 * z * z - r2), 2: z * (x * x - y * y),
+                3: x * (x * x - 3.0 * y * y)}[m]
+    raise ValueError("channels up to l = 3 are generated")
 
 
-def sp_projectors(mps, nps, rxyz, energies, hvol, rc=1.5):
-    """uv(nps, nlma), ia_tbl, rinv_uvu for s+p channels per atom (set_lma / calc_uv order)."""
+def sp_projectors(mps, nps, rxyz, energies, hvol, rc=1.5, lmax=1):
+    """uv(nps, nlma), ia_tbl, rinv_uvu for the channels l = 0 .. lmax per atom (set_lma / calc_uv
+    order: atoms, then l, then m = -l .. l).  The s and p energies are the seeded ones; the d and
+    f channels (lmax >= 2, synthetic) take smaller energies derived from them."""
     natom = len(mps)
-    nlma = 4 * natom
+    chans = [(l, mm) for l in range(lmax + 1) for mm in range(-l, l + 1)]
+    nlma = len(chans) * natom
     uv = np.zeros((nlma, nps))
     ia_tbl = np.zeros(nlma, dtype=np.int32)
     rinv = np.zeros(nlma)
@@ -316,10 +328,13 @@ def sp_projectors(mps, nps, rxyz, energies, hvol, rc=1.5):
         x, y, z = rxyz[ia, :m, 0], rxyz[ia, :m, 1], rxyz[ia, :m, 2]
         r2 = (x * x + y * y) + z * z
         radial = np.exp(-r2 / (rc * rc))
-        for l, mm in ((0, 0), (1, -1), (1, 0), (1, 1)):
+        for l, mm in chans:
             uv[lma, :m] = radial * ylm_monomial(x, y, z, l, mm)
             ia_tbl[lma] = ia + 1
-            rinv[lma] = energies[ia, l] * hvol
+            # d and f: smaller strengths keep the KB spectrum inside the 4th-order Taylor step's
+            # stable range at dt = 0.08 (their polynomials grow as r^l over the sphere)
+            e = energies[ia, l] if l <= 1 else (0.5 * energies[ia, l % 2] + 0.25 * (l - 1)) * 0.3 ** (l - 1)
+            rinv[lma] = e * hvol
             lma += 1
     return uv, ia_tbl, rinv, nlma
 
@@ -396,6 +411,7 @@ def make_si_system(
     rcut=2.5,
     host_orbitals=True,
     nspin=1,
+    lmax=1,
 ):
     """Seeded diamond-Si supercell of the kind the benchmark configurations name.
 
@@ -417,7 +433,7 @@ def make_si_system(
     energies = rng.uniform(-2.0, 2.0, size=(len(rion), 2))
     if with_projectors:
         mps, nps, jxyz, rxyz = projector_spheres(nn, hgs, al, rion, rcut)
-        uv, ia_tbl, rinv, nlma = sp_projectors(mps, nps, rxyz, energies, hvol, rc)
+        uv, ia_tbl, rinv, nlma = sp_projectors(mps, nps, rxyz, energies, hvol, rc, lmax)
         natom = len(rion)
     else:
         natom, nps, nlma = 0, 0, 0

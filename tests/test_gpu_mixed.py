@@ -50,6 +50,7 @@ CASES = {
     "wide_72x12x16": lambda: system.make_si_system(n=(72, 12, 16), no=3, nkgrid=(1, 1, 1), seed=29),
     "odd_21x18x16": lambda: system.make_si_system(n=(21, 18, 16), no=4, nkgrid=(1, 1, 1), seed=37),
     "spin2_16x14x12": lambda: system.make_si_system(n=(16, 14, 12), no=3, nkgrid=(1, 1, 2), seed=47, nspin=2),
+    "spdf_14cube": lambda: system.make_si_system(n=14, no=5, nkgrid=(1, 1, 2), seed=59, lmax=3),
 }
 
 

@@ -55,6 +55,10 @@ def _cases():
         "odd_21x18x16": lambda: system.make_si_system(n=(21, 18, 16), no=4, nkgrid=(1, 1, 1), seed=37),
         # spin-polarised (collinear): two channels, each with its own V_local
         "spin2_16x14x12": lambda: system.make_si_system(n=(16, 14, 12), no=3, nkgrid=(1, 1, 2), seed=47, nspin=2),
+        # d and f channels: 9 and 16 projectors per atom (the projection, fused KB pass and scatter
+        # templates for more than 4 projectors per atom)
+        "spd_12cube": lambda: system.make_si_system(n=12, no=4, nkgrid=(1, 1, 2), seed=53, lmax=2),
+        "spdf_14cube": lambda: system.make_si_system(n=14, no=5, nkgrid=(1, 1, 2), seed=59, lmax=3),
     }
 
 
@@ -159,11 +163,20 @@ def test_observables(torch_dev, oracle_lib, case):
     h.close()
 
 
-def test_rt_spin2_fixed_potential(torch_dev, oracle_lib):
-    """Spin-polarised fixed-potential run (two channels with their own V_local): J(t) summed over
-    the channels, dipole(t) and N_e(t) of the total density, the final orbitals."""
-    s = system.make_si_system(n=(16, 14, 12), no=3, nkgrid=(1, 1, 2), seed=47, nspin=2, nt=30)
-    print("[gpu] rt spin2 30 steps", flush=True)
+RT_CASES = {
+    # spin-polarised: two channels with their own V_local (J summed over the channels, dipole and
+    # N_e of the total density)
+    "spin2": lambda: system.make_si_system(n=(16, 14, 12), no=3, nkgrid=(1, 1, 2), seed=47, nspin=2, nt=30),
+    # s+p+d+f projectors (16 per atom): the current's moments with the next step's projection
+    "spdf": lambda: system.make_si_system(n=14, no=5, nkgrid=(1, 1, 2), seed=59, lmax=3, nt=30),
+}
+
+
+@pytest.mark.parametrize("name", list(RT_CASES))
+def test_rt_fixed_potential_more_shapes(torch_dev, oracle_lib, name):
+    """30 fixed-potential steps against the oracle: J(t), dipole(t), N_e(t), the final orbitals."""
+    s = RT_CASES[name]()
+    print(f"[gpu] rt {name} 30 steps", flush=True)
     from salmon2_amd.rt import RTPropagator
 
     rt = RTPropagator(s)
@@ -175,7 +188,7 @@ def test_rt_spin2_fixed_potential(torch_dev, oracle_lib):
     e_rne = rel(res["rne"], rne_ref)
     e_psi = rel(res["psi"].cpu().numpy()[..., : s.ny, :], psi_ref[..., : s.ny, :])
     for k, v in (("current", e_cur), ("dipole", e_dip), ("rNe", e_rne), ("psi", e_psi)):
-        _report("spin2-30", k, v)
+        _report(f"{name}-30", k, v)
     assert max(e_cur, e_dip, e_rne) <= SERIES_TOL
     assert e_psi <= HPSI_TOL
     rt.close()

@@ -67,8 +67,9 @@ def test_current_real_orbitals_gamma_zero(oracle_lib):
     assert np.abs(cur).max() == 0.0
 
 
-def test_current_nonlocal_dense(oracle_lib, tiny):
-    s = tiny
+@pytest.mark.parametrize("lmax", [1, 2, 3])
+def test_current_nonlocal_dense(oracle_lib, tiny, lmax):
+    s = tiny if lmax == 1 else system.make_si_system(n=12, no=4, nkgrid=(1, 1, 2), seed=7, lmax=lmax)
     O = oracle_lib.Oracle(s)
     kA = np.array([0.05, -0.02, 0.01])
     O.update_kvector(kA)

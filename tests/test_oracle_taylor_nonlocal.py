@@ -1,7 +1,7 @@
 """Pins the oracle's phase table, zpseudo and Taylor propagation.
 
 * zekr_uV against its closed form conj(exp(i(k+A).r)) uv;
-* zpseudo against a dense rank-nlma operator P diag(rinv) P^H;
+* zpseudo against a dense rank-nlma operator P diag(rinv) P^H (s+p, s+p+d and s+p+d+f channels);
 * taylor against the truncated series sum_n (-i H dt)^n/n! psi of the dense
   Hamiltonian, against scipy's expm to O(dt^5), and its buffer contract.
 """
@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 import scipy.linalg
 
+from salmon2_amd import system
 from tests import dense
 
 
@@ -22,8 +23,11 @@ def test_zekr_closed_form(oracle_lib, tiny):
         assert np.abs(O.zekr[ik] - ref).max() < 1e-15
 
 
-def test_zpseudo_dense(oracle_lib, tiny):
-    s = tiny
+@pytest.mark.parametrize("lmax", [1, 2, 3])
+def test_zpseudo_dense(oracle_lib, tiny, lmax):
+    # s+p (the workloads'), and with d (9 projectors per atom) and f (16) channels
+    s = tiny if lmax == 1 else system.make_si_system(n=12, no=4, nkgrid=(1, 1, 2), seed=7, lmax=lmax)
+    assert s.nlma == (lmax + 1) ** 2 * s.natom
     O = oracle_lib.Oracle(s)
     kA = np.array([0.0, 0.07, -0.02])
     O.update_kvector(kA)
