@@ -4,10 +4,11 @@ streams, device time index (salmon_hip_batch_*, salmon2_amd/batch.py).
 Every point's arithmetic is the same as in the single-system path (only the tiling of the
 work differs), so the batched run reproduces each system's own run to round-off (bound
 1e-13 relative); the autotuned shapes keep the results; systems are checked against the
-oracle (1e-9, the time-series bar).  Two sets: five hand-picked uneven systems (one of them
-spin-polarised; it and the smallest also against the oracle), and the 24 seeded systems of the
-batched workload (grids 16..40 per axis, 8..128 atoms, 16..256 orbitals, 1..27 k-points), every
-one against its own single-system run and the two smallest against the oracle.  Stepping past the field tables is refused on the host.
+oracle (1e-9, the time-series bar).  Two sets: six hand-picked uneven systems (one of them
+spin-polarised, one with s+p+d projectors; these two and the smallest also against the
+oracle), and the 24 seeded systems of the batched workload (grids 16..40 per axis, 8..128
+atoms, 16..256 orbitals, 1..27 k-points), every one against its own single-system run and the
+two smallest against the oracle.  Stepping past the field tables is refused on the host.
 
 Graph capture of the batched step is switched off in the library (a capture once ended in
 a host segmentation fault whose cause was not found); its replay test is skipped.
@@ -45,6 +46,8 @@ def _systems():
         system.make_si_system(ncell=(1, 1, 1), n=(40, 36, 32), no=3, seed=23, nt=NT),
         # spin-polarised member: two channels with their own V_local
         system.make_si_system(n=(16, 14, 12), no=3, nkgrid=(1, 1, 2), seed=47, nspin=2, nt=NT),
+        # s+p+d projectors (9 per atom): the packed KB launches take the largest member's template
+        system.make_si_system(n=12, no=4, nkgrid=(1, 1, 2), seed=53, lmax=2, nt=NT),
     ]
 
 
@@ -80,7 +83,7 @@ def test_batch_eager_autotune(torch_dev, oracle_lib):
 
     systems = _systems()
     singles = _singles(systems, [None] * len(systems), NT)
-    print("[gpu] batch of 5 uneven systems, eager", flush=True)
+    print("[gpu] batch of 6 uneven systems, eager", flush=True)
     br = BatchRunner(systems, NT)
     br.run(NT)
     eager = br.results(NT)
@@ -99,6 +102,7 @@ def test_batch_eager_autotune(torch_dev, oracle_lib):
     br.close()
     _vs_oracle(oracle_lib, "tiny", systems[0], systems[0].psi, eager[0], NT)
     _vs_oracle(oracle_lib, "spin2", systems[4], systems[4].psi, eager[4], NT)
+    _vs_oracle(oracle_lib, "spd", systems[5], systems[5].psi, eager[5], NT)
 
 
 def test_batch_24_seeded_systems(torch_dev, oracle_lib):
