@@ -30,9 +30,18 @@ def _exe():
     return build.DRIVER_EXE
 
 
-@pytest.mark.parametrize("name", ["tiny", "c1"])
+CHECK_CASES = {
+    "tiny": lambda: system.config("tiny"),
+    "c1": lambda: system.config("c1"),
+    # spin-polarised (V_local(ispin) per channel) and s+p+d projectors through the same wrappers
+    "spin2": lambda: system.make_si_system(n=(16, 14, 12), no=3, nkgrid=(1, 1, 2), seed=47, nspin=2),
+    "spd": lambda: system.make_si_system(n=12, no=4, nkgrid=(1, 1, 2), seed=53, lmax=2),
+}
+
+
+@pytest.mark.parametrize("name", list(CHECK_CASES))
 def test_fortran_dropin_matches_oracle(oracle_lib, tmp_path, name):
-    s = system.config(name)
+    s = CHECK_CASES[name]()
     A = np.array([0.011, -0.007, 0.004])
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     io.write_input(fin, s, A)
