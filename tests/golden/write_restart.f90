@@ -3,21 +3,26 @@
 ! unformatted, wfn.bin stream access in (ix,iy,iz,is,io,ik) order, Vh_stock.bin two
 ! sequential records, rtdata.bin the three records of write_rtdata), with values given by
 ! closed formulas the test recomputes.
-! Usage: write_restart <directory> [real]
+! Usage: write_restart <directory> [real|complex [nspin]]
 program write_restart
   implicit none
-  integer, parameter :: nx = 5, ny = 4, nz = 3, nspin = 1, no = 3, nk = 2
-  complex(8) :: zwf(nx, ny, nz, nspin, no, nk)
-  real(8) :: rwf(nx, ny, nz, nspin, no, nk)
-  real(8) :: rocc(no, nk, nspin), vh1(nx, ny, nz), vh2(nx, ny, nz)
+  integer, parameter :: nx = 5, ny = 4, nz = 3, no = 3, nk = 2
+  integer :: nspin
+  complex(8), allocatable :: zwf(:, :, :, :, :, :)
+  real(8), allocatable :: rwf(:, :, :, :, :, :), rocc(:, :, :)
+  real(8) :: vh1(nx, ny, nz), vh2(nx, ny, nz)
   real(8) :: ac_tot(3, 0:1236), ac_ind(3, 0:1236)
   integer :: ix, iy, iz, is, io, ik, itt
-  character(256) :: dir, mode
+  character(256) :: dir, mode, arg3
   logical :: if_real
 
   call get_command_argument(1, dir)
   call get_command_argument(2, mode)
+  call get_command_argument(3, arg3)
   if_real = trim(mode) == 'real'
+  nspin = 1
+  if (len_trim(arg3) > 0) read(arg3, *) nspin
+  allocate(zwf(nx, ny, nz, nspin, no, nk), rwf(nx, ny, nz, nspin, no, nk), rocc(no, nk, nspin))
   do ik = 1, nk
   do io = 1, no
   do is = 1, nspin
@@ -29,7 +34,7 @@ program write_restart
   end do
   end do
   end do
-  rocc(io, ik, is) = 2d0 - 0.1d0*io - 0.01d0*ik
+  rocc(io, ik, is) = 2d0 - 0.1d0*io - 0.01d0*ik - 0.001d0*(is - 1)
   end do
   end do
   end do

@@ -35,40 +35,43 @@ def fortran_dirs(tmp_path_factory):
         pytest.skip(f"flang could not build the writer: {r.stderr[-300:]}")
     out = {}
     for mode in ("complex", "real"):
-        p = d / mode
-        p.mkdir()
-        subprocess.run([exe, str(p)] + (["real"] if mode == "real" else []), check=True)
-        out[mode] = str(p)
+        for nspin in (1, 2):
+            p = d / f"{mode}{nspin}"
+            p.mkdir()
+            subprocess.run([exe, str(p), mode, str(nspin)], check=True)
+            out[mode, nspin] = str(p)
     return out
 
 
-def _closed_form(real):
+def _closed_form(real, nspin=1):
     nx, ny, nz = N
-    ik, io, iz, iy, ix = np.meshgrid(np.arange(1, NK + 1), np.arange(1, NO + 1), np.arange(1, nz + 1),
-                                     np.arange(1, ny + 1), np.arange(1, nx + 1), indexing="ij")
+    ik, io, isp, iz, iy, ix = np.meshgrid(np.arange(1, NK + 1), np.arange(1, NO + 1), np.arange(1, nspin + 1),
+                                          np.arange(1, nz + 1), np.arange(1, ny + 1), np.arange(1, nx + 1),
+                                          indexing="ij")
     if real:
-        v = ix - 2 * iy + 3 * iz + 0.125 * io - 7 * ik + 0j
+        psi = ix - 2 * iy + 3 * iz + 0.125 * io - 7 * ik + 0j
     else:
-        v = (ix + 10 * iy + 100 * iz + 0.5) + 1j * (io + 10 * ik + 0.25 * (ix - iy))
-    psi = v[:, :, None]
-    rocc = np.array([[[2.0 - 0.1 * io - 0.01 * ik for io in range(1, NO + 1)] for ik in range(1, NK + 1)]])
+        psi = (ix + 10 * iy + 100 * iz + 0.5 * isp) + 1j * (io + 10 * ik + 0.25 * (ix - iy))
+    rocc = np.array([[[2.0 - 0.1 * io - 0.01 * ik - 0.001 * (s - 1) for io in range(1, NO + 1)]
+                      for ik in range(1, NK + 1)] for s in range(1, nspin + 1)])
     z, y, x = np.meshgrid(np.arange(1, nz + 1), np.arange(1, ny + 1), np.arange(1, nx + 1), indexing="ij")
     vh1 = 0.5 * x - y + 0.25 * z
     return psi, rocc, (vh1, -vh1 + 1.0 / 3.0)
 
 
+@pytest.mark.parametrize("nspin", [1, 2])
 @pytest.mark.parametrize("mode", ["complex", "real"])
-def test_reads_fortran_written_restart(fortran_dirs, mode):
-    d = R.read_restart(fortran_dirs[mode], N, nspin=1, ly=N[1] + 1)
-    psi, rocc, vh = _closed_form(mode == "real")
+def test_reads_fortran_written_restart(fortran_dirs, mode, nspin):
+    d = R.read_restart(fortran_dirs[mode, nspin], N, nspin=nspin, ly=N[1] + 1)
+    psi, rocc, vh = _closed_form(mode == "real", nspin)
     assert d["iter"] == 1234 and d["nprocs"] == 4 and d["if_real_orbital"] == (mode == "real")
     assert (d["mk"], d["mo"]) == (NK, NO)
     assert np.array_equal(d["psi"][..., : N[1], :], psi)
     assert not d["psi"][..., N[1], :].any()  # the padding row stays zero
     assert np.array_equal(d["rocc"], rocc)
-    v1, v2 = R.read_vh_stock(fortran_dirs[mode], N)
+    v1, v2 = R.read_vh_stock(fortran_dirs[mode, nspin], N)
     assert np.array_equal(v1, vh[0]) and np.array_equal(v2, vh[1])
-    ac_tot, ac_ind, ac_ind0 = R.read_rtdata(fortran_dirs[mode])
+    ac_tot, ac_ind, ac_ind0 = R.read_rtdata(fortran_dirs[mode, nspin])
     ref_tot, ref_ind = _rtdata_closed_form()
     assert np.array_equal(ac_tot, ref_tot[1234:1236]) and np.array_equal(ac_ind, ref_ind[1234:1236])
     assert np.array_equal(ac_ind0, ref_ind[0])
@@ -80,14 +83,15 @@ def _rtdata_closed_form():
     return 1e-3 * t + 0.1 * d, -2e-4 * t + 0.01 * d * d
 
 
+@pytest.mark.parametrize("nspin", [1, 2])
 @pytest.mark.parametrize("mode", ["complex", "real"])
-def test_writer_reproduces_fortran_bytes(fortran_dirs, mode, tmp_path):
-    psi, rocc, vh = _closed_form(mode == "real")
+def test_writer_reproduces_fortran_bytes(fortran_dirs, mode, nspin, tmp_path):
+    psi, rocc, vh = _closed_form(mode == "real", nspin)
     R.write_checkpoint(str(tmp_path), psi, N[1], rocc, 1234, nprocs=4, if_real_orbital=(mode == "real"), vh_stock=vh)
     tot, ind = _rtdata_closed_form()
     R.write_rtdata(str(tmp_path), tot[1234:1236], ind[1234:1236], ind[0])
     for name in ("info.bin", "occupation.bin", "wfn.bin", "Vh_stock.bin", "rtdata.bin"):
-        a = open(os.path.join(fortran_dirs[mode], name), "rb").read()
+        a = open(os.path.join(fortran_dirs[mode, nspin], name), "rb").read()
         b = open(os.path.join(tmp_path, name), "rb").read()
         assert a == b, name
 
