@@ -549,7 +549,8 @@ def main():
     avg_ms = (k1["avg_ms"] * k1["launches"] + km["avg_ms"] * km["launches"]) / max(nl, 1)
     achieved = alg_bytes / (avg_ms / 1e3) / 1e9
     ty = min(s.ny, 256 // s.nx)
-    kname = f"k_stencilz<d2,double,d2,{2 if (ty + 8) * s.nx <= 512 else 3}>"
+    need = (ty + 8) * s.nx  # slot elements: 64-element chunks of the delayed-z kernel
+    kname = f"k_stencilz<d2,double,d2,{8 if need <= 512 else (10 if need <= 640 else 12)}>"
     ceil = copy_ceiling()
     out = {
         "metric": METRIC,

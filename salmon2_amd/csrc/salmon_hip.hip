@@ -1273,16 +1273,21 @@ __device__ __forceinline__ void stencil_tile8(const StencilArgs &a, int bid, dou
 // values (planes p-8 .. p) and completes plane p-4 with its z part.  The centre, x/y and z
 // partial sums are added in that order (hh = ((V t + klap0 t) - v_xy/2 + i w_xy) - v_z/2 + i w_z),
 // a regrouping of stencil_tile's sum.  Needs TZ % 4 == 0; the z loop runs TZ + 4 planes,
-// unrolled by 4 so that slot and queue indices are compile-time.
+// unrolled by 4 so that slot and queue indices are compile-time.  A slot holds NCK chunks of 64
+// elements (8: rows of up to 32 points; 10: 13 rows of 48 points, so that the 48-point rows keep
+// three blocks per CU; 12: up to 64 points); chunk c is moved by wave c % 4, and a wave with no
+// chunk left for its last DMA of a plane moves its previous chunk again (the same data to the
+// same place), so that every wave issues the same number of operations per plane.
 // ---------------------------------------------------------------------------
-template <typename TI, typename R, typename TO, int NCH>
+template <typename TI, typename R, typename TO, int NCK>
 __device__ __forceinline__ void stencil_tilez(const StencilArgs &a, int bid, double2 *lds) {
   using CH = typename Cx<R>::T;
   constexpr int BS = kTile;
   constexpr int WI = DmaW<TI>::W;
   constexpr unsigned EB = sizeof(TI);
-  constexpr int SLOT = NCH * BS;              // elements per slot: (TY + 8) * nx <= NCH * 256
-  constexpr int kOps = NCH * WI + 2 + 1;
+  constexpr int SLOT = 64 * NCK;              // elements per slot: (TY + 8) * nx <= 64 NCK
+  constexpr int NW = (NCK + 3) / 4;           // tile DMA chunks per wave and plane
+  constexpr int kOps = NW * WI + 2 + 1;
   TI *tileR = (TI *)lds;                      // 4 x SLOT
   double *vR = (double *)(tileR + 4 * SLOT);  // 4 x BS
 
@@ -1314,12 +1319,16 @@ __device__ __forceinline__ void stencil_tilez(const StencilArgs &a, int bid, dou
     const int ec = e < nslot ? e : 0;
     return (size_t)(ec % nx) + (size_t)nx * wrapi(y0 - 4 + ec / nx, ny);
   };
-  size_t toff[NCH][2];
+  size_t toff[NW][2];
+  unsigned kofs[NW];  // slot offset (elements) of the chunk of this wave's k-th DMA, wave-uniform
 #pragma unroll
-  for (int k = 0; k < NCH; ++k)
+  for (int k = 0; k < NW; ++k) {
+    const int kc = wave + 4 * k < NCK ? k : k - 1;
+    kofs[k] = __builtin_amdgcn_readfirstlane(256u * (unsigned)kc);
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk)
-      toff[k][kk] = sofs(64 * (wave + 4 * k) + (WI == 1 ? lane : 32 * kk + (lane >> 1)));
+      toff[k][kk] = sofs(64 * (wave + 4 * kc) + (WI == 1 ? lane : 32 * kk + (lane >> 1)));
+  }
   size_t voff[2];
 #pragma unroll
   for (int kk = 0; kk < 2; ++kk) {
@@ -1333,7 +1342,7 @@ __device__ __forceinline__ void stencil_tilez(const StencilArgs &a, int bid, dou
   auto dma_tile = [&](int plane, int slot) {
     const TI *src = in + zs * wrapi(plane, nz);
 #pragma unroll
-    for (int k = 0; k < NCH; ++k) dma_chunk<TI>(src, toff[k], tileA + EB * (unsigned)(SLOT * slot + 4 * 64 * k), lane);
+    for (int k = 0; k < NW; ++k) dma_chunk<TI>(src, toff[k], tileA + EB * (unsigned)(SLOT * slot) + EB * kofs[k], lane);
   };
   auto dma_v = [&](int plane, int slot) {
     const float *vb = (const float *)Vb + 2 * gplane * wrapi(plane, nz);
@@ -1441,21 +1450,21 @@ __device__ __forceinline__ void stencil_tilez(const StencilArgs &a, int bid, dou
   wait_vm<0>();
 }
 
-template <typename TI, typename R, typename TO, int NCH>
+template <typename TI, typename R, typename TO, int NCK>
 __global__ __launch_bounds__(kTile) void k_stencilz(StencilArgs a) {
   extern __shared__ double2 lds[];
   if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
   int bid = blockIdx.x;
   const int nfull = (a.nblk / 8) * 8;
   if (bid < nfull) bid = (bid % 8) * (a.nblk / 8) + bid / 8;  // XCD-aware tile order
-  stencil_tilez<TI, R, TO, NCH>(a, bid, lds);
+  stencil_tilez<TI, R, TO, NCK>(a, bid, lds);
 }
 
 // Persistent form over many systems (the batched step's terms without the Taylor sum): block b
 // takes tiles b, b + grid, ... of `tiles` = (system index, tile index).  stencil_tilez returns
 // with its DMAs drained (vmcnt 0); the barrier keeps the next tile's DMAs off slots that other
 // waves still read.
-template <typename TI, typename R, typename TO, int NCH>
+template <typename TI, typename R, typename TO, int NCK>
 __global__ __launch_bounds__(kTile) void k_stencilz_packed(const StencilArgs *__restrict__ sys,
                                                            const int2 *__restrict__ tiles, int ntiles) {
   extern __shared__ double2 lds[];
@@ -1464,7 +1473,7 @@ __global__ __launch_bounds__(kTile) void k_stencilz_packed(const StencilArgs *__
     const int si = __builtin_amdgcn_readfirstlane(tr.x);
     const int bid = __builtin_amdgcn_readfirstlane(tr.y);
     const StencilArgs a = sys[si];
-    stencil_tilez<TI, R, TO, NCH>(a, bid, lds);
+    stencil_tilez<TI, R, TO, NCK>(a, bid, lds);
     __syncthreads();
   }
 }
@@ -2872,30 +2881,39 @@ static bool tile8_pairs(const StencilArgs &a) { return a.nx % 2 == 0 && a.osz % 
 
 // y tiles with the delayed z part (stencil_tilez): the FP64 terms without the Taylor sum and the
 // mixed first term (FP64 input), z chunks of 4k planes; a plane's tile and halo rows fill a slot
-// of two 256-element chunks (rows of up to 32 points) or three (up to 64 points: 48-point rows
-// take 5 tile rows, 13 x 48 = 624 elements).
+// of 8, 10 or 12 64-element chunks (rows of up to 32 points; 48-point rows take 5 tile rows,
+// 13 x 48 = 624 elements in 10 chunks; up to 64 points).
 static bool tilez_fits(const salmon_hip_ctx *c, int stage, int nc, const StencilArgs &a) {
   return ((stage == kStagePlain && nc < 0) || stage == kStageMixedFirst) && a.nx <= 64 &&
          (a.TY + 8) * a.nx <= 3 * kTile && a.TY * a.nx <= kTile && a.TZ % 4 == 0 && a.nz % 4 == 0 && c->tilez;
 }
 
-template <typename TI, typename R, typename TO, int NCH>
+template <typename TI, typename R, typename TO, int NCK>
 static int launch_tilez_n(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
-  const size_t shmem = sizeof(TI) * 4 * NCH * kTile + sizeof(double) * 4 * kTile;
+  const size_t shmem = sizeof(TI) * 4 * 64 * NCK + sizeof(double) * 4 * kTile;
   static bool attr_set = false;
   if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencilz<TI, R, TO, NCH>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencilz<TI, R, TO, NCK>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)shmem));
     attr_set = true;
   }
-  hipExtLaunchKernelGGL((k_stencilz<TI, R, TO, NCH>), grid, dim3(kTile), shmem, c->stream, e0, e1, 0, a);
+  hipExtLaunchKernelGGL((k_stencilz<TI, R, TO, NCK>), grid, dim3(kTile), shmem, c->stream, e0, e1, 0, a);
   return 0;
+}
+
+// 64-element chunks per delayed-z slot for (TY + 8) rows of nx points: 8, 10 or 12
+static int tilez_chunks(int TY, int nx) {
+  const int need = (TY + 8) * nx;
+  return need <= 8 * 64 ? 8 : (need <= 10 * 64 ? 10 : 12);
 }
 
 template <typename TI, typename R, typename TO>
 static int launch_tilez_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
-  if ((a.TY + 8) * a.nx <= 2 * kTile) return launch_tilez_n<TI, R, TO, 2>(c, grid, e0, e1, a);
-  return launch_tilez_n<TI, R, TO, 3>(c, grid, e0, e1, a);
+  switch (tilez_chunks(a.TY, a.nx)) {
+    case 8: return launch_tilez_n<TI, R, TO, 8>(c, grid, e0, e1, a);
+    case 10: return launch_tilez_n<TI, R, TO, 10>(c, grid, e0, e1, a);
+    default: return launch_tilez_n<TI, R, TO, 12>(c, grid, e0, e1, a);
+  }
 }
 
 template <typename TI, typename R, typename TO, int NCH, bool PR>
@@ -3662,25 +3680,28 @@ int launch_packed(hipStream_t st, int hc, int nc, const StencilArgs *sys, const 
   return launch_packed_hc<8>(st, nc, sys, tiles, ntiles, grid);
 }
 
-template <int NCH>
+template <int NCK>
 int launch_packed_z_t(hipStream_t st, const StencilArgs *sys, const int2 *tiles, int ntiles, int grid) {
-  const size_t shmem = sizeof(double2) * 4 * NCH * kTile + sizeof(double) * 4 * kTile;
+  const size_t shmem = sizeof(double2) * 4 * 64 * NCK + sizeof(double) * 4 * kTile;
   static bool attr_set = false;
   if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencilz_packed<double2, double, double2, NCH>,
+    HIPCHK(hipFuncSetAttribute((const void *)k_stencilz_packed<double2, double, double2, NCK>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_stencilz_packed<double2, double, double2, NCH>), dim3(grid), dim3(kTile), shmem, st, sys,
+  hipLaunchKernelGGL((k_stencilz_packed<double2, double, double2, NCK>), dim3(grid), dim3(kTile), shmem, st, sys,
                      tiles, ntiles);
   LAUNCHCHK("k_stencilz_packed");
   return 0;
 }
 
-// the delayed-z y tiles for the terms without the Taylor sum, slots of nch x 256 elements
-int launch_packed_z(hipStream_t st, int nch, const StencilArgs *sys, const int2 *tiles, int ntiles, int grid) {
-  return nch == 2 ? launch_packed_z_t<2>(st, sys, tiles, ntiles, grid)
-                  : launch_packed_z_t<3>(st, sys, tiles, ntiles, grid);
+// the delayed-z y tiles for the terms without the Taylor sum, slots of nck 64-element chunks
+int launch_packed_z(hipStream_t st, int nck, const StencilArgs *sys, const int2 *tiles, int ntiles, int grid) {
+  switch (nck) {
+    case 8: return launch_packed_z_t<8>(st, sys, tiles, ntiles, grid);
+    case 10: return launch_packed_z_t<10>(st, sys, tiles, ntiles, grid);
+    default: return launch_packed_z_t<12>(st, sys, tiles, ntiles, grid);
+  }
 }
 
 }  // namespace
@@ -3809,7 +3830,7 @@ struct salmon_hip_batch {
     int2 *d_tiles[2] = {nullptr, nullptr};  // middle terms, last term (different z-chunking)
     int ntiles[2] = {0, 0}, grid[2] = {0, 0};
     StencilArgs *d_args = nullptr;      // [parity][term][member]
-    int zch = 0;                        // middle terms on the delayed-z kernel, slots of zch x 256 (0: no)
+    int zch = 0;                        // middle terms on the delayed-z kernel, slots of zch 64-element chunks (0: no)
   };
   std::vector<Group> groups;
   int n_hamil = 0;
@@ -3864,14 +3885,15 @@ void batch_buffers(salmon_hip_batch *b, int i, int p, void **hh, void **out) {
 int batch_tables(salmon_hip_batch *b) {
   for (auto &g : b->groups) {
     // the terms without the Taylor sum run on the delayed-z tiles when every member's rows fit
-    // its slots (nx <= 64: zch = hc + 1 chunks hold the TY + 8 rows) and its z extent is a
-    // multiple of 4; z chunks are then rounded up to whole multiples of 4 planes
-    const int zch = g.hc + 1;
+    // their slots (nx <= 64; the group takes its largest member's chunk count) and its z extent
+    // is a multiple of 4; z chunks are then rounded up to whole multiples of 4 planes
+    int zch = 0;
     bool zok = g.hc <= 2;
     for (int i : g.mem) {
       const salmon_hip_t c = b->h[i];
       zok = zok && c->tilez && c->nx <= 64 && c->nz % 4 == 0 && c->TY * c->nx <= kTile &&
-            (c->TY + 8) * c->nx <= zch * kTile;
+            (c->TY + 8) * c->nx <= 3 * kTile;
+      zch = std::max(zch, tilez_chunks(c->TY, c->nx));
     }
     g.zch = zok ? zch : 0;
     std::vector<int2> tiles[2];
