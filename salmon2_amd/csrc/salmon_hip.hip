@@ -26,6 +26,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdarg>
 #include <cstdio>
@@ -2816,15 +2817,23 @@ static size_t stencil_lds_bytes() {
          sizeof(double) * kRingH * BS;
 }
 
+// The dynamic-LDS limit of kernel f (hipFuncAttributeMaxDynamicSharedMemorySize), set once per
+// device: the attribute belongs to the kernel on the device that is current when it is set.
+static int lds_attr(const void *f, std::atomic<unsigned long long> &done, size_t bytes) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  const unsigned long long bit = 1ull << (dev & 63);
+  if (done.load(std::memory_order_relaxed) & bit) return 0;
+  HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  done.fetch_or(bit);
+  return 0;
+}
+
 template <typename TI, typename R, typename TO, int HC, int NC, int BS = kTile>
 static int launch_stencil_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
   const size_t shmem = stencil_lds_bytes<TI, HC, NC, BS>();
-  static bool attr_set = false;
-  if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencil<TI, R, TO, HC, NC, BS>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    attr_set = true;
-  }
+  static std::atomic<unsigned long long> attr_dev{0};
+  if (int rc = lds_attr((const void *)k_stencil<TI, R, TO, HC, NC, BS>, attr_dev, shmem)) return rc;
   hipExtLaunchKernelGGL((k_stencil<TI, R, TO, HC, NC, BS>), grid, dim3(BS), shmem, c->stream, e0, e1, 0, a);
   return 0;
 }
@@ -2849,12 +2858,8 @@ template <typename TI, typename R, typename TO, int PY>
 static int launch_plane_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
   constexpr int BS = 512, D = kPlaneDepth;
   const size_t shmem = sizeof(TI) * (5 + D) * PY * BS + sizeof(double) * (D + 1) * PY * BS;
-  static bool attr_set = false;
-  if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencil_plane<TI, R, TO, PY, BS, D>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    attr_set = true;
-  }
+  static std::atomic<unsigned long long> attr_dev{0};
+  if (int rc = lds_attr((const void *)k_stencil_plane<TI, R, TO, PY, BS, D>, attr_dev, shmem)) return rc;
   hipExtLaunchKernelGGL((k_stencil_plane<TI, R, TO, PY, BS, D>), grid, dim3(BS), shmem, c->stream, e0, e1, 0, a);
   return 0;
 }
@@ -2891,12 +2896,8 @@ static bool tilez_fits(const salmon_hip_ctx *c, int stage, int nc, const Stencil
 template <typename TI, typename R, typename TO, int NCK>
 static int launch_tilez_n(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
   const size_t shmem = sizeof(TI) * 4 * 64 * NCK + sizeof(double) * 4 * kTile;
-  static bool attr_set = false;
-  if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencilz<TI, R, TO, NCK>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)shmem));
-    attr_set = true;
-  }
+  static std::atomic<unsigned long long> attr_dev{0};
+  if (int rc = lds_attr((const void *)k_stencilz<TI, R, TO, NCK>, attr_dev, shmem)) return rc;
   hipExtLaunchKernelGGL((k_stencilz<TI, R, TO, NCK>), grid, dim3(kTile), shmem, c->stream, e0, e1, 0, a);
   return 0;
 }
@@ -2919,12 +2920,8 @@ static int launch_tilez_t(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_
 template <typename TI, typename R, typename TO, int NCH, bool PR>
 static int launch_tile8_n(salmon_hip_ctx *c, dim3 grid, hipEvent_t e0, hipEvent_t e1, const StencilArgs &a) {
   const size_t shmem = sizeof(TI) * 8 * NCH * kTile + sizeof(double) * 4 * kTile;
-  static bool attr_set = false;
-  if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencil8<TI, R, TO, NCH, PR>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    attr_set = true;
-  }
+  static std::atomic<unsigned long long> attr_dev{0};
+  if (int rc = lds_attr((const void *)k_stencil8<TI, R, TO, NCH, PR>, attr_dev, shmem)) return rc;
   hipExtLaunchKernelGGL((k_stencil8<TI, R, TO, NCH, PR>), grid, dim3(kTile), shmem, c->stream, e0, e1, 0, a);
   return 0;
 }
@@ -3648,12 +3645,9 @@ __global__ void k_set_itt(int *itt, int v) { *itt = v; }
 template <int HC, int NC>
 int launch_packed_t(hipStream_t st, const StencilArgs *sys, const int2 *tiles, int ntiles, int grid) {
   const size_t shmem = stencil_lds_bytes<double2, HC, NC, kTile>();
-  static bool attr_set = false;
-  if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencil_packed<double2, double, double2, HC, NC, kTile>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    attr_set = true;
-  }
+  static std::atomic<unsigned long long> attr_dev{0};
+  if (int rc = lds_attr((const void *)k_stencil_packed<double2, double, double2, HC, NC, kTile>, attr_dev, shmem))
+    return rc;
   hipLaunchKernelGGL((k_stencil_packed<double2, double, double2, HC, NC, kTile>), dim3(grid), dim3(kTile), shmem, st,
                      sys, tiles, ntiles);
   LAUNCHCHK("k_stencil_packed");
@@ -3683,12 +3677,9 @@ int launch_packed(hipStream_t st, int hc, int nc, const StencilArgs *sys, const 
 template <int NCK>
 int launch_packed_z_t(hipStream_t st, const StencilArgs *sys, const int2 *tiles, int ntiles, int grid) {
   const size_t shmem = sizeof(double2) * 4 * 64 * NCK + sizeof(double) * 4 * kTile;
-  static bool attr_set = false;
-  if (!attr_set) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_stencilz_packed<double2, double, double2, NCK>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    attr_set = true;
-  }
+  static std::atomic<unsigned long long> attr_dev{0};
+  if (int rc = lds_attr((const void *)k_stencilz_packed<double2, double, double2, NCK>, attr_dev, shmem))
+    return rc;
   hipLaunchKernelGGL((k_stencilz_packed<double2, double, double2, NCK>), dim3(grid), dim3(kTile), shmem, st, sys,
                      tiles, ntiles);
   LAUNCHCHK("k_stencilz_packed");
