@@ -80,6 +80,8 @@ __device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((unsign
 // the tap's neighbour and one of weights.  The input is [up(src1), src2] (channel
 // concatenation, src1 at half resolution when up1).
 // ---------------------------------------------------------------------------
+constexpr size_t kConvZero = 128 * 128 + 64;  // bf16 zeros read by k_conv (co x ci up to 128 x 128)
+
 struct ConvArgs {
   const bf16_t *src1, *src2;
   int c1, c2, up1;
@@ -89,6 +91,7 @@ struct ConvArgs {
   float *out_f;         // [npts][co] raw
   int co, relu;
   int nx, ny, nz, npts;
+  const bf16_t *zero;   // zero-filled, at least co * (c1 + c2) + 8 elements
 };
 
 template <int NT>
@@ -107,33 +110,37 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   f4v acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
-  for (int f0 = 0; f0 < K; f0 += 32) {
+  // the operands of K step f0 (32 wide: lane group kq takes f0 + 8 kq .. + 7)
+  // (branch-free: what lies outside the point range or past K is read from a zero buffer, so
+  // every lane issues the same loads and each wait can sit at the MFMA that uses the data)
+  auto load = [&](int f0, s8v &av, s8v (&bv)[NT]) __attribute__((always_inline)) {
     const int f = f0 + 8 * kq;
-    s8v av = s8v{0, 0, 0, 0, 0, 0, 0, 0};
-    s8v bv[NT];
-    if (f < K) {
-      const int tap = f / ci, c = f - tap * ci;
-      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-      const int xn = (x + dx + nx) % nx, yn = (y + dy + ny) % ny, zn = (z + dz + nz) % nz;
-      if (pv) {
-        if (c < a.c1) {
-          const size_t ph = a.up1 ? (size_t)(xn >> 1) + (size_t)(nx >> 1) * ((size_t)(yn >> 1) + (size_t)(ny >> 1) * (zn >> 1))
-                                  : (size_t)xn + (size_t)nx * ((size_t)yn + (size_t)ny * zn);
-          av = *(const s8v *)(a.src1 + ph * a.c1 + c);
-        } else {
-          const size_t pn = (size_t)xn + (size_t)nx * ((size_t)yn + (size_t)ny * zn);
-          av = *(const s8v *)(a.src2 + pn * a.c2 + (c - a.c1));
-        }
-      }
-      const bf16_t *wt = a.w + (size_t)tap * a.co * ci + c;
+    const bool fv = f < K;
+    const int fc = fv ? f : 0;
+    const int tap = fc / ci, c = fc - tap * ci;
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int xn = (x + dx + nx) % nx, yn = (y + dy + ny) % ny, zn = (z + dz + nz) % nz;
+    const size_t pn = (size_t)xn + (size_t)nx * ((size_t)yn + (size_t)ny * zn);
+    const size_t ph = a.up1 ? (size_t)(xn >> 1) + (size_t)(nx >> 1) * ((size_t)(yn >> 1) + (size_t)(ny >> 1) * (zn >> 1))
+                            : pn;
+    const bf16_t *src = c < a.c1 ? a.src1 + ph * a.c1 + c : a.src2 + pn * a.c2 + (c - a.c1);
+    av = *(const s8v *)((fv && pv) ? src : a.zero);
+    const bf16_t *wt = fv ? a.w + (size_t)tap * a.co * ci + c : a.zero;
 #pragma unroll
-      for (int t = 0; t < NT; ++t) bv[t] = *(const s8v *)(wt + (size_t)(t * 16 + row) * ci);
-    } else {
+    for (int t = 0; t < NT; ++t) bv[t] = *(const s8v *)(wt + (size_t)(t * 16 + row) * ci);
+  };
+  // two operand sets: the loads of step f0 + 32 are in flight while the MFMAs of step f0 run
+  // (the accumulation order over f0 is unchanged)
+  s8v avA, avB, bvA[NT], bvB[NT];
+  load(0, avA, bvA);
+  for (int f0 = 0; f0 < K; f0 += 64) {
+    if (f0 + 32 < K) load(f0 + 32, avB, bvB);
 #pragma unroll
-      for (int t = 0; t < NT; ++t) bv[t] = s8v{0, 0, 0, 0, 0, 0, 0, 0};
-    }
+    for (int t = 0; t < NT; ++t) acc[t] = mfma32(avA, bvA[t], acc[t]);
+    if (f0 + 32 >= K) break;
+    if (f0 + 64 < K) load(f0 + 64, avA, bvA);
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = mfma32(av, bv[t], acc[t]);
+    for (int t = 0; t < NT; ++t) acc[t] = mfma32(avB, bvB[t], acc[t]);
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t)
@@ -505,6 +512,7 @@ struct salmon_unet {
   bool packed = false;
   // activations (bf16): x0, e1a, e1b, p1, e2a, e2b, p2, e3a, e3b, d2a, d2b, d1a, d1b
   bf16_t *x0 = nullptr, *act[12] = {};
+  bf16_t *d_zero = nullptr;  // kConvZero zeros: the operands of k_conv outside the grid or past K
   // backward work
   float *d_out = nullptr, *d_dout = nullptr, *d_part = nullptr, *d_fx = nullptr, *d_fy = nullptr, *d_wpart = nullptr;
   bf16_t *g_a = nullptr, *g_b = nullptr;
@@ -534,6 +542,9 @@ int launch_conv(salmon_unet *u, const bf16_t *s1, int c1, int up1, const bf16_t 
   a.w = w; a.bias = bias; a.out_bf = out_bf; a.out_f = out_f; a.co = co; a.relu = relu;
   a.nx = u->n[level][0]; a.ny = u->n[level][1]; a.nz = u->n[level][2];
   a.npts = (int)u->np[level];
+  a.zero = u->d_zero;
+  if ((size_t)co * (c1 + c2) + 8 > kConvZero)
+    return ufail(SALMON_HIP_EUNSUPPORTED, "conv %d -> %d channels above the zero operand", c1 + c2, co);
   const unsigned nb = (unsigned)((a.npts + 63) / 64);
   switch (co / 16) {
     case 1: hipLaunchKernelGGL(k_conv<1>, dim3(nb), dim3(256), 0, u->stream, a); break;
@@ -736,7 +747,7 @@ int salmon_unet_create(int nx, int ny, int nz, unsigned long long seed, salmon_u
   const size_t n0 = u->np[0], n1 = u->np[1], n2 = u->np[2];
   const size_t act_n[12] = {n0 * 16, n0 * 16, n1 * 16, n1 * 32, n1 * 32, n2 * 32, n2 * 64, n2 * 64,
                             n1 * 32, n1 * 32, n0 * 16, n0 * 16};
-  if ((rc = ualloc(u, &u->x0, n0 * 16))) return bail(rc);
+  if ((rc = ualloc(u, &u->x0, n0 * 16)) || (rc = ualloc(u, &u->d_zero, kConvZero))) return bail(rc);
   for (int i = 0; i < 12; ++i)
     if ((rc = ualloc(u, &u->act[i], act_n[i]))) return bail(rc);
   // d_fx: data gradient (max np * ci) followed by the two kept skip gradients (n0 x 48, n1 x 96)
@@ -764,7 +775,8 @@ int salmon_unet_create(int nx, int ny, int nz, unsigned long long seed, salmon_u
   for (int c = 0; c < 16; ++c) w[u->head_off + c] = nh(gen);
   if (hipMemcpy(u->d_w, w.data(), sizeof(float) * off, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(u->d_g, 0, sizeof(float) * off) != hipSuccess || hipMemset(u->d_m, 0, sizeof(float) * off) != hipSuccess ||
-      hipMemset(u->d_v, 0, sizeof(float) * off) != hipSuccess || hipMemset(u->x0, 0, sizeof(bf16_t) * n0 * 16) != hipSuccess)
+      hipMemset(u->d_v, 0, sizeof(float) * off) != hipSuccess || hipMemset(u->x0, 0, sizeof(bf16_t) * n0 * 16) != hipSuccess ||
+      hipMemset(u->d_zero, 0, sizeof(bf16_t) * kConvZero) != hipSuccess)
     return bail(ufail(SALMON_HIP_EHIP, "predictor initialisation copies failed"));
   *out = u;
   return 0;
