@@ -94,10 +94,13 @@ struct ConvArgs {
   const bf16_t *zero;   // zero-filled, at least co * (c1 + c2) + 8 elements
 };
 
-template <int NT>
+// SK = false: each wave takes its own 16 points and the whole K; SK = true (coarse levels, few
+// points): the four waves of a block share 16 points and take every fourth K step each, and wave
+// 0 adds the four partial sums in a fixed order (deterministic).
+template <int NT, bool SK>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int p0 = (blockIdx.x * 4 + wave) * 16;
+  const int p0 = (SK ? blockIdx.x : blockIdx.x * 4 + wave) * 16;
   if (p0 >= a.npts) return;
   const int row = lane & 15, kq = lane >> 4;
   const int p = p0 + row;
@@ -131,16 +134,28 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   };
   // two operand sets: the loads of step f0 + 32 are in flight while the MFMAs of step f0 run
   // (the accumulation order over f0 is unchanged)
+  constexpr int kst = SK ? 4 * 32 : 32;  // this wave's K steps: fs, fs + kst, ...
+  const int fs = SK ? 32 * wave : 0;
   s8v avA, avB, bvA[NT], bvB[NT];
-  load(0, avA, bvA);
-  for (int f0 = 0; f0 < K; f0 += 64) {
-    if (f0 + 32 < K) load(f0 + 32, avB, bvB);
+  load(fs, avA, bvA);
+  for (int f0 = fs; f0 < K; f0 += 2 * kst) {
+    if (f0 + kst < K) load(f0 + kst, avB, bvB);
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = mfma32(avA, bvA[t], acc[t]);
-    if (f0 + 32 >= K) break;
-    if (f0 + 64 < K) load(f0 + 64, avA, bvA);
+    if (f0 + kst >= K) break;
+    if (f0 + 2 * kst < K) load(f0 + 2 * kst, avA, bvA);
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = mfma32(avB, bvB[t], acc[t]);
+  }
+  if constexpr (SK) {
+    __shared__ f4v red[3][NT][64];
+    if (wave > 0)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) red[wave - 1][t][lane] = acc[t];
+    __syncthreads();
+    if (wave != 0) return;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = ((acc[t] + red[0][t][lane]) + red[1][t][lane]) + red[2][t][lane];
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t)
@@ -545,15 +560,23 @@ int launch_conv(salmon_unet *u, const bf16_t *s1, int c1, int up1, const bf16_t 
   a.zero = u->d_zero;
   if ((size_t)co * (c1 + c2) + 8 > kConvZero)
     return ufail(SALMON_HIP_EUNSUPPORTED, "conv %d -> %d channels above the zero operand", c1 + c2, co);
-  const unsigned nb = (unsigned)((a.npts + 63) / 64);
+  // fewer 64-point blocks than CUs: split K over the four waves of 16-point blocks
+  const bool sk = (a.npts + 63) / 64 < 256;
+  const unsigned nb = (unsigned)(sk ? (a.npts + 15) / 16 : (a.npts + 63) / 64);
+#define CONV_LAUNCH(NTv)                                                                   \
+  do {                                                                                     \
+    if (sk) hipLaunchKernelGGL((k_conv<NTv, true>), dim3(nb), dim3(256), 0, u->stream, a); \
+    else hipLaunchKernelGGL((k_conv<NTv, false>), dim3(nb), dim3(256), 0, u->stream, a);   \
+  } while (0)
   switch (co / 16) {
-    case 1: hipLaunchKernelGGL(k_conv<1>, dim3(nb), dim3(256), 0, u->stream, a); break;
-    case 2: hipLaunchKernelGGL(k_conv<2>, dim3(nb), dim3(256), 0, u->stream, a); break;
-    case 3: hipLaunchKernelGGL(k_conv<3>, dim3(nb), dim3(256), 0, u->stream, a); break;
-    case 4: hipLaunchKernelGGL(k_conv<4>, dim3(nb), dim3(256), 0, u->stream, a); break;
-    case 6: hipLaunchKernelGGL(k_conv<6>, dim3(nb), dim3(256), 0, u->stream, a); break;
+    case 1: CONV_LAUNCH(1); break;
+    case 2: CONV_LAUNCH(2); break;
+    case 3: CONV_LAUNCH(3); break;
+    case 4: CONV_LAUNCH(4); break;
+    case 6: CONV_LAUNCH(6); break;
     default: return ufail(SALMON_HIP_EUNSUPPORTED, "conv with %d output channels", co);
   }
+#undef CONV_LAUNCH
   ULAUNCH("k_conv");
   return 0;
 }
