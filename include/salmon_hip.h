@@ -259,26 +259,30 @@ int salmon_hip_set_poisson(salmon_hip_t h, const double *al);
 int salmon_hip_hartree(salmon_hip_t h, const double *rho_dev, double *vh_dev);
 
 /*
- * PZ81 LDA for nspin = 1 (exchange_correlation with xc = 'PZ', src/xc/salmon_xc.f90:58-343,
- * builtin_pz.f90:23-83): vxc(nx,ny,nz) and, if exc_dev is not null, the energy
- * density per electron.  DEVICE arrays.
+ * PZ81 LDA (exchange_correlation with xc = 'PZ', src/xc/salmon_xc.f90:58-343): for
+ * nspin = 1 builtin_pz.f90:23-83 on rho(nx,ny,nz) into vxc(nx,ny,nz); for nspin = 2 the
+ * spin-polarised form builtin_pz_sp.f90:23-98 on the channel densities rho_s(nx,ny,nz,2)
+ * into vxc_s(nx,ny,nz,2).  If exc_dev is not null, the energy density per electron
+ * (nx,ny,nz).  DEVICE arrays.
  */
 int salmon_hip_xc_pz(salmon_hip_t h, const double *rho_dev, double *vxc_dev, double *exc_dev);
 
 /*
- * V_local = Vpsl + Vh + Vxc into the handle's local potential (update_vlocal,
- * src/common/hamiltonian.f90:813-843), nspin = 1.  With vmix_dev not null the
- * result is the predictor-corrector average (vmix + V)/2
- * (src/rt/time_evolution_step.f90:402-405).  If vout_dev is not null the result
- * is also written there.  All arrays on the DEVICE, (nx,ny,nz).
+ * V_local(ispin) = Vpsl + Vh + Vxc(ispin) into the handle's local potential (update_vlocal,
+ * src/common/hamiltonian.f90:813-843).  With vmix_dev not null the result is the
+ * predictor-corrector average (vmix + V)/2 per channel (src/rt/time_evolution_step.f90:402-409).
+ * If vout_dev is not null the result is also written there.  All arrays on the DEVICE:
+ * vpsl, vh (nx,ny,nz); vxc, vmix, vout (nx,ny,nz,nspin).
  */
 int salmon_hip_update_vlocal(salmon_hip_t h, const double *vpsl_dev, const double *vh_dev,
                              const double *vxc_dev, const double *vmix_dev, double *vout_dev);
 
 /*
- * hartree + exchange_correlation (PZ) + update_vlocal in one call from the density,
- * as time_evolution_step.f90:182-207 does every step with yn_fix_func = 'n'
- * (Vh kept in handle scratch, Vxc formed in the update kernel).
+ * hartree + exchange_correlation (PZ) + update_vlocal in one call from the density
+ * rho_s(nx,ny,nz,nspin) as calc_density returns it, as time_evolution_step.f90:160-207
+ * does every step with yn_fix_func = 'n': Vh of the total density (rho_s(1) + rho_s(2)
+ * for nspin = 2, summed in the first DFT pass), Vxc of the channel densities formed in
+ * the update kernel, Vh kept in handle scratch.  vmix, vout (nx,ny,nz,nspin) as above.
  */
 int salmon_hip_update_potential(salmon_hip_t h, const double *rho_dev, const double *vpsl_dev,
                                 const double *vmix_dev, double *vout_dev);

@@ -16,6 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libzstencil_ref.so")
+REF_XC_LIB = os.path.join(HERE, "_ref", "libxc_ref.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -71,6 +72,7 @@ def lib():
         L.orc_rt_run.restype = C.c_int
         L.orc_hartree.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
         L.orc_xc_pz.argtypes = [C.c_long, _dp, _dp, _dp]
+        L.orc_xc_pz_sp.argtypes = [C.c_long, _dp, _dp, _dp]
         L.orc_rt_run_scf.argtypes = [S, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
                                      _dp, _dp]
         L.orc_rt_run_scf.restype = C.c_int
@@ -213,7 +215,7 @@ class Oracle:
         ac = self._ac(nt, ac)
         p = np.array(s.psi if psi is None else psi, dtype=np.complex128, copy=True)
         vpsl = np.ascontiguousarray(s.vlocal[0], dtype=np.float64)
-        vloc = np.zeros_like(vpsl)
+        vloc = np.zeros((s.nspin,) + vpsl.shape) if s.nspin == 2 else np.zeros_like(vpsl)
         al = np.ascontiguousarray(s.al, dtype=np.float64)
         cur = np.zeros((nt, 3))
         dip = np.zeros((nt, 3))
@@ -236,7 +238,7 @@ class Oracle:
         ac = np.ascontiguousarray(s.pulse.table(s.dt, nt) if s.pulse else np.zeros((nt + 2, 3)))
         p = np.array(s.psi if psi is None else psi, dtype=np.complex128, copy=True)
         vpsl = np.ascontiguousarray(s.vlocal[0], dtype=np.float64)
-        vloc = np.zeros_like(vpsl)
+        vloc = np.zeros((s.nspin,) + vpsl.shape) if s.nspin == 2 else np.zeros_like(vpsl)
         al = np.ascontiguousarray(s.al, dtype=np.float64)
         cur = np.zeros((nt, 3))
         dip = np.zeros((nt, 3))
@@ -270,6 +272,18 @@ def xc_pz(rho):
     v = np.zeros_like(r)
     e = np.zeros_like(r)
     lib().orc_xc_pz(r.size, _p(r), _p(v), _p(e))
+    return v, e
+
+
+def xc_pz_sp(rho_s):
+    """PZ81 LDA, spin-polarised: rho_s is (2, ...) (the two channel densities); returns
+    (vxc_s (2, ...), exc (...))."""
+    r = np.ascontiguousarray(rho_s, dtype=np.float64)
+    if r.shape[0] != 2:
+        raise ValueError("rho_s: two spin channels along the first axis")
+    v = np.zeros_like(r)
+    e = np.zeros(r.shape[1:])
+    lib().orc_xc_pz_sp(r[0].size, _p(r), _p(v), _p(e))
     return v, e
 
 
@@ -317,3 +331,30 @@ def ref_zstencil(nx, ny, nz, ly, tpsi, V, lap0, lapt, nabt, mode="seq"):
                                 _p(t.view(np.float64)), _p(h.view(np.float64)), _p(Vv), C.byref(lap0_),
                                 _p(lt), _p(nb))
     return h
+
+
+def ref_xc_available() -> bool:
+    return os.path.exists(REF_XC_LIB)
+
+
+def ref_xc_pz(rho):
+    """SALMON's exc_cor_pz (builtin_pz.f90:23-51) compiled from the reference source, called as
+    exec_builtin_pz does for nspin = 1 (rho_s = rho * 0.5, salmon_xc.f90:778, 795); returns
+    (vxc, exc)."""
+    L = C.CDLL(REF_XC_LIB)
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    rs = np.ascontiguousarray(r * 0.5)
+    v, e, ee = np.zeros_like(r), np.zeros_like(r), np.zeros_like(r)
+    L.ref_exc_cor_pz(C.c_int(r.size), _p(rs), _p(e), _p(ee), _p(v))
+    return v, e
+
+
+def ref_xc_pz_sp(rho_s):
+    """SALMON's exc_cor_pz_sp (builtin_pz_sp.f90:23-40) compiled from the reference source;
+    rho_s is (2, ...); returns (vxc_s (2, ...), exc)."""
+    L = C.CDLL(REF_XC_LIB)
+    r = np.ascontiguousarray(rho_s, dtype=np.float64)
+    v = np.zeros_like(r)
+    e, ee = np.zeros(r.shape[1:]), np.zeros(r.shape[1:])
+    L.ref_exc_cor_pz_sp(C.c_int(r[0].size), _p(r), _p(e), _p(ee), _p(v))
+    return v, e

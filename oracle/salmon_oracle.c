@@ -31,8 +31,15 @@
  *                       periodic Poisson equation through the FD-free spectral
  *                       Laplacian (G^2 V(G) = 4 pi rho(G)); G = 0 removed.
  *   orc_xc_pz           v = d(rho e)/d rho by finite differences of rho*exc; the
- *                       published PZ81 values at rs = 1, 2, 5.
- *   orc_rt_run_scf      composition (time_evolution_step.f90:160-207, 354-423).
+ *                       published PZ81 values at rs = 1, 2, 5; exc_cor_pz compiled
+ *                       from the reference source (oracle/_ref).
+ *   orc_xc_pz_sp        exc_cor_pz_sp compiled from the reference source
+ *                       (oracle/_ref); v_s = d(rho e)/d rho_s by finite differences;
+ *                       zeta = 0 gives the unpolarised PZ81, zeta = 1 the published
+ *                       polarised parameters; exchanging the channels exchanges v.
+ *   orc_rt_run_scf      composition (time_evolution_step.f90:160-207, 354-423); with
+ *                       nspin = 2 and both channels holding the nspin = 1 orbitals it
+ *                       follows the nspin = 1 run (to the PZ offsets' 1e-10 scale).
  *   orc_rt_run_full     the above plus the 'lo' field feedback: the discrete plasma
  *                       oscillation cos(w dt) = 1 - 2 pi n dt^2 of one constant orbital.
  */
@@ -717,14 +724,76 @@ void orc_xc_pz(long n, const double *rho, double *vxc, double *exc_out) {
   }
 }
 
-/* V_local = Vpsl + Vh + Vxc (update_vlocal, src/common/hamiltonian.f90:813-843), one spin */
+/* PZ81 LDA, spin-polarised (collinear): exc_cor_pz_sp / PZxc_sp
+   (src/xc/builtin_pz_sp.f90:23-98), called with the channel densities as they are
+   (salmon_xc.f90:780, 797).  rho_s, vxc_s: (n, 2), channel-major; exc_out may be NULL.
+   Terms are added in the order the Fortran expressions give them (left to right). */
+void orc_xc_pz_sp(long n, const double *rho_s, double *vxc_s, double *exc_out) {
+  const double Pi = 3.141592653589793;
+  const double gammaU = -0.1423, gammaP = -0.0843;
+  const double beta1U = 1.0529, beta2U = 0.3334, beta1P = 1.3981, beta2P = 0.2611;
+  const double AU = 0.0311, BU = -0.048, CU = 0.002, DU = -0.0116;
+  const double AP = 0.01555, BP = -0.0269, CP = 0.0007, DP = -0.0048;
+  const double cx = 3.0 / 4.0 * pow(3.0 / (2 * Pi), 2.0 / 3.0);
+  const double c13 = -pow(2.0, 1.0 / 3.0) + 1.0;  /* (2^(1/3) - 1) with the sign of the source */
+  const double fden = pow(2.0, 4.0 / 3.0) - 2.0;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    const double trho = rho_s[i] + rho_s[n + i] + 1e-20;
+    const double tu = rho_s[i] + 5e-21, td = rho_s[n + i] + 5e-21;
+    const double rs = pow(3.0 / (4.0 * Pi * trho), 1.0 / 3.0);
+    const double zeta = (tu - td) / trho;
+    const double sf = (pow(1.0 + zeta, 4.0 / 3.0) + pow(1.0 - zeta, 4.0 / 3.0) - 2.0) / fden;
+    const double dsf = 4.0 / 3.0 * (pow(1.0 + zeta, 1.0 / 3.0) - pow(1.0 - zeta, 1.0 / 3.0)) / fden;
+    double exc = -cx / rs + c13 * cx / rs * sf;
+    double v[2];
+    for (int js = 0; js < 2; ++js) {
+      const double sg = js == 0 ? 1.0 : -1.0;
+      v[js] = exc - 1.0 / 3.0 * cx / rs + 1.0 / 3.0 * c13 * cx / rs * sf + c13 * cx / rs * (sg - zeta) * dsf;
+    }
+    double ecu, ecp, vcu, vcp;
+    if (rs > 1.0) {
+      const double du = 1.0 + beta1U * sqrt(rs) + beta2U * rs, dp = 1.0 + beta1P * sqrt(rs) + beta2P * rs;
+      ecu = gammaU / du;
+      ecp = gammaP / dp;
+      vcu = gammaU * (1.0 + 7.0 / 6.0 * beta1U * sqrt(rs) + 4.0 / 3.0 * beta2U * rs) / (du * du);
+      vcp = gammaP * (1.0 + 7.0 / 6.0 * beta1P * sqrt(rs) + 4.0 / 3.0 * beta2P * rs) / (dp * dp);
+    } else {
+      const double L = log(rs);
+      ecu = AU * L + BU + CU * rs * L + DU * rs;
+      ecp = AP * L + BP + CP * rs * L + DP * rs;
+      vcu = AU * L + (BU - AU / 3.0) + 2.0 / 3.0 * CU * rs * L + (2.0 * DU - CU) / 3.0 * rs;
+      vcp = AP * L + (BP - AP / 3.0) + 2.0 / 3.0 * CP * rs * L + (2.0 * DP - CP) / 3.0 * rs;
+    }
+    exc = exc + ecu + sf * (ecp - ecu);
+    for (int js = 0; js < 2; ++js) {
+      const double sg = js == 0 ? 1.0 : -1.0;
+      vxc_s[(size_t)js * n + i] = v[js] + vcu + sf * (vcp - vcu) + (ecp - ecu) * (sg - zeta) * dsf;
+    }
+    if (exc_out) exc_out[i] = exc;
+  }
+}
+
+/* V_local(is) = Vpsl + Vh + Vxc(is) (update_vlocal, src/common/hamiltonian.f90:813-843) from
+   the channel densities: Vh of the total density rho_s(1) + rho_s(2)
+   (time_evolution_step.f90:162-169), Vxc from PZ (nspin = 1) or spin-polarised PZ. */
 static void vlocal_from_rho(const orc_system *s, const double *vpsl, const double al[3], const double *rho,
                             double *vloc) {
   const size_t g = grid_size(s);
-  double *vh = (double *)malloc(sizeof(double) * g), *vxc = (double *)malloc(sizeof(double) * g);
-  orc_hartree(s->nx, s->ny, s->nz, al, rho, vh);
-  orc_xc_pz((long)g, rho, vxc, NULL);
-  for (size_t i = 0; i < g; ++i) vloc[i] = (vpsl[i] + vh[i]) + vxc[i];
+  const int ns = s->nspin;
+  double *vh = (double *)malloc(sizeof(double) * g), *vxc = (double *)malloc(sizeof(double) * g * ns);
+  if (ns == 2) {
+    double *rt = (double *)malloc(sizeof(double) * g);
+    for (size_t i = 0; i < g; ++i) rt[i] = rho[i] + rho[g + i];
+    orc_hartree(s->nx, s->ny, s->nz, al, rt, vh);
+    orc_xc_pz_sp((long)g, rho, vxc, NULL);
+    free(rt);
+  } else {
+    orc_hartree(s->nx, s->ny, s->nz, al, rho, vh);
+    orc_xc_pz((long)g, rho, vxc, NULL);
+  }
+  for (int is = 0; is < ns; ++is)
+    for (size_t i = 0; i < g; ++i) vloc[(size_t)is * g + i] = (vpsl[i] + vh[i]) + vxc[(size_t)is * g + i];
   free(vh);
   free(vxc);
 }
@@ -742,7 +811,6 @@ int orc_rt_run_full(orc_system *s, int scf, int pc, int lo, const double *vpsl, 
                     double dt, int n_hamil, int nt, double *ac, double *psi, double *curr_t, double *dip_t,
                     double *rne_t) {
   const double pi = 3.141592653589793238462643383279502884;
-  if (scf && s->nspin != 1) return -2;
   pc = pc && scf;
   const size_t tot = C2(orb_size(s) * (size_t)s->nspin * s->no * s->nk);
   const size_t g = grid_size(s);
@@ -751,7 +819,8 @@ int orc_rt_run_full(orc_system *s, int scf, int pc, int lo, const double *vpsl, 
   double *w = (double *)calloc(tot, sizeof(double));
   double *psi_tmp = pc ? (double *)malloc(sizeof(double) * tot) : NULL;
   double *rho = (double *)malloc(sizeof(double) * g * s->nspin);
-  double *v_tmp = (double *)malloc(sizeof(double) * g);
+  const size_t gv = g * (size_t)s->nspin;  /* V_local(ix,iy,iz,ispin) */
+  double *v_tmp = (double *)malloc(sizeof(double) * gv);
   double *zc = (double *)malloc(sizeof(double) * 2 * n_hamil);
   double *ac_ind = (double *)calloc(3 * (size_t)(nt + 2), sizeof(double));
   if (!b || !w || !rho || !v_tmp || !zc || !ac_ind || (pc && !psi_tmp)) return -1;
@@ -766,12 +835,12 @@ int orc_rt_run_full(orc_system *s, int scf, int pc, int lo, const double *vpsl, 
     for (int d = 0; d < 3; ++d) A[d] = 0.5 * (ac[3 * itt + d] + ac[3 * (itt - 1) + d]);
     if (s->natom > 0) orc_update_kvector(s, A);
     if (pc) {
-      memcpy(v_tmp, vloc, sizeof(double) * g);
+      memcpy(v_tmp, vloc, sizeof(double) * gv);
       memcpy(psi_tmp, a, sizeof(double) * tot);
       orc_taylor(s, A, zc, n_hamil, a, b, w);
       orc_density(s, b, rho);
       vlocal_from_rho(s, vpsl, al, rho, vloc);
-      for (size_t i = 0; i < g; ++i) vloc[i] = 0.5 * (v_tmp[i] + vloc[i]);
+      for (size_t i = 0; i < gv; ++i) vloc[i] = 0.5 * (v_tmp[i] + vloc[i]);
       memcpy(a, psi_tmp, sizeof(double) * tot);
     }
     orc_taylor(s, A, zc, n_hamil, a, b, w);

@@ -78,6 +78,7 @@ int orc_rt_run(orc_system *s, double dt, int n_hamil, int nt, const double *ac_t
 
 void orc_hartree(int nx, int ny, int nz, const double al[3], const double *rho, double *vh);
 void orc_xc_pz(long n, const double *rho, double *vxc, double *exc);
+void orc_xc_pz_sp(long n, const double *rho_s, double *vxc_s, double *exc);
 
 /* RT loop with the Hartree + PZ-LDA potential updated every step (yn_fix_func = 'n'),
    optionally with the predictor-corrector; vpsl is the fixed local pseudopotential part,

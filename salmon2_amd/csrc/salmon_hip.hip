@@ -19,7 +19,8 @@
 //   k_current_*       stencil_current + calc_current (density_matrix.f90:240-530)
 //   k_dipole_part/_sum calc_dip (src/rt/dip.f90:114-225)
 //   k_dft_axis        Hartree potential (poisson_ft, src/poisson/poisson_periodic.f90:23-242)
-//   k_xc_pz           PZ81 LDA (src/xc/builtin_pz.f90:23-83)
+//   k_xc_pz           PZ81 LDA (src/xc/builtin_pz.f90:23-83) and its spin-polarised form
+//                     (src/xc/builtin_pz_sp.f90:23-98)
 //   k_update_vlocal   update_vlocal (src/common/hamiltonian.f90:813-843) with the
 //                     predictor-corrector average (src/rt/time_evolution_step.f90:404)
 #include <hip/hip_runtime.h>
@@ -2072,6 +2073,7 @@ constexpr int kDftMaxN = 256;
 struct DftArgs {
   const double2 *in;  // complex input, or null with rin
   const double *rin;  // real input (first forward pass)
+  const double *rin2; // with rin: a second real input added to it (the other spin channel), or null
   double2 *out;       // complex output, or null with rout
   double *rout;       // real part output (last inverse pass)
   const double2 *tw;  // n x n, tw[k * n + j]
@@ -2098,7 +2100,7 @@ __global__ __launch_bounds__(256) void k_dft_axis(DftArgs a) {
       const long long c = c0 + cl;
       const long long o = c / inner, i = c - o * inner;
       const size_t g = ((size_t)o * n + j) * inner + i;
-      v = a.rin ? make_double2(a.rin[g], 0.0) : a.in[g];
+      v = a.rin ? make_double2(a.rin2 ? a.rin[g] + a.rin2[g] : a.rin[g], 0.0) : a.in[g];
     }
     dft_tile[j * cols + cl] = v;
   }
@@ -2168,22 +2170,76 @@ __device__ __forceinline__ void pz_point(double rho, double &vxc, double &exc) {
   exc = e;
 }
 
-__global__ __launch_bounds__(256) void k_xc_pz(const double *rho, double *vxc, double *exc, size_t n) {
+// PZ81 LDA, spin-polarised (exc_cor_pz_sp / PZxc_sp, src/xc/builtin_pz_sp.f90:23-98) at one
+// point from the two channel densities; terms in the order of the source's expressions.
+__device__ __forceinline__ void pz_sp_point(double ru, double rd, double &vu, double &vd, double &exc) {
+  const double Pi = 3.141592653589793;
+  const double gammaU = -0.1423, gammaP = -0.0843;
+  const double beta1U = 1.0529, beta2U = 0.3334, beta1P = 1.3981, beta2P = 0.2611;
+  const double AU = 0.0311, BU = -0.048, CU = 0.002, DU = -0.0116;
+  const double AP = 0.01555, BP = -0.0269, CP = 0.0007, DP = -0.0048;
+  const double cx = 3.0 / 4.0 * pow(3.0 / (2 * Pi), 2.0 / 3.0);
+  const double c13 = -pow(2.0, 1.0 / 3.0) + 1.0;
+  const double fden = pow(2.0, 4.0 / 3.0) - 2.0;
+  const double trho = ru + rd + 1e-20;
+  const double tu = ru + 5e-21, td = rd + 5e-21;
+  const double rs = pow(3.0 / (4.0 * Pi * trho), 1.0 / 3.0);
+  const double zeta = (tu - td) / trho;
+  const double sf = (pow(1.0 + zeta, 4.0 / 3.0) + pow(1.0 - zeta, 4.0 / 3.0) - 2.0) / fden;
+  const double dsf = 4.0 / 3.0 * (pow(1.0 + zeta, 1.0 / 3.0) - pow(1.0 - zeta, 1.0 / 3.0)) / fden;
+  double e = -cx / rs + c13 * cx / rs * sf;
+  const double vs = e - 1.0 / 3.0 * cx / rs + 1.0 / 3.0 * c13 * cx / rs * sf;  // common to both channels
+  double v0 = vs + c13 * cx / rs * (1.0 - zeta) * dsf;
+  double v1 = vs + c13 * cx / rs * (-1.0 - zeta) * dsf;
+  double ecu, ecp, vcu, vcp;
+  if (rs > 1.0) {
+    const double sq = sqrt(rs);
+    const double du = 1.0 + beta1U * sq + beta2U * rs, dp = 1.0 + beta1P * sq + beta2P * rs;
+    ecu = gammaU / du;
+    ecp = gammaP / dp;
+    vcu = gammaU * (1.0 + 7.0 / 6.0 * beta1U * sq + 4.0 / 3.0 * beta2U * rs) / (du * du);
+    vcp = gammaP * (1.0 + 7.0 / 6.0 * beta1P * sq + 4.0 / 3.0 * beta2P * rs) / (dp * dp);
+  } else {
+    const double L = log(rs);
+    ecu = AU * L + BU + CU * rs * L + DU * rs;
+    ecp = AP * L + BP + CP * rs * L + DP * rs;
+    vcu = AU * L + (BU - AU / 3.0) + 2.0 / 3.0 * CU * rs * L + (2.0 * DU - CU) / 3.0 * rs;
+    vcp = AP * L + (BP - AP / 3.0) + 2.0 / 3.0 * CP * rs * L + (2.0 * DP - CP) / 3.0 * rs;
+  }
+  e = e + ecu + sf * (ecp - ecu);
+  vu = v0 + vcu + sf * (vcp - vcu) + (ecp - ecu) * (1.0 - zeta) * dsf;
+  vd = v1 + vcu + sf * (vcp - vcu) + (ecp - ecu) * (-1.0 - zeta) * dsf;
+  exc = e;
+}
+
+// nspin = 1: rho, vxc (n); nspin = 2: rho_s, vxc_s (n, 2) channel-major; exc (n) or null
+__global__ __launch_bounds__(256) void k_xc_pz(const double *rho, double *vxc, double *exc, size_t n, int nspin) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double v, e;
-  pz_point(rho[i], v, e);
-  vxc[i] = v;
+  double e;
+  if (nspin == 2) {
+    double vu, vd;
+    pz_sp_point(rho[i], rho[n + i], vu, vd, e);
+    vxc[i] = vu;
+    vxc[n + i] = vd;
+  } else {
+    double v;
+    pz_point(rho[i], v, e);
+    vxc[i] = v;
+  }
   if (exc) exc[i] = e;
 }
 
-// update_vlocal (src/common/hamiltonian.f90:813-843): V = Vpsl + Vh + Vxc, with Vxc
-// either given or formed from rho on the fly; optionally the predictor-corrector
-// average V = (Vmix + V)/2 (src/rt/time_evolution_step.f90:404).
+// update_vlocal (src/common/hamiltonian.f90:813-843): V(is) = Vpsl + Vh + Vxc(is), with Vxc
+// either given or formed from rho on the fly (PZ, or spin-polarised PZ from the two channel
+// densities); optionally the predictor-corrector average V(is) = (Vmix(is) + V(is))/2
+// (src/rt/time_evolution_step.f90:404-409).  vpsl, vh: one channel; vxc, rho, vmix, V, vout:
+// nspin channels of n points.
 struct VlocArgs {
   const double *vpsl, *vh, *vxc, *rho, *vmix;
   double *V, *vout;
   size_t n;
+  int nspin;
   const int *skip;    // device flag: the launch does nothing when it is set (null: never)
 };
 
@@ -2191,17 +2247,23 @@ __global__ __launch_bounds__(256) void k_update_vlocal(VlocArgs a) {
   if (a.skip && *a.skip) return;  // a device-side gate switched this launch off
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
-  double vx;
+  double vx[2];
   if (a.rho) {
     double e;
-    pz_point(a.rho[i], vx, e);
+    if (a.nspin == 2) pz_sp_point(a.rho[i], a.rho[a.n + i], vx[0], vx[1], e);
+    else pz_point(a.rho[i], vx[0], e);
   } else {
-    vx = a.vxc[i];
+    vx[0] = a.vxc[i];
+    if (a.nspin == 2) vx[1] = a.vxc[a.n + i];
   }
-  double v = a.vpsl[i] + a.vh[i] + vx;
-  if (a.vmix) v = 0.5 * (a.vmix[i] + v);
-  a.V[i] = v;
-  if (a.vout) a.vout[i] = v;
+  const double base = a.vpsl[i] + a.vh[i];
+  for (int is = 0; is < a.nspin; ++is) {
+    const size_t j = (size_t)is * a.n + i;
+    double v = base + vx[is];
+    if (a.vmix) v = 0.5 * (a.vmix[j] + v);
+    a.V[j] = v;
+    if (a.vout) a.vout[j] = v;
+  }
 }
 
 // Streaming copy of n 16-byte elements (HBM ceiling of the box): dwordx4 loads and stores,
@@ -3541,11 +3603,11 @@ int salmon_hip_set_poisson(salmon_hip_t c, const double *al) {
 }
 
 static int launch_dft(salmon_hip_t c, int axis, double sg, const double *rin, const double2 *in, double2 *out,
-                      double *rout, const double *coef) {
+                      double *rout, const double *coef, const double *rin2 = nullptr) {
   const int n[3] = {c->nx, c->ny, c->nz};
   const int inner = axis == 0 ? 1 : (axis == 1 ? c->nx : c->nx * c->ny);
   DftArgs a;
-  a.in = in; a.rin = rin; a.out = out; a.rout = rout;
+  a.in = in; a.rin = rin; a.rin2 = rin2; a.out = out; a.rout = rout;
   a.tw = c->d_tw[axis];
   a.coef = coef;
   a.nrm = (double)c->ngrid;
@@ -3565,13 +3627,12 @@ static int launch_dft(salmon_hip_t c, int axis, double sg, const double *rin, co
   return 0;
 }
 
-int salmon_hip_hartree(salmon_hip_t c, const double *rho_dev, double *vh_dev) {
-  if (int rc = check(c)) return rc;
-  if (!rho_dev || !vh_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
-  if (!c->have_cell) return fail(SALMON_HIP_ESTATE, "set_poisson not called");
+// Hartree potential of rho (+ rho2 when not null: the total density of two spin channels,
+// rho_s(1) + rho_s(2), src/rt/time_evolution_step.f90:166-169, formed in the first pass)
+static int hartree_impl(salmon_hip_t c, const double *rho_dev, const double *rho2_dev, double *vh_dev) {
   double2 *b0 = c->d_fft[0], *b1 = c->d_fft[1];
   int rc;
-  if ((rc = launch_dft(c, 2, -1.0, rho_dev, nullptr, b0, nullptr, nullptr))) return rc;
+  if ((rc = launch_dft(c, 2, -1.0, rho_dev, nullptr, b0, nullptr, nullptr, rho2_dev))) return rc;
   if ((rc = launch_dft(c, 1, -1.0, nullptr, b0, b1, nullptr, nullptr))) return rc;
   if ((rc = launch_dft(c, 0, -1.0, nullptr, b1, b0, nullptr, c->d_gcoef))) return rc;
   if ((rc = launch_dft(c, 2, 1.0, nullptr, b0, b1, nullptr, nullptr))) return rc;
@@ -3579,12 +3640,18 @@ int salmon_hip_hartree(salmon_hip_t c, const double *rho_dev, double *vh_dev) {
   return launch_dft(c, 0, 1.0, nullptr, b0, nullptr, vh_dev, nullptr);
 }
 
+int salmon_hip_hartree(salmon_hip_t c, const double *rho_dev, double *vh_dev) {
+  if (int rc = check(c)) return rc;
+  if (!rho_dev || !vh_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+  if (!c->have_cell) return fail(SALMON_HIP_ESTATE, "set_poisson not called");
+  return hartree_impl(c, rho_dev, nullptr, vh_dev);
+}
+
 int salmon_hip_xc_pz(salmon_hip_t c, const double *rho_dev, double *vxc_dev, double *exc_dev) {
   if (int rc = check(c)) return rc;
   if (!rho_dev || !vxc_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
-  if (c->nspin != 1) return fail(SALMON_HIP_EUNSUPPORTED, "spin-polarised PZ not implemented");
   hipLaunchKernelGGL(k_xc_pz, dim3((unsigned)((c->ngrid + 255) / 256)), dim3(256), 0, c->stream, rho_dev,
-                     vxc_dev, exc_dev, c->ngrid);
+                     vxc_dev, exc_dev, c->ngrid, c->nspin);
   LAUNCHCHK("k_xc_pz");
   return 0;
 }
@@ -3596,6 +3663,7 @@ static int launch_vlocal(salmon_hip_t c, const double *vpsl, const double *vh, c
   a.V = c->d_V;
   a.vout = vout;
   a.n = c->ngrid;
+  a.nspin = c->nspin;
   a.skip = c->d_skip;
   hipLaunchKernelGGL(k_update_vlocal, dim3((unsigned)((c->ngrid + 255) / 256)), dim3(256), 0, c->stream, a);
   LAUNCHCHK("k_update_vlocal");
@@ -3607,7 +3675,6 @@ int salmon_hip_update_vlocal(salmon_hip_t c, const double *vpsl_dev, const doubl
                              const double *vmix_dev, double *vout_dev) {
   if (int rc = check(c)) return rc;
   if (!vpsl_dev || !vh_dev || !vxc_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
-  if (c->nspin != 1) return fail(SALMON_HIP_EUNSUPPORTED, "nspin=2 potential update not implemented");
   return launch_vlocal(c, vpsl_dev, vh_dev, vxc_dev, nullptr, vmix_dev, vout_dev);
 }
 
@@ -3615,8 +3682,8 @@ int salmon_hip_update_potential(salmon_hip_t c, const double *rho_dev, const dou
                                 const double *vmix_dev, double *vout_dev) {
   if (int rc = check(c)) return rc;
   if (!rho_dev || !vpsl_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
-  if (c->nspin != 1) return fail(SALMON_HIP_EUNSUPPORTED, "nspin=2 potential update not implemented");
-  if (int rc = salmon_hip_hartree(c, rho_dev, c->d_vh)) return rc;
+  if (!c->have_cell) return fail(SALMON_HIP_ESTATE, "set_poisson not called");
+  if (int rc = hartree_impl(c, rho_dev, c->nspin == 2 ? rho_dev + c->ngrid : nullptr, c->d_vh)) return rc;
   return launch_vlocal(c, vpsl_dev, c->d_vh, nullptr, rho_dev, vmix_dev, vout_dev);
 }
 

@@ -81,20 +81,23 @@ class RTPropagator:
         self.scf = bool(scf)
         self.pc = bool(pc) and self.scf  # with a fixed potential the corrector average is the identity
         if self.scf:
-            if sysm.nspin != 1:
-                raise ValueError("the self-consistent potential update supports nspin = 1")
+            # Vpsl is the spin-independent part (system.vlocal[0]); V_local(ispin) = Vpsl + Vh[rho_1 + rho_2]
+            # + Vxc(ispin)[rho_1, rho_2] with nspin = 2 (spin-polarised PZ)
             self.h.set_poisson(sysm.al)
             self.vpsl = torch.from_numpy(np.ascontiguousarray(sysm.vlocal[0])).to(self.dev)
-            self.vcur = torch.zeros_like(self.vpsl)
+            vshape = (sysm.nspin,) + tuple(self.vpsl.shape) if sysm.nspin == 2 else tuple(self.vpsl.shape)
+            self.vcur = torch.zeros(vshape, dtype=torch.float64, device=self.dev)
             self.h.density(self.psi_a, self.rho)
             self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
-            self.vmid = torch.zeros_like(self.vpsl)  # the predictor-corrector midpoint potential of the last step
-            self.dv = torch.zeros_like(self.vpsl)  # V_mid - V of the predictor (corrector from the predictor's H psi)
+            self.vmid = torch.zeros_like(self.vcur)  # the predictor-corrector midpoint potential of the last step
+            self.dv = torch.zeros_like(self.vcur)  # V_mid - V of the predictor (corrector from the predictor's H psi)
         # learned predictor (salmon2_amd.learned.Predictor): needs the self-consistent predictor-corrector step
         self.learned = learned
         if learned is not None:
             if not self.pc:
                 raise ValueError("the learned predictor replaces the predictor of the predictor-corrector (scf=pc=True)")
+            if sysm.nspin != 1:
+                raise ValueError("the learned predictor takes one potential channel (nspin = 1)")
             self.hist = [torch.zeros_like(self.vpsl) for _ in range(4)]  # V(t), V(t-dt), V(t-2dt), V(t-3dt)
             self.hist[0].copy_(self.vcur)
             self.nhist = 1
@@ -300,7 +303,7 @@ class RTPropagator:
         vh = None
         if self.scf:
             v = torch.zeros_like(self.vpsl)
-            self.h.hartree(self.rho, v)
+            self.h.hartree(self.rho[0] + self.rho[1] if self.s.nspin == 2 else self.rho, v)
             vh = (v.cpu().numpy(),) * 2
         restart.write_checkpoint(directory, self.psi_a.cpu().numpy(), self.s.ny, self.s.rocc, self.itt,
                                  sliced_block=sliced_block, vh_stock=vh)

@@ -84,10 +84,63 @@ def test_potential_update(torch_dev, oracle_lib, name):
     h.close()
 
 
-@pytest.mark.parametrize("name,pc,nt", [("tiny", False, 10), ("c1", True, 100), ("si64_32cube", True, 4)])
+SPIN_CASES = {
+    "spin2_16x14x12": lambda: system.make_si_system(n=(16, 14, 12), no=3, nkgrid=(1, 1, 2), nspin=2, seed=19),
+    "spin2_si8_16cube": lambda: system.make_si_system(ncell=(1, 1, 1), n=16, no=4, nkgrid=(1, 1, 1), nspin=2, seed=23),
+}
+
+
+@pytest.mark.parametrize("name", list(SPIN_CASES))
+def test_potential_update_spin(torch_dev, oracle_lib, name):
+    """nspin = 2: spin-polarised PZ on the channel densities, the Hartree potential of their sum,
+    V_local(ispin) = Vpsl + Vh + Vxc(ispin) and the per-channel predictor-corrector average."""
+    torch = torch_dev
+    s = SPIN_CASES[name]()
+    print(f"[gpu] potential spin case={name} grid={s.nx}x{s.ny}x{s.nz}", flush=True)
+    from salmon2_amd.hip import Handle
+
+    rng = np.random.default_rng(6)
+    shape = (s.nz, s.ny, s.nx)
+    rho_s = 0.01 + rng.random((2,) + shape) * 0.05
+    rho_s[1, 0, 0, :4] = 0.0          # fully polarised points
+    rho_s[0, 1, 0, :4] = 0.0
+    rho_s[1, 2, 0, :4] = rho_s[0, 2, 0, :4]  # unpolarised points
+    rho_s[:, 3, 0, 0] = 1e-8          # low-density branch (rs > 1)
+    vpsl = s.vlocal[0]
+    vmix = rng.standard_normal((2,) + shape)
+    h = Handle(s, stream=torch.cuda.current_stream())
+    h.set_poisson(s.al)
+    d = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in
+         dict(rho=rho_s, vpsl=vpsl, vmix=vmix).items()}
+    vh = torch.zeros(shape, dtype=torch.float64, device="cuda")
+    vxc = torch.zeros((2,) + shape, dtype=torch.float64, device="cuda")
+    exc = torch.zeros_like(vh)
+    v1, v2, v3 = (torch.zeros_like(vxc) for _ in range(3))
+    h.hartree(d["rho"][0] + d["rho"][1], vh)
+    h.xc_pz(d["rho"], vxc, exc)
+    h.update_vlocal(d["vpsl"], vh, vxc, vmix=d["vmix"], vout=v1)
+    h.update_potential(d["rho"], d["vpsl"], vout=v2)
+    h.update_potential(d["rho"], d["vpsl"], vmix=d["vmix"], vout=v3)
+    torch.cuda.synchronize()
+    vh_ref = oracle_lib.hartree(rho_s[0] + rho_s[1], s.al)
+    vxc_ref, exc_ref = oracle_lib.xc_pz_sp(rho_s)
+    v_ref = vpsl[None] + vh_ref[None] + vxc_ref
+    errs = dict(hartree=rel(vh.cpu().numpy(), vh_ref), vxc=rel(vxc.cpu().numpy(), vxc_ref),
+                exc=rel(exc.cpu().numpy(), exc_ref), vlocal_mix=rel(v1.cpu().numpy(), 0.5 * (vmix + v_ref)),
+                potential=rel(v2.cpu().numpy(), v_ref), potential_mix=rel(v3.cpu().numpy(), 0.5 * (vmix + v_ref)))
+    for k, v in errs.items():
+        print(f"[parity] {name} {k}: rel err {v:.3e}", flush=True)
+        assert v <= CALL_TOL, k
+    # the two channels' potentials differ where the channel densities do
+    assert np.abs(vxc_ref[0] - vxc_ref[1]).max() > 1e-3
+    h.close()
+
+
+@pytest.mark.parametrize("name,pc,nt", [("tiny", False, 10), ("c1", True, 100), ("si64_32cube", True, 4),
+                                        ("spin2_16x14x12", True, 20), ("spin2_si8_16cube", False, 10)])
 def test_rt_scf_run(torch_dev, oracle_lib, name, pc, nt):
     torch = torch_dev
-    s = CASES[name]()
+    s = (CASES.get(name) or SPIN_CASES[name])()
     print(f"[gpu] rt scf case={name} pc={pc} steps={nt}", flush=True)
     from salmon2_amd.rt import RTPropagator
 

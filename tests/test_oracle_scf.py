@@ -6,9 +6,16 @@
 * PZ81: exchange -0.458165/rs and the correlation values of the published
   parametrisation at rs = 0.5, 1, 2, 5; v_xc = d(rho e_xc)/d rho by central
   differences on both branches of the parametrisation;
+* PZ81 and its spin-polarised form: bitwise against exc_cor_pz / exc_cor_pz_sp compiled
+  from the reference sources (oracle/_ref, when built); for the polarised form also
+  v_s = d(rho e_xc)/d rho_s by central differences in each channel (a wrong sign of the
+  spin term or a wrong f'(zeta) fails it), zeta = 0 reducing to the unpolarised PZ81,
+  zeta = 1 to the published polarised parameters with exchange 2^(1/3) e_x, and the
+  channels' exchange exchanging the potentials exactly;
 * SCF loop: every step rebuilt from dense-matrix Taylor polynomials, numpy
   densities and the two pinned functionals, with and without the
-  predictor-corrector; electron number never grows.
+  predictor-corrector; electron number never grows; nspin = 2 with both channels
+  holding the nspin = 1 orbitals follows the nspin = 1 run.
 """
 import dataclasses
 
@@ -78,6 +85,93 @@ def test_pz_potential_is_energy_derivative(oracle_lib, rs):
     v, e = oracle_lib.xc_pz(r)
     dfd = (r[1] * e[1] - r[0] * e[0]) / (2 * h)
     assert abs(v[2] - dfd) < 1e-8 * abs(v[2])
+
+
+def test_pz_bitwise_vs_reference_build(oracle_lib):
+    if not oracle_lib.ref_xc_available():
+        pytest.skip("oracle/_ref/libxc_ref.so not built (needs the reference sources and flang)")
+    rng = np.random.default_rng(21)
+    rho = np.concatenate([10 ** rng.uniform(-9, 1.5, 4000), [0.0, 1e-14, 3 / (4 * np.pi), 0.2387, 0.2388]])
+    v, e = oracle_lib.xc_pz(rho)
+    vr, er = oracle_lib.ref_xc_pz(rho)
+    assert np.array_equal(v, vr) and np.array_equal(e, er)
+    ru = 10 ** rng.uniform(-9, 1.5, 4000)
+    rd = 10 ** rng.uniform(-9, 1.5, 4000)
+    rd[:500] = 0.0          # fully polarised
+    ru[500:1000] = 0.0      # fully polarised the other way
+    rd[1000:1500] = ru[1000:1500]  # unpolarised
+    rs = np.stack([np.append(ru, [0.0, 3 / (4 * np.pi)]), np.append(rd, [0.0, 0.0])])
+    v, e = oracle_lib.xc_pz_sp(rs)
+    vr, er = oracle_lib.ref_xc_pz_sp(rs)
+    assert np.all(np.isfinite(vr))
+    assert np.array_equal(v, vr) and np.array_equal(e, er)
+
+
+@pytest.mark.parametrize("rs", [0.4, 0.9, 1.3, 4.0])
+@pytest.mark.parametrize("zeta", [0.0, 0.35, -0.6, 0.9])
+def test_pz_sp_potential_is_energy_derivative(oracle_lib, rs, zeta):
+    rho = float(_rho_of_rs(rs))
+    ru, rd = 0.5 * rho * (1 + zeta), 0.5 * rho * (1 - zeta)
+    h = 1e-5 * rho
+    pts = [(ru, rd), (ru - h, rd), (ru + h, rd), (ru, rd - h), (ru, rd + h)]
+    r = np.array(pts).T.copy()
+    v, e = oracle_lib.xc_pz_sp(r)
+    tot = r.sum(axis=0) + 1e-20
+    f = tot * e
+    dup = (f[2] - f[1]) / (2 * h)
+    ddn = (f[4] - f[3]) / (2 * h)
+    assert abs(v[0, 0] - dup) < 1e-8 * abs(v[0, 0]), (v[0, 0], dup)
+    assert abs(v[1, 0] - ddn) < 1e-8 * abs(v[1, 0]), (v[1, 0], ddn)
+    # the two channels' potentials differ by the spin term whenever zeta != 0
+    assert (abs(v[0, 0] - v[1, 0]) > 1e-4) == (zeta != 0.0)
+
+
+def test_pz_sp_limits(oracle_lib):
+    rs = np.array([0.5, 1.0, 2.0, 5.0])
+    rho = _rho_of_rs(rs)
+    # zeta = 0: the unpolarised functional, up to the density offsets of the two forms (1e-10
+    # and 1e-20): a relative change of at most 1e-10/rho in rs
+    v0, e0 = oracle_lib.xc_pz(rho)
+    v, e = oracle_lib.xc_pz_sp(np.stack([0.5 * rho, 0.5 * rho]))
+    bound = 1e-10 / rho
+    assert np.all(np.abs(e - e0) <= bound * np.abs(e0))
+    assert np.all(np.abs(v[0] - v0) <= bound * np.abs(v0)) and np.array_equal(v[0], v[1])
+    # zeta = 1: exchange 2^(1/3) e_x and the polarised correlation of PZ81 (gamma -0.0843,
+    # beta1 1.3981, beta2 0.2611; A 0.01555, B -0.0269, C 0.0007, D -0.0048)
+    v1, e1 = oracle_lib.xc_pz_sp(np.stack([rho, np.zeros_like(rho)]))
+    ex = -0.4581652932831429 / rs * 2 ** (1 / 3)
+    ec_p = np.where(rs >= 1.0 + 1e-12, -0.0843 / (1 + 1.3981 * np.sqrt(rs) + 0.2611 * rs),
+                    0.01555 * np.log(rs) - 0.0269 + 0.0007 * rs * np.log(rs) - 0.0048 * rs)
+    assert np.abs(e1 - (ex + ec_p)).max() < 1e-12
+    # exchanging the channels exchanges the potentials, bit for bit
+    rng = np.random.default_rng(4)
+    r = rng.random((2, 64)) * 0.3
+    va, ea = oracle_lib.xc_pz_sp(r)
+    vb, eb = oracle_lib.xc_pz_sp(r[::-1].copy())
+    assert np.array_equal(va, vb[::-1]) and np.array_equal(ea, eb)
+
+
+def test_scf_spin_channels_follow_unpolarised(oracle_lib):
+    """nspin = 2 with both channels holding the nspin = 1 orbitals (one electron each): the
+    channel densities are half the nspin = 1 density, the Hartree potential sees their sum and
+    the polarised PZ at zeta = 0 equals the unpolarised one, so the predictor-corrector runs
+    agree to the scale of the PZ density offsets (1e-10 against 1e-20: a relative change of
+    1e-10/rho in the potential, ~1e-8 at this system's lowest densities)."""
+    from salmon2_amd import system
+
+    kw = dict(n=12, no=3, nkgrid=(1, 1, 1), seed=31, nt=4)
+    s1 = system.make_si_system(nspin=1, **kw)
+    s2 = system.make_si_system(nspin=2, **kw)
+    s2 = dataclasses.replace(s2, psi=np.repeat(s1.psi, 2, axis=2))
+    p1, c1, d1, r1, v1 = oracle_lib.Oracle(s1).rt_run_scf(True, nt=4)
+    p2, c2, d2, r2, v2 = oracle_lib.Oracle(s2).rt_run_scf(True, nt=4)
+    assert v2.shape == (2,) + v1.shape
+    tol = 2e-8
+    assert np.abs(v2[0] - v1).max() < tol * np.abs(v1).max() and np.array_equal(v2[0], v2[1])
+    assert np.abs(c2 - c1).max() < tol * np.abs(c1).max()
+    assert np.abs(d2 - d1).max() < tol * np.abs(d1).max()
+    assert np.abs(p2[:, :, 0] - p1[:, :, 0]).max() < tol * np.abs(p1).max()
+    assert np.array_equal(p2[:, :, 0], p2[:, :, 1])
 
 
 def _dense_density(s, psi):
