@@ -8,6 +8,14 @@
 !   update_kvector_nonlocalpt_hip   <- update_kvector_nonlocalpt (src/common/hamiltonian.f90:847-860)
 !   calc_density_hip                <- calc_density              (src/common/density_matrix.f90:26-42)
 !   calc_current_hip                <- calc_current              (src/common/density_matrix.f90:240-300)
+!   hartree_hip                     <- hartree                   (src/poisson/hartree.f90:24; periodic:
+!                                                                 poisson_ft, poisson_periodic.f90:23-242)
+!   exchange_correlation_hip        <- exchange_correlation      (src/xc/salmon_xc.f90:58-343), xc = 'PZ'
+!   update_vlocal_hip               <- update_vlocal             (src/common/hamiltonian.f90:813-843)
+!
+! The three potential wrappers take and return SALMON's host s_scalar arrays (a few
+! hundred KiB per call); update_vlocal_hip also leaves the new V_local on the device
+! as the handle's potential, so the next taylor_hip / hpsi_hip uploads nothing.
 !
 ! Residency (the reference's OpenACC path: `!$acc enter data copyin(spsi_in,
 ! spsi_out,tpsi)` in src/rt/initialization_rt.f90:234-240, then `present`
@@ -36,6 +44,7 @@ module salmon_hip_dropin
   public :: salmon_hip_enter_data, salmon_hip_exit_data, salmon_hip_update_host, salmon_hip_update_device
   public :: salmon_hip_dropin_sync
   public :: taylor_hip, hpsi_hip, update_kvector_nonlocalpt_hip, calc_density_hip, calc_current_hip
+  public :: hartree_hip, exchange_correlation_hip, update_vlocal_hip
 
   type(c_ptr), save :: h = c_null_ptr
   type(c_ptr), save :: d_a = c_null_ptr, d_b = c_null_ptr, d_w = c_null_ptr
@@ -49,6 +58,12 @@ module salmon_hip_dropin
   ! the last V_local uploaded (uploads are skipped while it is unchanged)
   real(8), allocatable, save :: v_last(:,:,:,:)
   logical, save :: have_v = .false.
+  ! potential wrappers: device buffers (first use) and the cell of the last set_poisson
+  type(c_ptr), save :: d_vh = c_null_ptr, d_vxc = c_null_ptr, d_exc = c_null_ptr, d_vpsl = c_null_ptr
+  type(c_ptr), save :: d_vout = c_null_ptr
+  real(8), save :: al_last(3) = 0d0
+  ! salmon_xctype_pz / salmon_xctype_none (src/xc/salmon_xc.f90:42-43)
+  integer, parameter :: xctype_none = 0, xctype_pz = 1
 
 contains
 
@@ -105,6 +120,15 @@ contains
     d_a = c_null_ptr; d_b = c_null_ptr; d_w = c_null_ptr
     call salmon_hip_check(salmon_hip_free(d_rho), 'free')
     call salmon_hip_check(salmon_hip_free(d_cur), 'free')
+    if (c_associated(d_vh)) then
+      call salmon_hip_check(salmon_hip_free(d_vh), 'free')
+      call salmon_hip_check(salmon_hip_free(d_vxc), 'free')
+      call salmon_hip_check(salmon_hip_free(d_exc), 'free')
+      call salmon_hip_check(salmon_hip_free(d_vpsl), 'free')
+      call salmon_hip_check(salmon_hip_free(d_vout), 'free')
+    end if
+    d_vh = c_null_ptr; d_vxc = c_null_ptr; d_exc = c_null_ptr; d_vpsl = c_null_ptr; d_vout = c_null_ptr
+    al_last = 0d0
     call salmon_hip_check(salmon_hip_destroy(h), 'destroy')
     if (allocated(v_last)) deallocate(v_last)
     h = c_null_ptr
@@ -346,5 +370,154 @@ contains
     call salmon_hip_check(salmon_hip_memcpy_dtoh(h, c, d_cur, int(size(c), c_size_t) * 8_c_size_t), 'download curr')
     curr(:, :, info%im_s) = c
   end subroutine calc_current_hip
+
+  subroutine pot_buffers()
+    integer(c_size_t) :: g
+    if (.not. c_associated(h)) stop 'salmon_hip: salmon_hip_dropin_setup not called'
+    if (c_associated(d_vh)) return
+    g = int(nx_s, c_size_t) * int(ny_s, c_size_t) * int(nz_s, c_size_t) * 8_c_size_t
+    call salmon_hip_check(salmon_hip_malloc(d_vh, g), 'malloc')
+    call salmon_hip_check(salmon_hip_malloc(d_vxc, g * nspin_s), 'malloc')
+    call salmon_hip_check(salmon_hip_malloc(d_exc, g), 'malloc')
+    call salmon_hip_check(salmon_hip_malloc(d_vpsl, g), 'malloc')
+    call salmon_hip_check(salmon_hip_malloc(d_vout, g * nspin_s), 'malloc')
+  end subroutine pot_buffers
+
+  ! f(is:ie) of one s_scalar -> the device array at d (one channel, nx*ny*nz doubles)
+  subroutine scalar_up(mg, x, d)
+    type(s_rgrid), intent(in) :: mg
+    type(s_scalar), intent(in) :: x
+    type(c_ptr), intent(in) :: d
+    real(8), allocatable :: t(:,:,:)
+    allocate(t(nx_s, ny_s, nz_s))
+    t = x%f(mg%is(1):mg%ie(1), mg%is(2):mg%ie(2), mg%is(3):mg%ie(3))
+    call salmon_hip_check(salmon_hip_memcpy_htod(h, d, t, int(size(t), c_size_t) * 8_c_size_t), 'upload')
+  end subroutine scalar_up
+
+  subroutine scalar_down(mg, d, x)
+    type(s_rgrid), intent(in) :: mg
+    type(c_ptr), intent(in) :: d
+    type(s_scalar), intent(inout) :: x
+    real(8), allocatable :: t(:,:,:)
+    allocate(t(nx_s, ny_s, nz_s))
+    call salmon_hip_check(salmon_hip_memcpy_dtoh(h, t, d, int(size(t), c_size_t) * 8_c_size_t), 'download')
+    x%f(mg%is(1):mg%ie(1), mg%is(2):mg%ie(2), mg%is(3):mg%ie(3)) = t
+  end subroutine scalar_down
+
+  ! address of channel `is` (1-based) of a device array of nspin channels
+  type(c_ptr) function channel(d, is)
+    type(c_ptr), intent(in) :: d
+    integer, intent(in) :: is
+    channel = transfer(transfer(d, 0_c_intptr_t) + int(is - 1, c_intptr_t) * int(nx_s, c_intptr_t) &
+                       * int(ny_s, c_intptr_t) * int(nz_s, c_intptr_t) * 8_c_intptr_t, c_null_ptr)
+  end function channel
+
+  ! Vh of the total density rho (periodic orthogonal cell: the FFT Poisson solver of
+  ! poisson_ft, with the cell lengths from system%primitive_a)
+  subroutine hartree_hip(lg, mg, info, system, fg, poisson, srg_scalar, stencil, rho, Vh)
+    type(s_rgrid), intent(in) :: lg, mg
+    type(s_parallel_info), intent(in) :: info
+    type(s_dft_system), intent(in) :: system
+    type(s_reciprocal_grid), intent(in) :: fg
+    type(s_poisson), intent(inout) :: poisson
+    type(s_sendrecv_grid), intent(inout) :: srg_scalar
+    type(s_stencil), intent(in) :: stencil
+    type(s_scalar), intent(in) :: rho
+    type(s_scalar), intent(inout) :: Vh
+    real(8) :: al(3)
+    integer :: i, j
+    call pot_buffers()
+    if (any(lg%num /= mg%num)) stop 'salmon_hip: r-space division is not supported'
+    do j = 1, 3
+      do i = 1, 3
+        if (i /= j .and. system%primitive_a(i, j) /= 0d0) stop 'salmon_hip: non-orthogonal cell in hartree_hip'
+      end do
+      al(j) = system%primitive_a(j, j)
+    end do
+    if (any(al /= al_last)) then
+      call salmon_hip_check(salmon_hip_set_poisson(h, al), 'set_poisson')
+      al_last = al
+    end if
+    call scalar_up(mg, rho, d_rho)
+    call salmon_hip_check(salmon_hip_hartree(h, d_rho, d_vh), 'hartree')
+    call scalar_down(mg, d_vh, Vh)
+  end subroutine hartree_hip
+
+  ! Vxc(ispin) and E_xc of PZ81 (nspin = 1) or its spin-polarised form (nspin = 2) from
+  ! the channel densities rho_s; E_xc = hvol sum_r e_xc trho in grid order
+  ! (salmon_xc.f90:234-253; trho = rho, or rho_1 + rho_2 + 1e-20 as builtin_pz_sp.f90:35)
+  subroutine exchange_correlation_hip(system, xc_func, mg, srg_scalar, srg, rho_s, ppn, info, spsi, stencil, &
+                                      Vxc, E_xc)
+    type(s_dft_system), intent(in) :: system
+    type(s_xc_functional), intent(in) :: xc_func
+    type(s_rgrid), intent(in) :: mg
+    type(s_sendrecv_grid), intent(inout) :: srg_scalar, srg
+    type(s_scalar), intent(in) :: rho_s(system%nspin)
+    type(s_pp_nlcc), intent(in) :: ppn
+    type(s_parallel_info), intent(in) :: info
+    type(s_orbital), intent(in) :: spsi
+    type(s_stencil), intent(in) :: stencil
+    type(s_scalar), intent(inout) :: Vxc(system%nspin)
+    real(8), intent(out) :: E_xc
+    real(8), allocatable :: e(:,:,:), r(:,:,:,:)
+    real(8) :: tot, trho
+    integer :: is, ix, iy, iz
+    call pot_buffers()
+    if (xc_func%xctype(1) /= xctype_pz .or. xc_func%xctype(2) /= xctype_none .or. &
+        xc_func%xctype(3) /= xctype_none) stop 'salmon_hip: exchange_correlation_hip supports xc = PZ only'
+    if (xc_func%ispin /= system%nspin - 1) stop 'salmon_hip: xc_func%ispin does not match nspin'
+    if (allocated(ppn%rho_nlcc)) then
+      if (any(ppn%rho_nlcc /= 0d0)) stop 'salmon_hip: nonlinear core correction is not supported'
+    end if
+    do is = 1, nspin_s
+      call scalar_up(mg, rho_s(is), channel(d_rho, is))
+    end do
+    call salmon_hip_check(salmon_hip_xc_pz(h, d_rho, d_vxc, d_exc), 'xc_pz')
+    do is = 1, nspin_s
+      call scalar_down(mg, channel(d_vxc, is), Vxc(is))
+    end do
+    allocate(e(nx_s, ny_s, nz_s), r(nx_s, ny_s, nz_s, nspin_s))
+    call salmon_hip_check(salmon_hip_memcpy_dtoh(h, e, d_exc, int(size(e), c_size_t) * 8_c_size_t), 'download')
+    do is = 1, nspin_s
+      r(:,:,:,is) = rho_s(is)%f(mg%is(1):mg%ie(1), mg%is(2):mg%ie(2), mg%is(3):mg%ie(3))
+    end do
+    tot = 0d0
+    do iz = 1, nz_s
+    do iy = 1, ny_s
+    do ix = 1, nx_s
+      if (nspin_s == 2) then
+        trho = r(ix, iy, iz, 1) + r(ix, iy, iz, 2) + 1d-20
+      else
+        trho = 2 * (r(ix, iy, iz, 1) * 0.5d0)
+      end if
+      tot = tot + e(ix, iy, iz) * trho
+    end do
+    end do
+    end do
+    E_xc = tot * system%hvol
+  end subroutine exchange_correlation_hip
+
+  ! V_local(is) = Vpsl + Vh + Vxc(is) on the device; the result becomes the handle's
+  ! potential (no upload at the next taylor_hip / hpsi_hip) and is written to Vlocal
+  subroutine update_vlocal_hip(mg, nspin, Vh, Vpsl, Vxc, Vlocal)
+    type(s_rgrid), intent(in) :: mg
+    integer, intent(in) :: nspin
+    type(s_scalar), intent(in) :: Vh, Vpsl, Vxc(nspin)
+    type(s_scalar) :: Vlocal(nspin)
+    integer :: is
+    call pot_buffers()
+    if (nspin /= nspin_s) stop 'salmon_hip: nspin does not match the handle'
+    call scalar_up(mg, Vh, d_vh)
+    call scalar_up(mg, Vpsl, d_vpsl)
+    do is = 1, nspin
+      call scalar_up(mg, Vxc(is), channel(d_vxc, is))
+    end do
+    call salmon_hip_check(salmon_hip_update_vlocal(h, d_vpsl, d_vh, d_vxc, c_null_ptr, d_vout), 'update_vlocal')
+    do is = 1, nspin
+      call scalar_down(mg, channel(d_vout, is), Vlocal(is))
+      v_last(:,:,:,is) = Vlocal(is)%f(mg%is(1):mg%ie(1), mg%is(2):mg%ie(2), mg%is(3):mg%ie(3))
+    end do
+    have_v = .true.
+  end subroutine update_vlocal_hip
 
 end module salmon_hip_dropin

@@ -104,6 +104,32 @@ module salmon_hip_mod
       integer(c_int) :: rc
     end function
 
+    function salmon_hip_set_poisson(h, al) bind(c, name='salmon_hip_set_poisson') result(rc)
+      import
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: al(3)
+      integer(c_int) :: rc
+    end function
+
+    function salmon_hip_hartree(h, rho_dev, vh_dev) bind(c, name='salmon_hip_hartree') result(rc)
+      import
+      type(c_ptr), value :: h, rho_dev, vh_dev
+      integer(c_int) :: rc
+    end function
+
+    function salmon_hip_xc_pz(h, rho_dev, vxc_dev, exc_dev) bind(c, name='salmon_hip_xc_pz') result(rc)
+      import
+      type(c_ptr), value :: h, rho_dev, vxc_dev, exc_dev
+      integer(c_int) :: rc
+    end function
+
+    function salmon_hip_update_vlocal(h, vpsl_dev, vh_dev, vxc_dev, vmix_dev, vout_dev) &
+        bind(c, name='salmon_hip_update_vlocal') result(rc)
+      import
+      type(c_ptr), value :: h, vpsl_dev, vh_dev, vxc_dev, vmix_dev, vout_dev
+      integer(c_int) :: rc
+    end function
+
     function salmon_hip_synchronize(h) bind(c, name='salmon_hip_synchronize') result(rc)
       import
       type(c_ptr), value :: h

@@ -5,8 +5,10 @@
 ! Only the derived types and components that the propagation hot path reads
 ! are declared, with SALMON's names and array shapes (src/common/structures.f90:
 ! s_scalar :36-38, s_dft_system :45-66, s_rgrid :86-94, s_parallel_info :127-146,
-! s_orbital :191-196, s_stencil :198-203, s_pp_grid :254-319, s_rt :472-492), so
-! the same wrapper source compiles unchanged against SALMON's full module.
+! s_orbital :191-196, s_stencil :198-203, s_pp_grid :254-319, s_pp_nlcc :321-324,
+! s_xc_functional :327-344, s_reciprocal_grid :346-353, s_poisson :356-381,
+! s_rt :472-492), so the same wrapper source compiles unchanged against SALMON's
+! full module.
 ! Inside SALMON this file is not used.
 !
 module structures
@@ -19,7 +21,7 @@ module structures
   type s_dft_system
     logical :: if_real_orbital = .false.
     integer :: ngrid, nspin, no, nk, nion
-    real(8) :: hvol, hgs(3)
+    real(8) :: hvol, hgs(3), primitive_a(3,3)
     real(8), allocatable :: vec_k(:,:)    ! (3, nk)
     real(8), allocatable :: wtk(:)        ! (nk)
     real(8), allocatable :: rocc(:,:,:)   ! (no, nk, nspin)
@@ -65,6 +67,23 @@ module structures
     real(8), allocatable :: rinv_uvu(:)
     complex(8), allocatable :: zekr_uv(:,:,:)
   end type s_pp_grid
+
+  type s_pp_nlcc
+    real(8), allocatable :: rho_nlcc(:,:,:)
+  end type s_pp_nlcc
+
+  type s_xc_functional
+    integer :: xctype(3)
+    integer :: ispin
+  end type s_xc_functional
+
+  type s_reciprocal_grid
+    real(8), allocatable :: coef(:,:,:)
+  end type s_reciprocal_grid
+
+  type s_poisson
+    integer :: iterVh
+  end type s_poisson
 
   type s_rt
     complex(8), allocatable :: zc(:)

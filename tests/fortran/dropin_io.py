@@ -14,7 +14,8 @@ SRC = os.path.join(HERE, "test_dropin.f90")
 
 def write_input(path, s, vec_ac, psi=None, nt=0, nwarm=0):
     """System s (salmon2_amd.system.System; orbitals `psi` or s.psi), the field vec_ac of the
-    single-call checks, and for the RT loop nt steps (nwarm untimed) with s's pulse table."""
+    single-call checks, for the RT loop nt steps (nwarm untimed) with s's pulse table, and the
+    cell lengths (system%primitive_a of an orthogonal cell)."""
     zc = s.zc()
     psi = s.psi if psi is None else psi
     ac = s.pulse.table(s.dt, nt) if (s.pulse is not None and nt > 0) else np.zeros((nt + 2, 3))
@@ -35,6 +36,7 @@ def write_input(path, s, vec_ac, psi=None, nt=0, nwarm=0):
         np.ascontiguousarray(psi, np.complex128).tofile(f)
         np.array([nt, nwarm], dtype=np.int32).tofile(f)
         np.ascontiguousarray(ac, np.float64).tofile(f)
+        np.ascontiguousarray(s.al, np.float64).tofile(f)
 
 
 class _Reader:
@@ -71,3 +73,16 @@ def read_rt(path, s, nt):
     psi = r.take(n, np.complex128).reshape(s.nk, s.no, s.nspin, s.nz, s.ly, s.nx)
     assert r.done()
     return dict(elapsed=el, curr=jt, psi=psi)
+
+
+def read_scf(path, s):
+    r = _Reader(path)
+    g = s.nx * s.ny * s.nz
+    shp3 = (s.nz, s.ny, s.nx)
+    out = dict(vh=r.take(g, np.float64).reshape(shp3),
+               vxc=r.take(g * s.nspin, np.float64).reshape((s.nspin,) + shp3),
+               exc_total=float(r.take(1, np.float64)[0]),
+               vlocal=r.take(g * s.nspin, np.float64).reshape((s.nspin,) + shp3),
+               tout=r.take(s.psi.size, np.complex128).reshape(s.psi.shape))
+    assert r.done()
+    return out

@@ -17,6 +17,12 @@
 !       calc_density_hip, vec_Ac = Ac(itt), calc_current_hip.  The steps after the
 !       warm-up are timed between two device synchronisations; writes the elapsed
 !       seconds, J(t) of every step and the final orbitals.
+!   test_dropin scf <input> <output>
+!       the self-consistent potential update of time_evolution_step (yn_fix_func = 'n',
+!       src/rt/time_evolution_step.f90:160-207) through the wrappers: calc_density_hip,
+!       rho = rho_s(1) [+ rho_s(2)], hartree_hip, exchange_correlation_hip (PZ),
+!       update_vlocal_hip with Vpsl = the input V_local(1), then taylor_hip with the new
+!       V_local; writes Vh, Vxc, E_xc, V_local and psi(t+dt).
 !
 program test_dropin
   use iso_c_binding
@@ -27,7 +33,15 @@ program test_dropin
   type(s_dft_system) :: system
   type(s_parallel_info) :: info
   type(s_stencil) :: stencil
-  type(s_sendrecv_grid) :: srg
+  type(s_sendrecv_grid) :: srg, srg_scalar
+  type(s_rgrid) :: lg
+  type(s_reciprocal_grid) :: fg
+  type(s_poisson) :: poisson
+  type(s_xc_functional) :: xc_func
+  type(s_pp_nlcc) :: ppn
+  type(s_scalar) :: Vh, Vpsl, rhot
+  type(s_scalar), allocatable :: Vxc(:)
+  real(8) :: al(3), E_xc
   type(s_pp_grid) :: ppg
   type(s_rt) :: rt
   type(s_orbital), target :: psi, hpsi_out, tin, tout, work
@@ -84,7 +98,12 @@ program test_dropin
   read(u) nt, nwarm
   allocate(ac(3, 0:nt + 1))
   read(u) ac
+  read(u) al
   close(u)
+  system%primitive_a = 0d0
+  do is = 1, 3
+    system%primitive_a(is, is) = al(is)
+  end do
   allocate(tin%zwf, source=psi%zwf)
   allocate(tout%zwf, mold=psi%zwf)
   allocate(work%zwf, mold=psi%zwf)
@@ -170,8 +189,37 @@ program test_dropin
       write(u) tin%zwf
     end if
     close(u)
+  else if (trim(mode) == 'scf') then
+    lg = mg
+    xc_func%xctype = [1, 0, 0]  ! salmon_xctype_pz, none, none
+    xc_func%ispin = nspin - 1
+    allocate(Vh%f(nx, ny, nz), Vpsl%f(nx, ny, nz), rhot%f(nx, ny, nz), Vxc(nspin))
+    do is = 1, nspin
+      allocate(Vxc(is)%f(nx, ny, nz))
+    end do
+    Vpsl%f = V_local(1)%f
+    call update_kvector_nonlocalpt_hip(info%ik_s, info%ik_e, system, ppg)
+    call calc_density_hip(system, rho, psi, info, mg)
+    rhot%f = rho(1, 1)%f
+    if (nspin == 2) rhot%f = rho(1, 1)%f + rho(2, 1)%f
+    call hartree_hip(lg, mg, info, system, fg, poisson, srg_scalar, stencil, rhot, Vh)
+    call exchange_correlation_hip(system, xc_func, mg, srg_scalar, srg, rho(:, 1), ppn, info, psi, stencil, &
+                                  Vxc, E_xc)
+    call update_vlocal_hip(mg, nspin, Vh, Vpsl, Vxc, V_local)
+    call taylor_hip(mg, system, info, stencil, srg, tin, tout, work, ppg, V_local, rt)
+    open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write(u) Vh%f
+    do is = 1, nspin
+      write(u) Vxc(is)%f
+    end do
+    write(u) E_xc
+    do is = 1, nspin
+      write(u) V_local(is)%f
+    end do
+    write(u) tout%zwf
+    close(u)
   else
-    stop 'usage: test_dropin check|rt <input> <output>'
+    stop 'usage: test_dropin check|rt|scf <input> <output>'
   end if
   call salmon_hip_dropin_release()
 end program test_dropin

@@ -7,8 +7,11 @@ its test driver tests/fortran/test_dropin.f90:
   bitwise equal to the host-array one;
 * rt: the fixed-potential RT loop driven from Fortran with spsi_in, spsi_out and tpsi resident
   -- J(t) against the oracle's run (1e-9) and bitwise equal to the Python-hosted run of the
-  same library.
+  same library;
+* scf: hartree_hip, exchange_correlation_hip (PZ, nspin = 1 and 2) and update_vlocal_hip on
+  SALMON-type host arrays, then taylor_hip with the new V_local, against the oracle (1e-12).
 """
+import dataclasses
 import os
 import subprocess
 
@@ -85,3 +88,37 @@ def test_fortran_resident_rt_loop(oracle_lib, tmp_path):
           f"{np.abs(d['curr'] - py['curr']).max():.2e}", flush=True)
     assert e <= 1e-9
     assert np.array_equal(d["curr"], py["curr"])
+
+
+@pytest.mark.parametrize("name", ["c1", "spin2"])
+def test_fortran_potential_wrappers(oracle_lib, tmp_path, name):
+    s = CHECK_CASES[name]()
+    A = np.array([0.006, 0.013, -0.009])
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    io.write_input(fin, s, A)
+    print(f"[gpu] fortran drop-in potential wrappers case={name}", flush=True)
+    r = subprocess.run([_exe(), "scf", str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = io.read_scf(fout, s)
+    O = oracle_lib.Oracle(s)
+    rho = O.density(s.psi)
+    rhot = rho[0] + rho[1] if s.nspin == 2 else rho[0]
+    vh = oracle_lib.hartree(rhot, s.al)
+    if s.nspin == 2:
+        vxc, exc = oracle_lib.xc_pz_sp(rho)
+        trho = rho[0] + rho[1] + 1e-20
+    else:
+        v1, exc = oracle_lib.xc_pz(rho[0])
+        vxc = v1[None]
+        trho = rho[0]
+    e_xc = float((exc * trho).sum() * s.hvol)
+    vloc = s.vlocal[0][None] + vh[None] + vxc
+    O2 = oracle_lib.Oracle(dataclasses.replace(s, vlocal=np.ascontiguousarray(vloc)))
+    O2.update_kvector(A)
+    tout, _, _ = O2.taylor(A, s.psi)
+    ny = s.ny
+    errs = dict(vh=rel(d["vh"], vh), vxc=rel(d["vxc"], vxc), e_xc=abs(d["exc_total"] - e_xc) / abs(e_xc),
+                vlocal=rel(d["vlocal"], vloc), taylor=rel(d["tout"][..., :ny, :], tout[..., :ny, :]))
+    print(f"[parity] fortran potential {name}: " + " ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-12, k
