@@ -278,6 +278,15 @@ int salmon_hip_update_vlocal(salmon_hip_t h, const double *vpsl_dev, const doubl
                              const double *vxc_dev, const double *vmix_dev, double *vout_dev);
 
 /*
+ * The 'aetrs' propagator's potential extrapolation between its two half steps
+ * (time_evolution_half_step_etrs, src/rt/time_evolution_step.f90:432-445) on the handle's
+ * local potential: V_new = 3 V - 3 V_old1 + V_old2, V_old2 = V_old1, V_old1 = V, V = V_new.
+ * vold1_dev, vold2_dev (nx,ny,nz,nspin) DEVICE arrays, updated in place (start both at the
+ * initial V_local, initialization_rt.f90:290-291); V_new also to vout_dev if not null.
+ */
+int salmon_hip_vlocal_extrapolate(salmon_hip_t h, double *vold1_dev, double *vold2_dev, double *vout_dev);
+
+/*
  * hartree + exchange_correlation (PZ) + update_vlocal in one call from the density
  * rho_s(nx,ny,nz,nspin) as calc_density returns it, as time_evolution_step.f90:160-207
  * does every step with yn_fix_func = 'n': Vh of the total density (rho_s(1) + rho_s(2)

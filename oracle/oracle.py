@@ -79,6 +79,9 @@ def lib():
         L.orc_rt_run_full.argtypes = [S, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int,
                                       _dp, _dp, _dp, _dp, _dp]
         L.orc_rt_run_full.restype = C.c_int
+        L.orc_rt_run_prop.argtypes = (L.orc_rt_run_full.argtypes[:4] + [C.c_int]
+                                      + L.orc_rt_run_full.argtypes[4:])
+        L.orc_rt_run_prop.restype = C.c_int
         _lib = L
     return _lib
 
@@ -231,8 +234,9 @@ class Oracle:
         return p, cur, dip, rne, vloc
 
 
-    def rt_run_full(self, scf=False, pc=False, trans_longi="tr", nt=None, psi=None):
-        """General loop; returns final psi, J(t), dipole(t), rNe(t), Ac_tot (nt+2, 3), final V_local."""
+    def rt_run_full(self, scf=False, pc=False, trans_longi="tr", nt=None, psi=None, propagator="middlepoint"):
+        """General loop ('middlepoint' or 'aetrs'); returns final psi, J(t), dipole(t), rNe(t),
+        Ac_tot (nt+2, 3), final V_local."""
         s = self.s
         nt = s.nt if nt is None else nt
         ac = np.ascontiguousarray(s.pulse.table(s.dt, nt) if s.pulse else np.zeros((nt + 2, 3)))
@@ -245,11 +249,13 @@ class Oracle:
         rne = np.zeros(nt)
         if trans_longi not in ("tr", "lo"):
             raise ValueError(trans_longi)
+        if propagator not in ("middlepoint", "aetrs"):
+            raise ValueError(propagator)
         saved = self.o.vlocal
         try:
-            rc = lib().orc_rt_run_full(self.ref, int(bool(scf)), int(bool(pc)), int(trans_longi == "lo"), _p(vpsl),
-                                       _p(vloc), _p(al), s.dt, s.n_hamil, nt, _p(ac), _p(p.view(np.float64)),
-                                       _p(cur), _p(dip), _p(rne))
+            rc = lib().orc_rt_run_prop(self.ref, int(bool(scf)), int(bool(pc)), int(trans_longi == "lo"),
+                                       int(propagator == "aetrs"), _p(vpsl), _p(vloc), _p(al), s.dt, s.n_hamil, nt,
+                                       _p(ac), _p(p.view(np.float64)), _p(cur), _p(dip), _p(rne))
         finally:
             self.o.vlocal = saved
         if rc != 0:

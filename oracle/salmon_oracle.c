@@ -810,8 +810,24 @@ static void vlocal_from_rho(const orc_system *s, const double *vpsl, const doubl
 int orc_rt_run_full(orc_system *s, int scf, int pc, int lo, const double *vpsl, double *vloc, const double al[3],
                     double dt, int n_hamil, int nt, double *ac, double *psi, double *curr_t, double *dip_t,
                     double *rne_t) {
+  return orc_rt_run_prop(s, scf, pc, lo, 0, vpsl, vloc, al, dt, n_hamil, nt, ac, psi, curr_t, dip_t, rne_t);
+}
+
+/*
+ * The same loop with the propagator chosen: aetrs = 0 'middlepoint' as above; aetrs = 1
+ * 'aetrs' (time_evolution_step.f90:119-123, 140-141, 425-475): zc of dt/2
+ * (initialization_rt.f90:492-508); per step vec_Ac = Ac_tot(itt), a Taylor half step with
+ * V_local(t), then V_local extrapolated, V_new = 3 V - 3 V_old1 + V_old2 with the history
+ * shifted (V_old2 = V_old1, V_old1 = V; both start at the initial V_local,
+ * initialization_rt.f90:290-291), vec_Ac = Ac_ext(itt+1) + 2 Ac_ind(itt) - Ac_ind(itt-1), and
+ * the second half step; the predictor-corrector is not used with aetrs.  With a fixed
+ * potential the extrapolation is the identity (3V - 3V + V = V exactly) and is skipped.
+ */
+int orc_rt_run_prop(orc_system *s, int scf, int pc, int lo, int aetrs, const double *vpsl, double *vloc,
+                    const double al[3], double dt, int n_hamil, int nt, double *ac, double *psi, double *curr_t,
+                    double *dip_t, double *rne_t) {
   const double pi = 3.141592653589793238462643383279502884;
-  pc = pc && scf;
+  pc = pc && scf && !aetrs;
   const size_t tot = C2(orb_size(s) * (size_t)s->nspin * s->no * s->nk);
   const size_t g = grid_size(s);
   double *a = psi;
@@ -823,17 +839,43 @@ int orc_rt_run_full(orc_system *s, int scf, int pc, int lo, const double *vpsl, 
   double *v_tmp = (double *)malloc(sizeof(double) * gv);
   double *zc = (double *)malloc(sizeof(double) * 2 * n_hamil);
   double *ac_ind = (double *)calloc(3 * (size_t)(nt + 2), sizeof(double));
-  if (!b || !w || !rho || !v_tmp || !zc || !ac_ind || (pc && !psi_tmp)) return -1;
-  orc_taylor_coefs(dt, n_hamil, 0, zc);
+  double *vold = (aetrs && scf) ? (double *)malloc(sizeof(double) * 2 * gv) : NULL;
+  if (!b || !w || !rho || !v_tmp || !zc || !ac_ind || (pc && !psi_tmp) || (aetrs && scf && !vold)) return -1;
+  orc_taylor_coefs(dt, n_hamil, aetrs, zc);
   if (scf) {
     s->vlocal = vloc;
     orc_density(s, a, rho);
     vlocal_from_rho(s, vpsl, al, rho, vloc);
+    if (vold) {
+      memcpy(vold, vloc, sizeof(double) * gv);
+      memcpy(vold + gv, vloc, sizeof(double) * gv);
+    }
   }
   for (int itt = 1; itt <= nt; ++itt) {
     double A[3];
-    for (int d = 0; d < 3; ++d) A[d] = 0.5 * (ac[3 * itt + d] + ac[3 * (itt - 1) + d]);
+    for (int d = 0; d < 3; ++d)
+      A[d] = aetrs ? ac[3 * itt + d] : 0.5 * (ac[3 * itt + d] + ac[3 * (itt - 1) + d]);
     if (s->natom > 0) orc_update_kvector(s, A);
+    if (aetrs) {
+      orc_taylor(s, A, zc, n_hamil, a, b, w);
+      if (vold) {
+        double *v1 = vold, *v2 = vold + gv;
+        for (size_t i = 0; i < gv; ++i) {
+          const double vn = 3.0 * vloc[i] - 3.0 * v1[i] + v2[i];
+          v2[i] = v1[i];
+          v1[i] = vloc[i];
+          vloc[i] = vn;
+        }
+      }
+      /* Ac_ext(itt+1) + 2 Ac_ind(itt) - Ac_ind(itt-1); ac[itt+1] holds Ac_ext(itt+1) until the
+         field update at the end of this step */
+      for (int d = 0; d < 3; ++d)
+        A[d] = ac[3 * (itt + 1) + d] + 2.0 * ac_ind[3 * itt + d] - ac_ind[3 * (itt - 1) + d];
+      if (s->natom > 0) orc_update_kvector(s, A);
+      double *t = a;
+      a = b;
+      b = t;
+    }
     if (pc) {
       memcpy(v_tmp, vloc, sizeof(double) * gv);
       memcpy(psi_tmp, a, sizeof(double) * tot);
@@ -875,7 +917,7 @@ int orc_rt_run_full(orc_system *s, int scf, int pc, int lo, const double *vpsl, 
   } else {
     free(b);
   }
-  free(w); free(psi_tmp); free(rho); free(v_tmp); free(zc); free(ac_ind);
+  free(w); free(psi_tmp); free(rho); free(v_tmp); free(zc); free(ac_ind); free(vold);
   return 0;
 }
 

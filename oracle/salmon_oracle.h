@@ -86,6 +86,10 @@ void orc_xc_pz_sp(long n, const double *rho_s, double *vxc_s, double *exc);
 int orc_rt_run_full(orc_system *s, int scf, int pc, int lo, const double *vpsl, double *vloc, const double al[3],
                     double dt, int n_hamil, int nt, double *ac, double *psi, double *curr_t, double *dip_t,
                     double *rne_t);
+/* The same with the propagator: aetrs = 0 'middlepoint', 1 'aetrs'. */
+int orc_rt_run_prop(orc_system *s, int scf, int pc, int lo, int aetrs, const double *vpsl, double *vloc,
+                    const double al[3], double dt, int n_hamil, int nt, double *ac, double *psi, double *curr_t,
+                    double *dip_t, double *rne_t);
 int orc_rt_run_scf(orc_system *s, const double *vpsl, double *vloc, const double al[3], int pc, double dt,
                    int n_hamil, int nt, const double *ac_tot, double *psi, double *curr_t, double *dip_t,
                    double *rne_t);

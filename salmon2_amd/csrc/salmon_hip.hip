@@ -2266,6 +2266,20 @@ __global__ __launch_bounds__(256) void k_update_vlocal(VlocArgs a) {
   }
 }
 
+// The aetrs propagator's potential extrapolation (src/rt/time_evolution_step.f90:432-445):
+// V_new = 3 V - 3 V_old1 + V_old2, then V_old2 = V_old1, V_old1 = V, V = V_new, over all
+// spin channels of the handle's potential.
+__global__ __launch_bounds__(256) void k_vextrap(double *V, double *v1, double *v2, double *vout, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = V[i];
+  const double vn = 3.0 * v - 3.0 * v1[i] + v2[i];
+  v2[i] = v1[i];
+  v1[i] = v;
+  V[i] = vn;
+  if (vout) vout[i] = vn;
+}
+
 // Streaming copy of n 16-byte elements (HBM ceiling of the box): dwordx4 loads and stores,
 // non-temporal, four elements per thread in flight.
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
@@ -3676,6 +3690,17 @@ int salmon_hip_update_vlocal(salmon_hip_t c, const double *vpsl_dev, const doubl
   if (int rc = check(c)) return rc;
   if (!vpsl_dev || !vh_dev || !vxc_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
   return launch_vlocal(c, vpsl_dev, vh_dev, vxc_dev, nullptr, vmix_dev, vout_dev);
+}
+
+int salmon_hip_vlocal_extrapolate(salmon_hip_t c, double *vold1_dev, double *vold2_dev, double *vout_dev) {
+  if (int rc = check(c)) return rc;
+  if (!vold1_dev || !vold2_dev) return fail(SALMON_HIP_EINVAL, "null pointer");
+  if (!c->have_V) return fail(SALMON_HIP_ESTATE, "V_local not set");
+  const size_t n = c->ngrid * (size_t)c->nspin;
+  hipLaunchKernelGGL(k_vextrap, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->d_V, vold1_dev,
+                     vold2_dev, vout_dev, n);
+  LAUNCHCHK("k_vextrap");
+  return 0;
 }
 
 int salmon_hip_update_potential(salmon_hip_t c, const double *rho_dev, const double *vpsl_dev,

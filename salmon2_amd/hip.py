@@ -56,6 +56,7 @@ EXPORTS = (
     "salmon_hip_xc_pz",
     "salmon_hip_update_vlocal",
     "salmon_hip_update_potential",
+    "salmon_hip_vlocal_extrapolate",
     "salmon_hip_synchronize",
     "salmon_hip_prof_start",
     "salmon_hip_prof_stop",
@@ -128,6 +129,7 @@ def load():
     L.salmon_hip_xc_pz.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.salmon_hip_update_vlocal.argtypes = [C.c_void_p] + [C.c_void_p] * 5
     L.salmon_hip_update_potential.argtypes = [C.c_void_p] + [C.c_void_p] * 4
+    L.salmon_hip_vlocal_extrapolate.argtypes = [C.c_void_p] * 4
     L.salmon_hip_synchronize.argtypes = [C.c_void_p]
     L.salmon_hip_prof_start.argtypes = [C.c_void_p, C.c_int]
     L.salmon_hip_prof_stop.argtypes = [C.c_void_p, C.c_int, _dp, _ip]
@@ -313,6 +315,12 @@ class Handle:
         opt = [C.c_void_p(_ptr(x) if x is not None else None) for x in (vmix, vout)]
         _chk(load().salmon_hip_update_vlocal(self.h, C.c_void_p(_ptr(vpsl)), C.c_void_p(_ptr(vh)),
                                              C.c_void_p(_ptr(vxc)), *opt), "update_vlocal")
+
+    def vlocal_extrapolate(self, vold1, vold2, vout=None):
+        """aetrs: V = 3 V - 3 vold1 + vold2 on the handle's potential, history shifted in place."""
+        _chk(load().salmon_hip_vlocal_extrapolate(self.h, C.c_void_p(_ptr(vold1)), C.c_void_p(_ptr(vold2)),
+                                                  C.c_void_p(_ptr(vout) if vout is not None else None)),
+             "vlocal_extrapolate")
 
     def update_potential(self, rho, vpsl, vmix=None, vout=None):
         opt = [C.c_void_p(_ptr(x) if x is not None else None) for x in (vmix, vout)]

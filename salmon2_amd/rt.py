@@ -40,9 +40,18 @@ keeps Ac_tot = Ac_ext; ``"lo"`` adds the induced field Ac_ind(t+dt) =
 step).  ``write_rt_data`` writes SYSname_rt.data as write_rt_data_3d does
 (src/io/write.f90:686-790, atomic units).
 
+``propagator="aetrs"`` follows time_evolution_half_step_etrs
+(src/rt/time_evolution_step.f90:119-123, 140-141, 425-475): Taylor coefficients of
+dt/2; a half step with V(t) and A = Ac_tot(itt), the potential extrapolated to
+3 V - 3 V(t-dt) + V(t-2dt) on the device (salmon_hip_vlocal_extrapolate; the
+identity with a fixed potential, so skipped there), A = Ac_ext(itt+1) +
+2 Ac_ind(itt) - Ac_ind(itt-1), the second half step.  As in the reference the
+predictor-corrector is not used with it.
+
 Orbitals stay resident in HBM for the whole run; per-step observables are
-written to device arrays and read back once at the end.  torch is used only
-for device allocation and the stream.
+written to device arrays and read back once at the end.  torch is used for
+device allocation, the streams and events, and the elementwise V_mid - V of the
+corrector shortcut.
 """
 from __future__ import annotations
 
@@ -56,7 +65,7 @@ from .hip import Handle
 
 class RTPropagator:
     def __init__(self, sysm, device="cuda:0", psi=None, scf=False, pc=False, precision="fp64", trans_longi="tr",
-                 learned=None, record=False, fuse_current=True):
+                 learned=None, record=False, fuse_current=True, propagator="middlepoint"):
         self.s = sysm
         self.dev = torch.device(device)
         torch.cuda.set_device(self.dev)
@@ -73,13 +82,18 @@ class RTPropagator:
         self.psi_b = torch.zeros_like(self.psi_a)
         self.work = torch.zeros_like(self.psi_a)
         self.rho = torch.zeros((sysm.nspin, sysm.nz, sysm.ny, sysm.nx), dtype=torch.float64, device=self.dev)
-        self.zc = sysm.zc()
+        if propagator not in ("middlepoint", "aetrs"):
+            raise ValueError(f"propagator {propagator!r}: 'middlepoint' or 'aetrs'")
+        self.aetrs = propagator == "aetrs"
+        self.zc = sysm.zc(aetrs=self.aetrs)
         self.itt = 0
         if trans_longi not in ("tr", "lo"):
             raise ValueError(f"trans_longi {trans_longi!r}: 'tr' or 'lo'")
         self.trans_longi = trans_longi
         self.scf = bool(scf)
-        self.pc = bool(pc) and self.scf  # with a fixed potential the corrector average is the identity
+        # with a fixed potential the corrector average is the identity; aetrs takes precedence over the
+        # predictor-corrector (time_evolution_step.f90:140-144)
+        self.pc = bool(pc) and self.scf and not self.aetrs
         if self.scf:
             # Vpsl is the spin-independent part (system.vlocal[0]); V_local(ispin) = Vpsl + Vh[rho_1 + rho_2]
             # + Vxc(ispin)[rho_1, rho_2] with nspin = 2 (spin-polarised PZ)
@@ -91,11 +105,15 @@ class RTPropagator:
             self.h.update_potential(self.rho, self.vpsl, vout=self.vcur)
             self.vmid = torch.zeros_like(self.vcur)  # the predictor-corrector midpoint potential of the last step
             self.dv = torch.zeros_like(self.vcur)  # V_mid - V of the predictor (corrector from the predictor's H psi)
+            if self.aetrs:  # V(t-dt), V(t-2dt) of the extrapolation, both the initial V at the start
+                self.vold1 = self.vcur.clone()
+                self.vold2 = self.vcur.clone()
         # learned predictor (salmon2_amd.learned.Predictor): needs the self-consistent predictor-corrector step
         self.learned = learned
         if learned is not None:
             if not self.pc:
-                raise ValueError("the learned predictor replaces the predictor of the predictor-corrector (scf=pc=True)")
+                raise ValueError("the learned predictor replaces the predictor of the predictor-corrector "
+                                 "(scf=pc=True, propagator='middlepoint')")
             if sysm.nspin != 1:
                 raise ValueError("the learned predictor takes one potential channel (nspin = 1)")
             self.hist = [torch.zeros_like(self.vpsl) for _ in range(4)]  # V(t), V(t-dt), V(t-2dt), V(t-3dt)
@@ -119,7 +137,7 @@ class RTPropagator:
         self.record = [] if record else None
         # the current's projection of the new orbitals also forms the next step's first projection
         # (salmon_hip_current_next) when the next field is known in advance ('tr')
-        self.fuse_current = bool(fuse_current) and sysm.natom > 0
+        self.fuse_current = bool(fuse_current) and sysm.natom > 0 and not self.aetrs
         self._vpred_rec = None
         # the current of a predictor-corrector step runs on a second stream while the density and
         # the potential update run on the first (independent reads of the new orbitals); joined at
@@ -145,12 +163,19 @@ class RTPropagator:
         """One RT step itt -> itt+1 (time_evolution_step)."""
         self.itt += 1
         itt = self.itt
-        A = 0.5 * (self.ac[itt] + self.ac[itt - 1])
+        A = self.ac[itt] if self.aetrs else 0.5 * (self.ac[itt] + self.ac[itt - 1])
         self.h.update_kvector(A)
         if self.learned is not None and self.nhist >= 4:
             self._learned_step()
         else:
-            if self.pc:
+            if self.aetrs:
+                self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
+                if self.scf:
+                    self.h.vlocal_extrapolate(self.vold1, self.vold2, vout=self.vcur)
+                self.h.update_kvector(self.ac_ext[itt + 1] + 2.0 * self.ac_ind[itt] - self.ac_ind[itt - 1])
+                self.psi_a, self.psi_b = self.psi_b, self.psi_a
+                self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
+            elif self.pc:
                 self.h.taylor(self.zc, self.psi_a, self.psi_b, self.work)
                 self.h.density(self.psi_b, self.rho)
                 self.h.update_potential(self.rho, self.vpsl, vmix=self.vcur, vout=self.vmid)

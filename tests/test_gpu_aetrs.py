@@ -1,0 +1,90 @@
+"""The 'aetrs' propagator on the GPU (time_evolution_half_step_etrs,
+src/rt/time_evolution_step.f90:119-123, 140-141, 425-475) against the oracle's loop:
+two Taylor half steps with A = Ac_tot(itt) and then Ac(itt+1), the potential extrapolated
+3 V - 3 V(t-dt) + V(t-2dt) between them on the device.  Fixed and self-consistent
+potentials, nspin = 1 and 2, the 'lo' field; series <= 1e-9, final orbitals and V_local
+<= 1e-10.  Also the extrapolation entry point alone against numpy (<= 1e-15).
+"""
+import numpy as np
+import pytest
+
+from salmon2_amd import system
+
+pytestmark = pytest.mark.gpu
+
+SERIES_TOL = 1e-9
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
+    from salmon2_amd import hip
+
+    hip.load()
+    return torch
+
+
+CASES = {
+    "tiny": lambda: system.config("tiny"),
+    "c1": lambda: system.config("c1"),
+    "si64_32cube": lambda: system.make_si_system(ncell=(2, 2, 2), n=32, no=4, nkgrid=(1, 1, 1), seed=11),
+    "spin2_16x14x12": lambda: system.make_si_system(n=(16, 14, 12), no=3, nkgrid=(1, 1, 2), nspin=2, seed=19),
+}
+
+
+@pytest.mark.parametrize("name,scf,lo,nt", [("c1", False, False, 30), ("c1", True, False, 30), ("tiny", True, True, 20),
+                                            ("si64_32cube", True, False, 4), ("spin2_16x14x12", True, False, 10)])
+def test_rt_aetrs_run(torch_dev, oracle_lib, name, scf, lo, nt):
+    torch = torch_dev
+    s = CASES[name]()
+    tl = "lo" if lo else "tr"
+    print(f"[gpu] rt aetrs case={name} scf={scf} trans_longi={tl} steps={nt}", flush=True)
+    from salmon2_amd.rt import RTPropagator
+
+    rt = RTPropagator(s, scf=scf, trans_longi=tl, propagator="aetrs")
+    res = rt.run(nt)
+    vfin = rt.vcur.cpu().numpy() if scf else None
+    psi = res["psi"].cpu().numpy()
+    rt.close()
+    O = oracle_lib.Oracle(s)
+    psi_ref, cur, dip, rne, ac, v_ref = O.rt_run_full(scf=scf, trans_longi=tl, nt=nt, propagator="aetrs")
+    errs = dict(current=rel(res["curr"], cur), dipole=rel(res["dip"], dip), rNe=rel(res["rne"], rne),
+                ac_tot=rel(res["ac_tot"], ac[1 : nt + 1]))
+    for k, v in errs.items():
+        print(f"[parity] aetrs {name}{'-scf' if scf else ''}-{tl}-{nt} {k}: rel err {v:.3e}", flush=True)
+        assert v <= SERIES_TOL, k
+    e_psi = rel(psi[..., : s.ny, :], psi_ref[..., : s.ny, :])
+    e_v = rel(vfin, v_ref) if scf else 0.0
+    print(f"[parity] aetrs {name} final psi {e_psi:.3e} V_local {e_v:.3e}", flush=True)
+    assert e_psi <= 1e-10 and e_v <= 1e-10
+    # a different propagator than the middle point: the trajectories part
+    _, cur_mp, *_ = O.rt_run_full(scf=scf, trans_longi=tl, nt=nt)
+    assert np.abs(cur - cur_mp).max() > 1e-9 * np.abs(cur).max()
+    torch.cuda.empty_cache()
+
+
+def test_vlocal_extrapolate(torch_dev):
+    torch = torch_dev
+    s = CASES["spin2_16x14x12"]()
+    from salmon2_amd.hip import Handle
+
+    rng = np.random.default_rng(12)
+    shape = (2, s.nz, s.ny, s.nx)
+    v, v1, v2 = (rng.standard_normal(shape) for _ in range(3))
+    h = Handle(s, stream=torch.cuda.current_stream())
+    h.set_vlocal(v)
+    d1, d2 = torch.from_numpy(v1).cuda(), torch.from_numpy(v2).cuda()
+    out = torch.zeros_like(d1)
+    h.vlocal_extrapolate(d1, d2, vout=out)
+    torch.cuda.synchronize()
+    ref = 3.0 * v - 3.0 * v1 + v2
+    print(f"[parity] vlocal_extrapolate: rel err {rel(out.cpu().numpy(), ref):.3e}", flush=True)
+    assert rel(out.cpu().numpy(), ref) <= 1e-15
+    assert np.array_equal(d1.cpu().numpy(), v) and np.array_equal(d2.cpu().numpy(), v1)
+    h.close()

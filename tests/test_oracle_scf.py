@@ -14,7 +14,7 @@
   channels' exchange exchanging the potentials exactly;
 * SCF loop: every step rebuilt from dense-matrix Taylor polynomials, numpy
   densities and the two pinned functionals, with and without the
-  predictor-corrector; electron number never grows; nspin = 2 with both channels
+  predictor-corrector, and with the 'aetrs' propagator (two half steps, V extrapolated); electron number never grows; nspin = 2 with both channels
   holding the nspin = 1 orbitals follows the nspin = 1 run.
 """
 import dataclasses
@@ -229,6 +229,66 @@ def test_scf_loop_dense(oracle_lib, tiny, pc):
     # the truncated Taylor series contracts the high-frequency part of random orbitals
     # (|1 - ix - x^2/2 + i x^3/6 + x^4/24| < 1 for 0 < x < 2 sqrt 2), so rNe never grows
     assert np.all(np.diff(rne) <= 1e-12 * rne[0])
+
+
+@pytest.mark.parametrize("scf", [False, True])
+def test_aetrs_loop_dense(oracle_lib, tiny, scf):
+    """The 'aetrs' propagator (time_evolution_step.f90:119-123, 425-475) rebuilt step by step from
+    dense Taylor polynomials: a half step (dt/2) with V(t) and A = Ac_tot(itt), V extrapolated
+    3V - 3V(t-dt) + V(t-2dt) (history starting at the initial V), a half step with it and
+    A = Ac(itt+1); the current with Ac_tot(itt)."""
+    s = tiny
+    nt = 3
+    p_or, cur, dip, rne, ac_or, vloc = oracle_lib.Oracle(s).rt_run_full(scf=scf, nt=nt, propagator="aetrs")
+    vpsl = s.vlocal[0]
+    ac = s.pulse.table(s.dt, nt)
+    s0 = dataclasses.replace(s, vlocal=np.zeros_like(s.vlocal))
+
+    def potential(rho):
+        return vpsl + oracle_lib.hartree(rho, s.al) + oracle_lib.xc_pz(rho)[0]
+
+    def ham(A):
+        return [dense.full_hamiltonian(s0, s.vec_k[ik] + A) for ik in range(s.nk)]
+
+    psi = np.array(s.psi)
+    V = potential(_dense_density(s, psi)) if scf else vpsl
+    v1 = v2 = V
+    for itt in range(1, nt + 1):
+        psi = _dense_step(s, ham(ac[itt]), V, psi, s.dt / 2, s.n_hamil)
+        if scf:
+            V, v1, v2 = 3.0 * V - 3.0 * v1 + v2, V, v1
+        psi = _dense_step(s, ham(ac[itt + 1]), V, psi, s.dt / 2, s.n_hamil)
+        rho = _dense_density(s, psi)
+        if scf:
+            V = potential(rho)
+        assert abs(rne[itt - 1] - rho.sum() * s.hvol) < 1e-12 * rne[itt - 1]
+    assert np.abs(p_or - psi).max() < 1e-12 * np.abs(psi).max()
+    if scf:
+        assert np.abs(vloc - V).max() < 1e-12 * np.abs(V).max()
+    # a different propagator: the trajectories part at the scale of the field's step
+    _, cur_mp, _, _, _, _ = oracle_lib.Oracle(s).rt_run_full(scf=scf, nt=nt)
+    assert np.abs(cur - cur_mp).max() > 1e-9 * np.abs(cur).max()
+
+
+def test_aetrs_fixed_field_free_is_two_half_steps(oracle_lib, tiny):
+    """Without a field and with a fixed potential an aetrs step is two Taylor half steps of the same
+    H: its distance from exp(-iH dt) is 2 (E dt/2)^5/5! against (E dt)^5/5! for one full Taylor
+    step, 2^4 times smaller (a half step of the wrong length fails this)."""
+    import scipy.linalg
+
+    s = dataclasses.replace(tiny, pulse=None)
+    O = oracle_lib.Oracle(s)
+    p_ae, *_ = O.rt_run_full(nt=1, propagator="aetrs")
+    p_mp, *_ = O.rt_run_full(nt=1)
+    err_ae = err_mp = 0.0
+    for ik in range(s.nk):
+        H = dense.full_hamiltonian(s, s.vec_k[ik])
+        U = scipy.linalg.expm(-1j * s.dt * H)
+        for io in range(s.no):
+            ex = U @ dense.interior(s.psi[ik, io, 0], s)
+            err_ae = max(err_ae, np.abs(dense.interior(p_ae[ik, io, 0], s) - ex).max())
+            err_mp = max(err_mp, np.abs(dense.interior(p_mp[ik, io, 0], s) - ex).max())
+    assert 12 < err_mp / err_ae < 20, (err_ae, err_mp)
 
 
 def test_scf_pc_differs_from_fixed_potential(oracle_lib, tiny):
