@@ -130,6 +130,44 @@ module salmon_hip_mod
       integer(c_int) :: rc
     end function
 
+    function salmon_hip_update_potential(h, rho_dev, vpsl_dev, vmix_dev, vout_dev) &
+        bind(c, name='salmon_hip_update_potential') result(rc)
+      import
+      type(c_ptr), value :: h, rho_dev, vpsl_dev, vmix_dev, vout_dev
+      integer(c_int) :: rc
+    end function
+
+    function salmon_hip_vlocal_extrapolate(h, vold1_dev, vold2_dev, vout_dev) &
+        bind(c, name='salmon_hip_vlocal_extrapolate') result(rc)
+      import
+      type(c_ptr), value :: h, vold1_dev, vold2_dev, vout_dev
+      integer(c_int) :: rc
+    end function
+
+    function salmon_hip_set_vlocal_device(h, vlocal_dev) bind(c, name='salmon_hip_set_vlocal_device') result(rc)
+      import
+      type(c_ptr), value :: h, vlocal_dev
+      integer(c_int) :: rc
+    end function
+
+    function salmon_hip_taylor_reuse(h, zc, n_hamil, tspsi_in, tspsi_out, sshtpsi, dv_dev) &
+        bind(c, name='salmon_hip_taylor_reuse') result(rc)
+      import
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(in) :: zc(*)
+      integer(c_int), value :: n_hamil
+      type(c_ptr), value :: tspsi_in, tspsi_out, sshtpsi, dv_dev
+      integer(c_int) :: rc
+    end function
+
+    ! mode: SALMON_HIP_PREC_FP64 = 0, SALMON_HIP_PREC_MIXED = 1
+    function salmon_hip_set_precision(h, mode) bind(c, name='salmon_hip_set_precision') result(rc)
+      import
+      type(c_ptr), value :: h
+      integer(c_int), value :: mode
+      integer(c_int) :: rc
+    end function
+
     function salmon_hip_synchronize(h) bind(c, name='salmon_hip_synchronize') result(rc)
       import
       type(c_ptr), value :: h
