@@ -159,6 +159,22 @@ def _ptr(t) -> int:
     return t.data_ptr()
 
 
+def _need(t, nbytes, what, f64=True):
+    """A torch tensor handed to the library as a device array must be a contiguous GPU tensor of
+    at least nbytes (float64 where the library reads or writes doubles); raw int addresses are
+    the caller's responsibility.  Raises ValueError instead of letting a kernel run past it."""
+    if t is None or isinstance(t, int):
+        return
+    import torch
+
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{what}: a contiguous GPU tensor is required")
+    if f64 and t.dtype != torch.float64:
+        raise ValueError(f"{what}: float64 expected, got {t.dtype}")
+    if t.numel() * t.element_size() < nbytes:
+        raise ValueError(f"{what}: {t.numel() * t.element_size()} bytes, at least {nbytes} needed")
+
+
 class Handle:
     """One grid + orbital set on the GPU (wraps salmon_hip_create .. destroy)."""
 
@@ -176,6 +192,10 @@ class Handle:
                                  _p(nab), float(sysm.coef_lap0), _p(veck), _p(rocc), _p(wtk), _p(hgs),
                                  float(sysm.hvol), C.byref(h)), "salmon_hip_create")
         self.h = h
+        # bytes of one grid channel (double), of V_local / rho (nspin channels), of an orbital set
+        self._g = 8 * sysm.nx * sysm.ny * sysm.nz
+        self._gs = self._g * sysm.nspin
+        self._orb = 16 * sysm.nk * sysm.no * sysm.nspin * sysm.nz * sysm.ly * sysm.nx
         if stream is not None:
             self.set_stream(stream)
         if sysm.natom > 0:
@@ -209,9 +229,12 @@ class Handle:
 
     def set_vlocal(self, V):
         v = np.ascontiguousarray(V, dtype=np.float64)
+        if v.nbytes != self._gs:
+            raise ValueError(f"set_vlocal: V_local(nx,ny,nz,nspin) of {self._gs // 8} values expected, got {v.size}")
         _chk(load().salmon_hip_set_vlocal(self.h, _p(v)), "set_vlocal")
 
     def set_vlocal_device(self, Vdev):
+        _need(Vdev, self._gs, "set_vlocal_device")
         _chk(load().salmon_hip_set_vlocal_device(self.h, C.c_void_p(_ptr(Vdev))), "set_vlocal_device")
 
     def update_kvector(self, vec_ac):
@@ -223,9 +246,13 @@ class Handle:
         _chk(load().salmon_hip_set_vec_ac(self.h, _p(a)), "set_vec_ac")
 
     def hpsi(self, tpsi, htpsi):
+        _need(tpsi, self._orb, "hpsi tpsi", f64=False)
+        _need(htpsi, self._orb, "hpsi htpsi", f64=False)
         _chk(load().salmon_hip_hpsi(self.h, C.c_void_p(_ptr(tpsi)), C.c_void_p(_ptr(htpsi))), "hpsi")
 
     def taylor(self, zc, tin, tout, work):
+        for t, w in ((tin, "tin"), (tout, "tout"), (work, "work")):
+            _need(t, self._orb, "taylor " + w, f64=False)
         z = np.ascontiguousarray(np.asarray(zc, dtype=np.complex128)).view(np.float64)
         _chk(load().salmon_hip_taylor(self.h, _p(z), len(z) // 2, C.c_void_p(_ptr(tin)), C.c_void_p(_ptr(tout)),
                                       C.c_void_p(_ptr(work))), "taylor")
@@ -233,6 +260,9 @@ class Handle:
     def taylor_reuse(self, zc, tin, tout, work, dv):
         """The Taylor step of tin when only V_local changed since the last taylor call on tin:
         H psi is updated by dv * psi (dv = V_now - V_then, device array) instead of recomputed."""
+        for t, w in ((tin, "tin"), (tout, "tout"), (work, "work")):
+            _need(t, self._orb, "taylor_reuse " + w, f64=False)
+        _need(dv, self._gs, "taylor_reuse dv")
         z = np.ascontiguousarray(np.asarray(zc, dtype=np.complex128)).view(np.float64)
         _chk(load().salmon_hip_taylor_reuse(self.h, _p(z), len(z) // 2, C.c_void_p(_ptr(tin)), C.c_void_p(_ptr(tout)),
                                             C.c_void_p(_ptr(work)), C.c_void_p(_ptr(dv))), "taylor_reuse")
@@ -278,6 +308,8 @@ class Handle:
         _chk(load().salmon_hip_set_precision(self.h, self.PREC[mode]), "set_precision")
 
     def density(self, psi, rho):
+        _need(psi, self._orb, "density psi", f64=False)
+        _need(rho, self._gs, "density rho")
         _chk(load().salmon_hip_density(self.h, C.c_void_p(_ptr(psi)), C.c_void_p(_ptr(rho))), "density")
 
     def density_noalloc(self, psi, rho):
@@ -286,6 +318,8 @@ class Handle:
              "density_noalloc")
 
     def current(self, vec_ac, psi, curr):
+        _need(psi, self._orb, "current psi", f64=False)
+        _need(curr, 24 * self.s.nspin, "current curr")
         a = np.ascontiguousarray(vec_ac, dtype=np.float64)
         _chk(load().salmon_hip_current(self.h, _p(a), C.c_void_p(_ptr(psi)), C.c_void_p(_ptr(curr))), "current")
 
@@ -298,6 +332,8 @@ class Handle:
              "current_next")
 
     def dipole(self, rho, out):
+        _need(rho, self._gs, "dipole rho")
+        _need(out, 32, "dipole out")
         _chk(load().salmon_hip_dipole(self.h, C.c_void_p(_ptr(rho)), C.c_void_p(_ptr(out))), "dipole")
 
     def set_poisson(self, al):
@@ -305,24 +341,37 @@ class Handle:
         _chk(load().salmon_hip_set_poisson(self.h, _p(a)), "set_poisson")
 
     def hartree(self, rho, vh):
+        _need(rho, self._g, "hartree rho")
+        _need(vh, self._g, "hartree vh")
         _chk(load().salmon_hip_hartree(self.h, C.c_void_p(_ptr(rho)), C.c_void_p(_ptr(vh))), "hartree")
 
     def xc_pz(self, rho, vxc, exc=None):
+        _need(rho, self._gs, "xc_pz rho_s")
+        _need(vxc, self._gs, "xc_pz vxc")
+        _need(exc, self._g, "xc_pz exc")
         _chk(load().salmon_hip_xc_pz(self.h, C.c_void_p(_ptr(rho)), C.c_void_p(_ptr(vxc)),
                                      C.c_void_p(_ptr(exc) if exc is not None else None)), "xc_pz")
 
     def update_vlocal(self, vpsl, vh, vxc, vmix=None, vout=None):
+        for t, w, n in ((vpsl, "vpsl", self._g), (vh, "vh", self._g), (vxc, "vxc", self._gs), (vmix, "vmix", self._gs),
+                        (vout, "vout", self._gs)):
+            _need(t, n, "update_vlocal " + w)
         opt = [C.c_void_p(_ptr(x) if x is not None else None) for x in (vmix, vout)]
         _chk(load().salmon_hip_update_vlocal(self.h, C.c_void_p(_ptr(vpsl)), C.c_void_p(_ptr(vh)),
                                              C.c_void_p(_ptr(vxc)), *opt), "update_vlocal")
 
     def vlocal_extrapolate(self, vold1, vold2, vout=None):
         """aetrs: V = 3 V - 3 vold1 + vold2 on the handle's potential, history shifted in place."""
+        for t, w in ((vold1, "vold1"), (vold2, "vold2"), (vout, "vout")):
+            _need(t, self._gs, "vlocal_extrapolate " + w)
         _chk(load().salmon_hip_vlocal_extrapolate(self.h, C.c_void_p(_ptr(vold1)), C.c_void_p(_ptr(vold2)),
                                                   C.c_void_p(_ptr(vout) if vout is not None else None)),
              "vlocal_extrapolate")
 
     def update_potential(self, rho, vpsl, vmix=None, vout=None):
+        for t, w, n in ((rho, "rho_s", self._gs), (vpsl, "vpsl", self._g), (vmix, "vmix", self._gs),
+                        (vout, "vout", self._gs)):
+            _need(t, n, "update_potential " + w)
         opt = [C.c_void_p(_ptr(x) if x is not None else None) for x in (vmix, vout)]
         _chk(load().salmon_hip_update_potential(self.h, C.c_void_p(_ptr(rho)), C.c_void_p(_ptr(vpsl)), *opt),
              "update_potential")

@@ -41,3 +41,19 @@ def test_library_exports_every_declared_symbol():
 
 def test_last_error_is_empty_string_initially():
     assert hip.load().salmon_hip_last_error() in (b"",)
+
+
+def test_device_array_checks():
+    """The Python handle refuses tensors the library would run past (host tensors, wrong dtype,
+    too few bytes) before any launch; raw int addresses pass through."""
+    import pytest
+    import torch
+
+    from salmon2_amd.hip import _need
+
+    _need(None, 8, "x")
+    _need(12345, 8, "x")
+    with pytest.raises(ValueError, match="GPU tensor"):
+        _need(torch.zeros(4, dtype=torch.float64), 8, "x")
+    with pytest.raises(ValueError, match="GPU tensor"):
+        _need(torch.zeros((4, 4), dtype=torch.float64).t(), 8, "x")
