@@ -307,13 +307,14 @@ class RTPropagator:
 
     def results(self):
         torch.cuda.synchronize(self.dev)
-        curr = self.curr_t.sum(dim=1).cpu().numpy()
+        curr_s = self.curr_t.cpu().numpy()  # (nt, nspin, 3): the current of each spin channel
+        curr = curr_s.sum(axis=1) if self.s.nspin == 2 else curr_s[:, 0]
         dip = self.dip_t.cpu().numpy()
         dt = self.s.dt
         n = len(curr)
         e_ext = -(self.ac_ext[1 : n + 1] - self.ac_ext[0:n]) / dt
         e_tot = -(self.ac[1 : n + 1] - self.ac[0:n]) / dt
-        return dict(psi=self.psi_a, curr=curr, dip=dip[:, :3], rne=dip[:, 3], ac_ext=self.ac_ext[1 : n + 1],
+        return dict(psi=self.psi_a, curr=curr, curr_s=curr_s, dip=dip[:, :3], rne=dip[:, 3], ac_ext=self.ac_ext[1 : n + 1],
                     ac_tot=self.ac[1 : n + 1], e_ext=e_ext, e_tot=e_tot)
 
     def checkpoint(self, directory, sliced_block=0):
@@ -362,7 +363,8 @@ class RTPropagator:
         return p
 
     def write_rt_data(self, path, res=None):
-        """SYSname_rt.data of the steps run so far (write_rt_data_3d, unpolarised, atomic units)."""
+        """SYSname_rt.data of the steps run so far (write_rt_data_3d, atomic units; spin = 'polarized'
+        columns for nspin = 2)."""
         res = self.results() if res is None else res
         write_rt_data(path, self.s.dt, res)
 
@@ -383,16 +385,27 @@ def _fe(x: float, d: int = 15, e: int = 3) -> str:
 
 def write_rt_data(path, dt, res):
     """Header and rows of SYSname_rt.data (src/io/write.f90:700-790): time, Ac_ext, E_ext,
-    Ac_tot, E_tot, Jm, columns 1-16, format (F16.8, 99(1X, E23.15E3))."""
+    Ac_tot, E_tot, then Jm (columns 14-16, spin = 'unpolarized') or, when res["curr_s"] holds two
+    spin channels, Jm_u and Jm_d (columns 14-19, spin = 'polarized'); format
+    (F16.8, 99(1X, E23.15E3))."""
+    cs = res.get("curr_s")
+    polarized = cs is not None and np.asarray(cs).shape[1] == 2
     lines = ["# Real time calculation: ", "# Ac_ext: External vector potential field",
              "# E_ext: External electric field", "# Ac_tot: Total vector potential field",
-             "# E_tot: Total electric field", "# Jm: Matter current density (electrons)"]
+             "# E_tot: Total electric field"]
     names = ["Time", "Ac_ext_x", "Ac_ext_y", "Ac_ext_z", "E_ext_x", "E_ext_y", "E_ext_z", "Ac_tot_x", "Ac_tot_y",
-             "Ac_tot_z", "E_tot_x", "E_tot_y", "E_tot_z", "Jm_x", "Jm_y", "Jm_z"]
+             "Ac_tot_z", "E_tot_x", "E_tot_y", "E_tot_z"]
+    if polarized:
+        lines += ["# Jm_u: Matter current density for spin-up electrons",
+                  "# Jm_d: Matter current density for spin-down electrons"]
+        names += ["Jm_u_x", "Jm_u_y", "Jm_u_z", "Jm_d_x", "Jm_d_y", "Jm_d_z"]
+    else:
+        lines.append("# Jm: Matter current density (electrons)")
+        names += ["Jm_x", "Jm_y", "Jm_z"]
     lines.append("#" + "".join(f" {i + 1}:{nm}[a.u.]" for i, nm in enumerate(names)))
     for i in range(len(res["curr"])):
-        vals = list(res["ac_ext"][i]) + list(res["e_ext"][i]) + list(res["ac_tot"][i]) + list(res["e_tot"][i]) + \
-            list(res["curr"][i])
+        jm = list(cs[i][0]) + list(cs[i][1]) if polarized else list(res["curr"][i])
+        vals = list(res["ac_ext"][i]) + list(res["e_ext"][i]) + list(res["ac_tot"][i]) + list(res["e_tot"][i]) + jm
         lines.append(f"{(i + 1) * dt:16.8f}" + "".join(" " + _fe(float(v)) for v in vals))
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")

@@ -153,6 +153,10 @@ def test_rt_scf_run(torch_dev, oracle_lib, name, pc, nt):
     for k, v in errs.items():
         print(f"[parity] {name}-scf{'-pc' if pc else ''}-{nt} {k}: rel err {v:.3e}", flush=True)
         assert v <= SERIES_TOL, k
+    if s.nspin == 2:
+        # per-channel currents (SYSname_rt.data's Jm_u / Jm_d columns) add up to the total
+        assert res["curr_s"].shape == (nt, 2, 3) and np.array_equal(res["curr_s"].sum(axis=1), res["curr"])
+        assert np.abs(res["curr_s"][:, 0] - res["curr_s"][:, 1]).max() > 1e-3 * np.abs(res["curr"]).max()
     e_psi = rel(res["psi"].cpu().numpy()[..., : s.ny, :], psi_ref[..., : s.ny, :])
     e_v = rel(vfin, v_ref)
     print(f"[parity] {name} final psi {e_psi:.3e} V_local {e_v:.3e}", flush=True)

@@ -35,3 +35,30 @@ def test_rt_data_round_trip(tmp_path):
         assert np.all(np.abs(got - ref) <= 5e-15 * np.abs(ref))
     # the reference's verification reads column 16 (index 15) as Jm_z
     assert float(lines[7 + nt - 1].split()[15]) == float(_fe(res["curr"][-1][2]))
+
+
+def test_rt_data_polarized(tmp_path):
+    """nspin = 2: the Jm_u / Jm_d header lines and columns 14-19 of spin = 'polarized'
+    (write.f90:713-719, 744-751, 775-778)."""
+    rng = np.random.default_rng(3)
+    nt, dt = 5, 0.08
+    res = {k: rng.standard_normal((nt, 3)) * 1e-3 for k in ("ac_ext", "e_ext", "ac_tot", "e_tot")}
+    res["curr_s"] = rng.standard_normal((nt, 2, 3)) * 1e-4
+    res["curr"] = res["curr_s"].sum(axis=1)
+    p = tmp_path / "Si_rt.data"
+    write_rt_data(p, dt, res)
+    lines = p.read_text().splitlines()
+    assert lines[5] == "# Jm_u: Matter current density for spin-up electrons"
+    assert lines[6] == "# Jm_d: Matter current density for spin-down electrons"
+    assert lines[7].endswith("14:Jm_u_x[a.u.] 15:Jm_u_y[a.u.] 16:Jm_u_z[a.u.] 17:Jm_d_x[a.u.] 18:Jm_d_y[a.u.] "
+                             "19:Jm_d_z[a.u.]")
+    assert len(lines) == 8 + nt
+    for i, ln in enumerate(lines[8:]):
+        f = [float(v) for v in ln.split()]
+        assert len(f) == 19
+        ref = np.concatenate([res["curr_s"][i, 0], res["curr_s"][i, 1]])
+        assert np.all(np.abs(np.array(f[13:]) - ref) <= 5e-15 * np.abs(ref))
+    # one channel: the unpolarised layout
+    res1 = dict(res, curr_s=res["curr_s"][:, :1])
+    write_rt_data(p, dt, res1)
+    assert len(p.read_text().splitlines()[7].split()) == 16  # first row after the 7 header lines
