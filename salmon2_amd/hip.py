@@ -314,6 +314,8 @@ class Handle:
 
     def density_noalloc(self, psi, rho):
         """density with its partial sums in the library's per-GPU device array (no allocation)."""
+        _need(psi, self._orb, "density_noalloc psi", f64=False)
+        _need(rho, self._gs, "density_noalloc rho")
         _chk(load().salmon_hip_density_noalloc(self.h, C.c_void_p(_ptr(psi)), C.c_void_p(_ptr(rho))),
              "density_noalloc")
 
@@ -326,6 +328,8 @@ class Handle:
     def current_next(self, vec_ac, psi, curr, vec_ac_next):
         """current of psi, and the phase table of vec_ac_next with psi's projection under it, built
         ahead for the next update_kvector(vec_ac_next) and taylor/hpsi of psi."""
+        _need(psi, self._orb, "current_next psi", f64=False)
+        _need(curr, 24 * self.s.nspin, "current_next curr")
         a = np.ascontiguousarray(vec_ac, dtype=np.float64)
         an = np.ascontiguousarray(vec_ac_next, dtype=np.float64)
         _chk(load().salmon_hip_current_next(self.h, _p(a), C.c_void_p(_ptr(psi)), C.c_void_p(_ptr(curr)), _p(an)),
